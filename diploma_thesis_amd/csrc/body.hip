@@ -1,0 +1,361 @@
+// body.hip -- rigid-body integration, particle pose / grain integration with key generation,
+// the segmented per-body reduction, and small utility kernels.
+#include "kernels.hpp"
+
+namespace arsim {
+
+namespace {
+
+// GPUintegratorKernel (Kernels/rigidBodyKernels.cu:262-348), with the orientation carried as a
+// cumulative quaternion (q_cum = normalize(q_inc * q_cum)) and I_w^-1 = R_cum I_b^-1 R_cum^T.
+// Blocks past bodyBlocks clear the cellStart entries the previous step's sorted keys set, so the
+// cell table is empty again before this step's reorder without an O(cells) fill.
+__global__ void k_body_integrate(BodyBufs b, int B, DevPhys ph, const unsigned *__restrict__ prevKeys, int N,
+                                 unsigned *__restrict__ cellStart, int bodyBlocks) {
+    if ((int)blockIdx.x >= bodyBlocks) {
+        int i = ((int)blockIdx.x - bodyBlocks) * blockDim.x + threadIdx.x;
+        if (i < N) {
+            unsigned h = prevKeys[i];
+            if (i == 0 || h != prevKeys[i - 1]) cellStart[h] = kEmpty;
+        }
+        return;
+    }
+    int bi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bi >= B) return;
+    const bool ref = (ph.flags & kRefUnits) != 0;
+    float4 X = b.pos[bi], V = b.vel[bi], W = b.ang[bi];
+    f3 x = xyz(X), v = xyz(V), w = xyz(W);
+    x = add3(x, mk3(v.x * ph.dt, v.y * ph.dt, v.z * ph.dt));
+    float w2 = dot3(w, w);
+    float theta = sqrtf(w2) * ph.dt;
+    qf qi{0.f, 0.f, 0.f, 1.f};
+    if (theta > 1e-5f) {
+        float inv = 1.0f / sqrtf(w2);
+        f3 n = mk3(w.x * inv, w.y * inv, w.z * inv);
+        float sh, ch;
+        sincos_det(theta / 2.0f, &sh, &ch);
+        qi = qf{sh * n.x, sh * n.y, sh * n.z, ch};
+    }
+    qi = qnorm(qi);
+    float4 Qc = b.qcum[bi];
+    qf qc = qnorm(qmul(qi, qf{Qc.x, Qc.y, Qc.z, Qc.w}));
+    float R[9], RT[9], Ib[9], tmp[9], Iw[9];
+    mat3_cast(qc, R);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) RT[c * 3 + r] = R[r * 3 + c];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ib[k] = b.Ib[(size_t)bi * 9 + k];
+    mat_mul(R, Ib, tmp);
+    mat_mul(tmp, RT, Iw);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) b.Iw[(size_t)bi * 9 + k] = Iw[k];
+    f3 gs = ref ? mk3(ph.gx, ph.gy, ph.gz) : mk3(ph.gx * ph.dt, ph.gy * ph.dt, ph.gz * ph.dt);
+    v = add3(v, gs);
+    w = mk3(w.x * ph.Wdamping, w.y * ph.Wdamping, w.z * ph.Wdamping);
+    b.pos[bi] = f4(x, X.w);
+    b.vel[bi] = f4(v, V.w);
+    b.ang[bi] = f4(w, W.w);
+    b.qinc[bi] = make_float4(qi.x, qi.y, qi.z, qi.w);
+    b.qcum[bi] = make_float4(qc.x, qc.y, qc.z, qc.w);
+    if (b.model) {
+        float *m = b.model + (size_t)bi * 16;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            m[c * 4 + 0] = R[c * 3 + 0];
+            m[c * 4 + 1] = R[c * 3 + 1];
+            m[c * 4 + 2] = R[c * 3 + 2];
+            m[c * 4 + 3] = 0.0f;
+        }
+        m[12] = x.x; m[13] = x.y; m[14] = x.z; m[15] = 1.0f;
+    }
+}
+
+// computeGlobalAttributes (rigidBodyKernels.cu:386-421) for body particles, integrate_functor
+// (particleKernels.cu:50-129) for grains, then the broadphase key of the updated position:
+// calcHashD (particleKernels.cu:170-189) or generateMortonCodes (BVHkernels.cu:670-716).
+template <int MODE>
+__global__ void k_pose_key(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko, int doBodies,
+                           int doGrains) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int rbi = p.rb ? p.rb[i] : -1;
+    const int s = p.scene ? (int)p.scene[i] : 0;
+    SceneGeom g;
+    load_geom(sv, s, g);
+    float4 P;
+    if (rbi >= 0) {
+        if (doBodies) {
+            float4 Qc = b.qcum[rbi];
+            float R[9];
+            mat3_cast(qf{Qc.x, Qc.y, Qc.z, Qc.w}, R);
+            f3 r = mat_vec(R, xyz(p.bodyFrame[i]));
+            p.relPos[i] = f4(r, 0.0f);
+            float4 CM = b.pos[rbi];
+            P = f4(add3(xyz(CM), r), CM.w);
+            p.pos[i] = P;
+            float4 V = b.vel[rbi], W = b.ang[rbi];
+            p.vel[i] = f4(add3(xyz(V), cross3(xyz(W), r)), V.w);
+        } else {
+            P = p.pos[i];
+        }
+    } else {
+        P = p.pos[i];
+        if (doGrains) {
+            float4 Vv = p.vel[i];
+            f3 x = xyz(P), v = xyz(Vv);
+            v = add3(v, mk3(ph.gx * ph.dt, ph.gy * ph.dt, ph.gz * ph.dt));
+            v = mk3(v.x * ph.globalDamping, v.y * ph.globalDamping, v.z * ph.globalDamping);
+            x = add3(x, mk3(v.x * ph.dt, v.y * ph.dt, v.z * ph.dt));
+            if (ph.flags & kGrainClamp) {
+                // reference box clamp: 0.01 margin added in double, then compared in float
+                float hx = (float)((double)g.boxMax[0] + 0.01), hy = (float)((double)g.boxMax[1] + 0.01),
+                      hz = (float)((double)g.boxMax[2] + 0.01);
+                float lx = (float)((double)g.boxMin[0] - 0.01), ly = (float)((double)g.boxMin[1] - 0.01),
+                      lz = (float)((double)g.boxMin[2] - 0.01);
+                if (x.x > hx - ph.r) { x.x = hx - ph.r; v.x *= ph.boundaryDamping; }
+                if (x.x < lx + ph.r) { x.x = lx + ph.r; v.x *= ph.boundaryDamping; }
+                if (x.y > hy - ph.r && v.y > 0.0f) { x.y = hy - ph.r; v.y *= ph.boundaryDamping; }
+                if (x.z > hz - ph.r) { x.z = hz - ph.r; v.z *= ph.boundaryDamping; }
+                if (x.z < lz + ph.r) { x.z = lz + ph.r; v.z *= ph.boundaryDamping; }
+                if (x.y < ly + ph.r) { x.y = ly + ph.r; v.y *= ph.boundaryDamping; }
+            }
+            P = f4(x, P.w);
+            p.pos[i] = P;
+            p.vel[i] = f4(v, Vv.w);
+        }
+    }
+    if (MODE == kKeyNone) return;
+    if (MODE == kKeyGrid) {
+        int gx, gy, gz;
+        grid_pos(xyz(P), g, gx, gy, gz);
+        ((unsigned *)ko.keys)[i] = g.cellBase + grid_hash(gx, gy, gz, g);
+    } else {
+        float lo0 = g.mLo[0], lo1 = g.mLo[1], lo2 = g.mLo[2];
+        float e0 = g.mExt[0], e1 = g.mExt[1], e2 = g.mExt[2];
+        if (ko.box) {
+            float4 mn = ko.box[0], mx = ko.box[1];
+            lo0 = mn.x; lo1 = mn.y; lo2 = mn.z;
+            e0 = mx.x - mn.x; e1 = mx.y - mn.y; e2 = mx.z - mn.z;
+        }
+        float u = mnorm(P.x, lo0, e0), v = mnorm(P.y, lo1, e1), w = mnorm(P.z, lo2, e2);
+        if (MODE == kKeyMorton30) ((unsigned *)ko.keys)[i] = morton30(u, v, w);
+        else if (MODE == kKeyMorton63) ((uint64_t *)ko.keys)[i] = morton63(u, v, w);
+        else ((uint64_t *)ko.keys)[i] = ((uint64_t)(unsigned)s << 30) | (uint64_t)morton30(u, v, w);
+    }
+    if (ko.index) ko.index[i] = (unsigned)i;
+}
+
+// Segmented per-body reduction + apply (the intended sum of UniformGridDEM.cu:421-460).
+// One wave per body: lane l sums particles l, l+64, ... of the body's contiguous range, then a
+// butterfly over lane distance 32..1; lane 0 applies v += J dt / m and ω += I_w^-1 τ dt.
+// Torque per particle is pTorque when given, else relativePos × pForce (the same expression
+// the contact kernel would have stored).
+__global__ __launch_bounds__(256) void k_reduce_apply(const float4 *__restrict__ pF, const float4 *__restrict__ pT,
+                                                      const float4 *__restrict__ rel, BodyBufs b, int B, DevPhys ph) {
+    const int body = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (body >= B) return;
+    const int st = b.pstart[body], cnt = b.pcount[body];
+    const int j0 = ((ph.flags & kDropFirst) && body > 0) ? 1 : 0;
+    f3 aF = mk3(0.f, 0.f, 0.f), aT = mk3(0.f, 0.f, 0.f);
+    for (int j = j0 + lane; j < cnt; j += 64) {
+        size_t o = (size_t)st + j;
+        f3 F = xyz(pF[o]);
+        aF = add3(aF, F);
+        f3 T = pT ? xyz(pT[o]) : cross3(xyz(rel[o]), F);
+        aT = add3(aT, T);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        aF.x = aF.x + __shfl_xor(aF.x, off, 64);
+        aF.y = aF.y + __shfl_xor(aF.y, off, 64);
+        aF.z = aF.z + __shfl_xor(aF.z, off, 64);
+        aT.x = aT.x + __shfl_xor(aT.x, off, 64);
+        aT.y = aT.y + __shfl_xor(aT.y, off, 64);
+        aT.z = aT.z + __shfl_xor(aT.z, off, 64);
+    }
+    if (lane != 0) return;
+    const float sc = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
+    const float m = b.mass[body];
+    f3 dv = mk3((aF.x * sc) / m, (aF.y * sc) / m, (aF.z * sc) / m);
+    float Iw[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Iw[k] = b.Iw[(size_t)body * 9 + k];
+    f3 dw = mat_vec(Iw, mk3(aT.x * sc, aT.y * sc, aT.z * sc));
+    float4 V = b.vel[body], W = b.ang[body];
+    V.x += dv.x; V.y += dv.y; V.z += dv.z;
+    W.x += dw.x; W.y += dw.y; W.z += dw.z;
+    b.vel[body] = V;
+    b.ang[body] = W;
+}
+
+__global__ void k_fill_u32(unsigned *p, unsigned v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+constexpr int kAabbBlocks = 512;
+
+__device__ __forceinline__ float4 min4(float4 a, float4 b) {
+    return make_float4(b.x < a.x ? b.x : a.x, b.y < a.y ? b.y : a.y, b.z < a.z ? b.z : a.z, b.w < a.w ? b.w : a.w);
+}
+__device__ __forceinline__ float4 max4(float4 a, float4 b) {
+    return make_float4(b.x > a.x ? b.x : a.x, b.y > a.y ? b.y : a.y, b.z > a.z ? b.z : a.z, b.w > a.w ? b.w : a.w);
+}
+
+// findAABBCub (BVHSource/createMortonCodes.cu:37-107): exact min / max, two stages.
+__global__ __launch_bounds__(256) void k_aabb_partial(const float4 *__restrict__ pos, int n, float4 *partials) {
+    __shared__ float4 smin[256], smax[256];
+    const float inf = __int_as_float(0x7f800000);
+    float4 mn = make_float4(inf, inf, inf, inf), mx = make_float4(-inf, -inf, -inf, -inf);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        float4 p = pos[i];
+        mn = min4(mn, p);
+        mx = max4(mx, p);
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smin[threadIdx.x] = min4(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = max4(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = smin[0];
+        partials[2 * blockIdx.x + 1] = smax[0];
+    }
+}
+__global__ __launch_bounds__(256) void k_aabb_final(const float4 *partials, int np, float4 *outMin, float4 *outMax) {
+    __shared__ float4 smin[256], smax[256];
+    const float inf = __int_as_float(0x7f800000);
+    float4 mn = make_float4(inf, inf, inf, inf), mx = make_float4(-inf, -inf, -inf, -inf);
+    for (int i = threadIdx.x; i < np; i += 256) {
+        mn = min4(mn, partials[2 * i]);
+        mx = max4(mx, partials[2 * i + 1]);
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smin[threadIdx.x] = min4(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = max4(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *outMin = smin[0];
+        *outMax = smax[0];
+    }
+}
+
+__global__ void k_check_finite(const float4 *__restrict__ a, int n, int *flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 v = a[i];
+    bool bad = !(isfinite(v.x) && isfinite(v.y) && isfinite(v.z));
+    if (bad) atomicOr(flag, 1);
+}
+
+__global__ void k_copy_offset(int *dst, const int *src, int n, int delta) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = src[i];
+    dst[i] = v >= 0 ? v + delta : v;
+}
+
+__global__ void k_fill_u16(uint16_t *p, uint16_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+inline unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace
+
+void launch_copy_offset(int *dst, const int *src, int n, int delta, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy_offset, dim3(blocks_for(n, 256)), dim3(256), 0, st, dst, src, n, delta);
+}
+
+void launch_fill_u16(uint16_t *p, uint16_t value, size_t count, hipStream_t st) {
+    if (count == 0) return;
+    unsigned blocks = (unsigned)((count + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u16, dim3(blocks), dim3(256), 0, st, p, value, count);
+}
+
+void launch_body_integrate(const BodyBufs &b, int numBodies, const DevPhys &ph, const unsigned *prevSortedKeys,
+                           int numParticles, unsigned *cellStart, hipStream_t st) {
+    const int block = 128;
+    int bodyBlocks = (int)blocks_for(numBodies, block);
+    int clearBlocks = (prevSortedKeys && cellStart) ? (int)blocks_for(numParticles, block) : 0;
+    if (bodyBlocks + clearBlocks == 0) return;
+    hipLaunchKernelGGL(k_body_integrate, dim3(bodyBlocks + clearBlocks), dim3(block), 0, st, b, numBodies, ph,
+                       prevSortedKeys, numParticles, cellStart, bodyBlocks);
+}
+
+void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph, const SceneView &sv,
+                     int mode, const KeyOut &ko, bool doBodies, bool doGrains, hipStream_t st) {
+    if (numParticles <= 0) return;
+    const int block = 256;
+    dim3 grid(blocks_for(numParticles, block));
+    int db = doBodies ? 1 : 0, dg = doGrains ? 1 : 0;
+    switch (mode) {
+        case kKeyGrid:
+            hipLaunchKernelGGL(k_pose_key<kKeyGrid>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+            break;
+        case kKeyMorton30:
+            hipLaunchKernelGGL(k_pose_key<kKeyMorton30>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+            break;
+        case kKeyMorton63:
+            hipLaunchKernelGGL(k_pose_key<kKeyMorton63>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+            break;
+        case kKeyForest30:
+            hipLaunchKernelGGL(k_pose_key<kKeyForest30>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+            break;
+        default:
+            hipLaunchKernelGGL(k_pose_key<kKeyNone>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+            break;
+    }
+}
+
+void launch_reduce_apply(const float4 *pForce, const float4 *pTorque, const float4 *relPos, const BodyBufs &b,
+                         int numBodies, const DevPhys &ph, hipStream_t st) {
+    if (numBodies <= 0) return;
+    const int block = 256;
+    dim3 grid(blocks_for((long long)numBodies * 64, block));
+    hipLaunchKernelGGL(k_reduce_apply, grid, dim3(block), 0, st, pForce, pTorque, relPos, b, numBodies, ph);
+}
+
+void launch_fill_u32(unsigned *p, unsigned value, size_t count, hipStream_t st) {
+    if (count == 0) return;
+    unsigned blocks = (unsigned)((count + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, st, p, value, count);
+}
+
+int aabb_max_partials() { return kAabbBlocks; }
+
+void launch_aabb(const float4 *pos, int n, float4 *outMin, float4 *outMax, float4 *partials, int maxPartials,
+                 hipStream_t st) {
+    int blocks = (int)blocks_for(n, 256 * 8);
+    if (blocks < 1) blocks = 1;
+    if (blocks > maxPartials) blocks = maxPartials;
+    hipLaunchKernelGGL(k_aabb_partial, dim3(blocks), dim3(256), 0, st, pos, n, partials);
+    hipLaunchKernelGGL(k_aabb_final, dim3(1), dim3(256), 0, st, partials, blocks, outMin, outMax);
+}
+
+void launch_check_finite(const float4 *a, int n, int *flag, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_check_finite, dim3(blocks_for(n, 256)), dim3(256), 0, st, a, n, flag);
+}
+
+}  // namespace arsim

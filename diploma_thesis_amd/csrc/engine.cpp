@@ -1,0 +1,724 @@
+// engine.cpp -- scene-level step engine and its extern "C" entry points.
+//
+// The step is the stage order of updateUniformGridDEM (RigidBodies/UniformGridUpdate.cpp:939-999)
+// and of the LBVH update (RigidBodies/BVHUpdate.cpp:606-759, Particles/updateFunctions.cpp:2183-2330):
+//   body integrate -> particle pose + grain integrate + broadphase key -> radix sort ->
+//   (grid) cell bounds + reorder -> 27-cell DEM          |
+//   (LBVH) Karras build -> leaf init + refit -> traversal DEM
+//   -> segmented per-body reduce + apply.
+// All buffers are allocated once when the engine is built; a step enqueues kernels only, on one
+// stream, with no host synchronisation, and can be replayed from a captured hipGraph.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "host_common.hpp"
+#include "params.hpp"
+
+using arsim_host::fail;
+
+namespace arsim {
+
+namespace {
+
+int ceil_log2(long long v) {
+    int b = 0;
+    while ((1ll << b) < v) ++b;
+    return b;
+}
+
+bool same_phys(const DevPhys &a, const DevPhys &b) { return std::memcmp(&a, &b, sizeof(DevPhys)) == 0; }
+
+}  // namespace
+
+template <class T> int Engine::alloc(T **p, size_t count) {
+    void *d = nullptr;
+    size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    hipError_t e = hipMalloc(&d, bytes);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? ARSIM_ERR_OUT_OF_MEMORY : ARSIM_ERR_HIP,
+                    std::string("engine allocation: ") + hipGetErrorString(e));
+    allocs.push_back(d);
+    devBytes += bytes;
+    *p = (T *)d;
+    return ARSIM_OK;
+}
+
+Engine::~Engine() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    for (void *p : allocs) hipFree(p);
+    if (st && ownStream) hipStreamDestroy(st);
+}
+
+void Engine::invalidate_graph() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    graphReady = false;
+}
+
+SceneView Engine::scene_view() const {
+    SceneView sv{};
+    sv.count = S;
+    sv.table = geomD;
+    if (S == 1) sv.single = geoms[0];
+    return sv;
+}
+
+BodyBufs Engine::body_bufs() const {
+    BodyBufs b{};
+    b.pos = rbPos; b.vel = rbVel; b.ang = rbAng; b.qinc = qinc; b.qcum = qcum;
+    b.Ib = Ib; b.Iw = Iw; b.model = nullptr; b.mass = mass; b.pstart = pstartD; b.pcount = pcountD;
+    return b;
+}
+
+PartBufs Engine::part_bufs() const {
+    PartBufs p{};
+    p.pos = pos; p.vel = vel; p.relPos = relPos; p.bodyFrame = bodyFrame; p.rb = rb;
+    p.scene = S > 1 ? scene : nullptr;
+    return p;
+}
+
+const char *const *Engine::stage_names(int method, int *count) {
+    static const char *const grid[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "dem_grid",
+                                       "reduce_apply"};
+    static const char *const lbvh[] = {"body_integrate", "pose_key", "sort", "lbvh_build", "lbvh_refit", "dem_lbvh",
+                                       "reduce_apply"};
+    if (method == ARSIM_METHOD_GRID) { *count = 6; return grid; }
+    *count = 7;
+    return lbvh;
+}
+
+size_t Engine::scene_bytes_estimate() const { return devBytes; }
+
+int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning, int dev, hipStream_t stream) {
+    if (src.empty()) return fail(ARSIM_ERR_INVALID_ARG, "engine: no scenes");
+    if (src.size() > 65535) return fail(ARSIM_ERR_INVALID_ARG, "engine: at most 65535 scenes");
+    device = dev;
+    ARSIM_HIP(hipSetDevice(device));
+    if (stream) { st = stream; ownStream = false; }
+    else ARSIM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    if (tuning) tu = *tuning;
+    if (tu.demBlock <= 0 || tu.demBlock > 1024 || (tu.demBlock & 63)) return fail(ARSIM_ERR_INVALID_ARG, "bad demBlock");
+    if (!sort_tuning_supported(tu.sortItemsPerThread, tu.sortRadixBits))
+        return fail(ARSIM_ERR_INVALID_ARG, "unsupported sort tuning");
+
+    // ---- layout
+    S = (int)src.size();
+    struct Desc { int n, b; ArSimParams p; ArSimStepConfig c; };
+    std::vector<Desc> d(S);
+    for (int s = 0; s < S; ++s) {
+        if (src[s].host) {
+            const ArSimSceneHost *h = src[s].host;
+            if (h->numParticles <= 0 || h->numBodies < 0)
+                return fail(ARSIM_ERR_INVALID_ARG, "engine: scene needs particles");
+            if (!h->pos || !h->vel || !h->rbIndices || !h->bodyFramePos)
+                return fail(ARSIM_ERR_INVALID_ARG, "engine: null particle array");
+            if (h->numBodies > 0 && (!h->rbPositions || !h->rbVelocities || !h->rbAngularVelocity || !h->rbQuaternion ||
+                                     !h->rbInertia || !h->rbMass || !h->particlesPerObject))
+                return fail(ARSIM_ERR_INVALID_ARG, "engine: null body array");
+            d[s] = Desc{h->numParticles, h->numBodies, h->params, h->cfg};
+        } else if (src[s].from) {
+            const Engine *o = src[s].from;
+            int k = src[s].fromScene;
+            if (k < 0 || k >= o->S) return fail(ARSIM_ERR_INVALID_ARG, "engine: bad source scene");
+            d[s] = Desc{o->pCount[k], o->bCount[k], o->p0, o->c0};
+            // per-scene geometry of the source
+            const SceneGeom &g = o->geoms[k];
+            d[s].p.gridSize = {g.gridMask[0] + 1u, g.gridMask[1] + 1u, g.gridMask[2] + 1u};
+            for (int a = 0; a < 3; ++a) { d[s].c.boxMin[a] = g.boxMin[a]; d[s].c.boxMax[a] = g.boxMax[a]; }
+            d[s].c.wallMask = g.wallMask;
+        } else {
+            return fail(ARSIM_ERR_INVALID_ARG, "engine: empty scene source");
+        }
+    }
+    p0 = d[0].p;
+    c0 = d[0].c;
+    method = c0.method;
+    flags = c0.flags;
+    ph = make_phys(p0, c0);
+    if (method < 0 || method > 2) return fail(ARSIM_ERR_INVALID_ARG, "engine: bad method");
+    if (method == ARSIM_METHOD_LBVH63 && S > 1)
+        return fail(ARSIM_ERR_UNSUPPORTED, "engine: 63-bit Morton codes leave no key bits for scene ids");
+    if ((flags & ARSIM_DYNAMIC_AABB) && S > 1)
+        return fail(ARSIM_ERR_UNSUPPORTED, "engine: a dynamic Morton box is per scene only with one scene");
+    geoms.resize(S);
+    pStart.resize(S); pCount.resize(S); bStart.resize(S); bCount.resize(S);
+    long long nTot = 0, bTot = 0;
+    int maxCellBits = 0;
+    for (int s = 0; s < S; ++s) {
+        if (d[s].c.method != method) return fail(ARSIM_ERR_INVALID_ARG, "engine: scenes mix methods");
+        if (!same_phys(make_phys(d[s].p, d[s].c), ph))
+            return fail(ARSIM_ERR_INVALID_ARG, "engine: scenes must share the physics parameters");
+        pStart[s] = (int)nTot; pCount[s] = d[s].n;
+        bStart[s] = (int)bTot; bCount[s] = d[s].b;
+        nTot += d[s].n; bTot += d[s].b;
+        int rc = make_geom(d[s].p, d[s].c, pStart[s], pCount[s], geoms[s]);
+        if (rc) return rc;
+        if (method == ARSIM_METHOD_GRID) {
+            long long cells = (long long)d[s].p.gridSize.x * d[s].p.gridSize.y * d[s].p.gridSize.z;
+            if (cells <= 0) return fail(ARSIM_ERR_INVALID_ARG, "engine: grid has no cells");
+            maxCellBits = std::max(maxCellBits, ceil_log2(cells));
+        }
+    }
+    if (nTot >= (1ll << 30)) return fail(ARSIM_ERR_INVALID_ARG, "engine: too many particles");
+    N = (int)nTot;
+    B = (int)bTot;
+    sceneBits = S > 1 ? ceil_log2(S) : 0;
+    if (method == ARSIM_METHOD_GRID) {
+        cellBits = maxCellBits;
+        keyBits = cellBits + sceneBits;
+        if (keyBits > 32) return fail(ARSIM_ERR_OVERFLOW, "engine: scene id + cell bits exceed 32");
+        keyBytes = 4;
+        numCells = (size_t)1 << keyBits;
+        for (int s = 0; s < S; ++s) geoms[s].cellBase = (unsigned)s << cellBits;
+    } else if (method == ARSIM_METHOD_LBVH30 && S == 1) {
+        keyBits = 30; keyBytes = 4;
+    } else if (method == ARSIM_METHOD_LBVH63) {
+        keyBits = 63; keyBytes = 8;
+    } else {
+        keyBits = 30 + sceneBits; keyBytes = 8;
+    }
+
+    // ---- allocation (everything a step touches, once)
+    int rc = 0;
+#define A(ptr, cnt) if ((rc = alloc(&(ptr), (cnt))) != 0) return rc
+    A(pos, N); A(vel, N); A(relPos, N); A(bodyFrame, N); A(pForce, N);
+    A(rb, N); A(scene, N);
+    size_t nb = std::max(B, 1);
+    A(rbPos, nb); A(rbVel, nb); A(rbAng, nb); A(qinc, nb); A(qcum, nb);
+    A(Ib, nb * 9); A(Iw, nb * 9); A(mass, nb); A(pstartD, nb); A(pcountD, nb);
+    A(geomD, (size_t)S); A(sceneStartD, (size_t)S); A(sceneCountD, (size_t)S);
+    {
+        unsigned char *ka = nullptr, *kb = nullptr;
+        A(ka, (size_t)N * keyBytes); A(kb, (size_t)N * keyBytes);
+        keysA = ka; keysB = kb;
+    }
+    A(valsA, N); A(valsB, N);
+    sortWsBytes = sort_workspace_bytes(N, keyBytes);
+    {
+        unsigned char *w = nullptr;
+        A(w, sortWsBytes);
+        sortWs = w;
+    }
+    A(overflowD, 4); A(flagD, 4);
+    if (method == ARSIM_METHOD_GRID) {
+        A(cellStart, numCells); A(cellEnd, numCells);
+        A(sortedPos, N); A(sortedVel, N);
+    } else {
+        size_t nodes = (size_t)2 * N;
+        A(tree.left, nodes); A(tree.right, nodes); A(tree.parent, nodes);
+        A(tree.bounds, 2 * nodes); A(tree.cms, N); A(tree.sortedVel, N); A(tree.counter, N);
+        if (S > 1) A(tree.sceneRoot, S);
+        A(aabbPart, (size_t)2 * aabb_max_partials()); A(aabbBox, 2);
+    }
+#undef A
+
+    // ---- fill
+    std::vector<int> hStart(S), hCount(S);
+    for (int s = 0; s < S; ++s) { hStart[s] = pStart[s]; hCount[s] = pCount[s]; }
+    std::vector<float4> zeroN;
+    for (int s = 0; s < S; ++s) {
+        const size_t po = pStart[s], bo = bStart[s], n = pCount[s], b = bCount[s];
+        if (src[s].host) {
+            const ArSimSceneHost *h = src[s].host;
+            ARSIM_HIP(hipMemcpyAsync(pos + po, h->pos, n * 16, hipMemcpyHostToDevice, st));
+            ARSIM_HIP(hipMemcpyAsync(vel + po, h->vel, n * 16, hipMemcpyHostToDevice, st));
+            ARSIM_HIP(hipMemcpyAsync(bodyFrame + po, h->bodyFramePos, n * 16, hipMemcpyHostToDevice, st));
+            ARSIM_HIP(hipMemsetAsync(relPos + po, 0, n * 16, st));
+            ARSIM_HIP(hipMemsetAsync(pForce + po, 0, n * 16, st));
+            std::vector<int> rbg(n);
+            for (size_t i = 0; i < n; ++i) {
+                int v = h->rbIndices[i];
+                if (v >= (int)b) return fail(ARSIM_ERR_INVALID_ARG, "engine: rbIndex out of range");
+                rbg[i] = v >= 0 ? v + (int)bo : -1;
+            }
+            ARSIM_HIP(hipMemcpyAsync(rb + po, rbg.data(), n * 4, hipMemcpyHostToDevice, st));
+            ARSIM_HIP(hipStreamSynchronize(st));   // rbg is a host temporary
+            if (b > 0) {
+                ARSIM_HIP(hipMemcpyAsync(rbPos + bo, h->rbPositions, b * 16, hipMemcpyHostToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(rbVel + bo, h->rbVelocities, b * 16, hipMemcpyHostToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(rbAng + bo, h->rbAngularVelocity, b * 16, hipMemcpyHostToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(qcum + bo, h->rbQuaternion, b * 16, hipMemcpyHostToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(Ib + bo * 9, h->rbInertia, b * 36, hipMemcpyHostToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(Iw + bo * 9, h->rbInertia, b * 36, hipMemcpyHostToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(mass + bo, h->rbMass, b * 4, hipMemcpyHostToDevice, st));
+                std::vector<float4> ident(b, make_float4(0.f, 0.f, 0.f, 1.f));
+                std::vector<int> ps(b), pc(b);
+                int acc = (int)po;
+                for (size_t k = 0; k < b; ++k) {
+                    ps[k] = acc;
+                    pc[k] = h->particlesPerObject[k];
+                    acc += pc[k];
+                    for (int j = ps[k]; j < acc; ++j)
+                        if (j - (int)po >= (int)n || h->rbIndices[j - po] != (int)k)
+                            return fail(ARSIM_ERR_INVALID_ARG, "engine: body particles must be contiguous, in body order");
+                }
+                ARSIM_HIP(hipMemcpyAsync(qinc + bo, ident.data(), b * 16, hipMemcpyHostToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(pstartD + bo, ps.data(), b * 4, hipMemcpyHostToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(pcountD + bo, pc.data(), b * 4, hipMemcpyHostToDevice, st));
+                ARSIM_HIP(hipStreamSynchronize(st));
+            }
+        } else {
+            const Engine *o = src[s].from;
+            const int k = src[s].fromScene;
+            const size_t opo = o->pStart[k], obo = o->bStart[k];
+            ARSIM_HIP(hipMemcpyAsync(pos + po, o->pos + opo, n * 16, hipMemcpyDeviceToDevice, st));
+            ARSIM_HIP(hipMemcpyAsync(vel + po, o->vel + opo, n * 16, hipMemcpyDeviceToDevice, st));
+            ARSIM_HIP(hipMemcpyAsync(bodyFrame + po, o->bodyFrame + opo, n * 16, hipMemcpyDeviceToDevice, st));
+            ARSIM_HIP(hipMemcpyAsync(relPos + po, o->relPos + opo, n * 16, hipMemcpyDeviceToDevice, st));
+            ARSIM_HIP(hipMemcpyAsync(pForce + po, o->pForce + opo, n * 16, hipMemcpyDeviceToDevice, st));
+            launch_copy_offset(rb + po, o->rb + opo, (int)n, (int)bo - (int)obo, st);
+            if (b > 0) {
+                ARSIM_HIP(hipMemcpyAsync(rbPos + bo, o->rbPos + obo, b * 16, hipMemcpyDeviceToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(rbVel + bo, o->rbVel + obo, b * 16, hipMemcpyDeviceToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(rbAng + bo, o->rbAng + obo, b * 16, hipMemcpyDeviceToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(qinc + bo, o->qinc + obo, b * 16, hipMemcpyDeviceToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(qcum + bo, o->qcum + obo, b * 16, hipMemcpyDeviceToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(Ib + bo * 9, o->Ib + obo * 9, b * 36, hipMemcpyDeviceToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(Iw + bo * 9, o->Iw + obo * 9, b * 36, hipMemcpyDeviceToDevice, st));
+                ARSIM_HIP(hipMemcpyAsync(mass + bo, o->mass + obo, b * 4, hipMemcpyDeviceToDevice, st));
+                launch_copy_offset(pstartD + bo, o->pstartD + obo, (int)b, (int)po - (int)opo, st);
+                ARSIM_HIP(hipMemcpyAsync(pcountD + bo, o->pcountD + obo, b * 4, hipMemcpyDeviceToDevice, st));
+            }
+        }
+        launch_fill_u16(scene + po, (uint16_t)s, n, st);
+    }
+    ARSIM_HIP(hipMemcpyAsync(geomD, geoms.data(), sizeof(SceneGeom) * S, hipMemcpyHostToDevice, st));
+    ARSIM_HIP(hipMemcpyAsync(sceneStartD, hStart.data(), 4 * (size_t)S, hipMemcpyHostToDevice, st));
+    ARSIM_HIP(hipMemcpyAsync(sceneCountD, hCount.data(), 4 * (size_t)S, hipMemcpyHostToDevice, st));
+    if (tree.sceneRoot) launch_fill_u32((unsigned *)tree.sceneRoot, 0xffffffffu, (size_t)S, st);
+    launch_fill_u32((unsigned *)overflowD, 0u, 1, st);
+    if (cellStart) launch_fill_u32(cellStart, kEmpty, numCells, st);
+    ARSIM_HIP(hipGetLastError());
+    ARSIM_HIP(hipStreamSynchronize(st));
+    first = true;
+    prevSorted = nullptr;
+    return ARSIM_OK;
+}
+
+int Engine::reset_grid_tables() {
+    invalidate_graph();
+    if (cellStart) launch_fill_u32(cellStart, kEmpty, numCells, st);
+    first = true;
+    prevSorted = nullptr;
+    ARSIM_HIP(hipGetLastError());
+    return ARSIM_OK;
+}
+
+int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
+    const BodyBufs bb = body_bufs();
+    const PartBufs pb = part_bufs();
+    const SceneView sv = scene_view();
+    SortTuning stt;
+    stt.ipt = tu.sortItemsPerThread;
+    stt.radixBits = tu.sortRadixBits;
+    int k = 0;
+    auto rec = [&]() {
+        if (ev) hipEventRecord(ev[k], st);
+        ++k;
+    };
+    rec();
+    if (method == ARSIM_METHOD_GRID) {
+        launch_body_integrate(bb, B, ph, first ? nullptr : (const unsigned *)prevSorted, N, cellStart, st);
+        rec();
+        KeyOut ko{keysA, nullptr, nullptr};
+        launch_pose_key(pb, bb, N, ph, sv, kKeyGrid, ko, true, true, st);
+        rec();
+        unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
+        if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
+            return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
+        rec();
+        launch_reorder(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, sortedPos, sortedVel, cellStart, cellEnd, N, st);
+        rec();
+        DemOut o{};
+        o.pForce = pForce; o.vel = vel; o.relPos = relPos;
+        launch_dem_grid(sortedPos, sortedVel, vA, cellStart, cellEnd, N, ph, sv, S > 1, o, tu.demBlock, st);
+        rec();
+        launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
+        rec();
+        prevSorted = kA;
+        lastKeys = kA;
+        lastIdx = vA;
+    } else {
+        launch_body_integrate(bb, B, ph, nullptr, 0, nullptr, st);
+        rec();
+        const int mode = S > 1 ? kKeyForest30 : (method == ARSIM_METHOD_LBVH30 ? kKeyMorton30 : kKeyMorton63);
+        if (flags & ARSIM_DYNAMIC_AABB) {
+            KeyOut none{};
+            launch_pose_key(pb, bb, N, ph, sv, kKeyNone, none, true, true, st);
+            launch_aabb(pos, N, aabbBox, aabbBox + 1, aabbPart, aabb_max_partials(), st);
+            KeyOut ko{keysA, nullptr, aabbBox};
+            launch_pose_key(pb, bb, N, ph, sv, mode, ko, false, false, st);
+        } else {
+            KeyOut ko{keysA, nullptr, nullptr};
+            launch_pose_key(pb, bb, N, ph, sv, mode, ko, true, true, st);
+        }
+        rec();
+        void *sortedKeys;
+        unsigned *vA = valsA, *vB = valsB;
+        if (keyBytes == 4) {
+            unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB;
+            if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
+                return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
+            sortedKeys = kA;
+        } else {
+            uint64_t *kA = (uint64_t *)keysA, *kB = (uint64_t *)keysB;
+            if (sort_pairs<uint64_t>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
+                return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
+            sortedKeys = kA;
+        }
+        rec();
+        launch_lbvh_build(sortedKeys, keyBytes, N, S > 1 ? 30 : 0, sceneStartD, sceneCountD, S, tree, st);
+        rec();
+        launch_lbvh_refit((const int *)vA, pos, vel, rb, S > 1 ? scene : nullptr, ph.r, N, tree, st);
+        rec();
+        DemOut o{};
+        o.pForce = pForce; o.vel = vel; o.relPos = relPos;
+        launch_dem_lbvh((const int *)vA, tree, N, ph, sv, S > 1, o, tu.demBlock, st, overflowD);
+        rec();
+        launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
+        rec();
+        lastKeys = sortedKeys;
+        lastIdx = vA;
+    }
+    first = false;
+    if (numStages) *numStages = k - 1;
+    return ARSIM_OK;
+}
+
+int Engine::step(int n) {
+    for (int i = 0; i < n; ++i) {
+        if (tu.useGraph && !first) {
+            if (!graphReady) {
+                hipGraph_t g = nullptr;
+                ARSIM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                int rc = enqueue_step(nullptr, nullptr);
+                hipError_t e = hipStreamEndCapture(st, &g);
+                if (rc) { if (g) hipGraphDestroy(g); return rc; }
+                if (e != hipSuccess) return fail(ARSIM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+                e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (e != hipSuccess) return fail(ARSIM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+                graphReady = true;
+            }
+            ARSIM_HIP(hipGraphLaunch(gexec, st));
+        } else {
+            int rc = enqueue_step(nullptr, nullptr);
+            if (rc) return rc;
+        }
+    }
+    ARSIM_HIP(hipGetLastError());
+    return ARSIM_OK;
+}
+
+// Broadphase of the current state (no integration) and contact-list emission.
+int Engine::enqueue_broadphase(int *pairsD, int *countsD, int maxPer) {
+    invalidate_graph();
+    const BodyBufs bb = body_bufs();
+    const PartBufs pb = part_bufs();
+    const SceneView sv = scene_view();
+    SortTuning stt;
+    stt.ipt = tu.sortItemsPerThread;
+    stt.radixBits = tu.sortRadixBits;
+    DemOut o{};
+    o.pairs = pairsD; o.counts = countsD; o.maxPer = maxPer;
+    if (method == ARSIM_METHOD_GRID) {
+        if (!first) launch_body_integrate(bb, 0, ph, (const unsigned *)prevSorted, N, cellStart, st);
+        KeyOut ko{keysA, nullptr, nullptr};
+        launch_pose_key(pb, bb, N, ph, sv, kKeyGrid, ko, false, false, st);
+        unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
+        if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
+            return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
+        launch_reorder(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, sortedPos, sortedVel, cellStart, cellEnd, N, st);
+        launch_dem_grid(sortedPos, sortedVel, vA, cellStart, cellEnd, N, ph, sv, S > 1, o, tu.demBlock, st);
+        prevSorted = kA;
+        lastKeys = kA;
+        lastIdx = vA;
+        first = false;
+    } else {
+        const int mode = S > 1 ? kKeyForest30 : (method == ARSIM_METHOD_LBVH30 ? kKeyMorton30 : kKeyMorton63);
+        if (flags & ARSIM_DYNAMIC_AABB) launch_aabb(pos, N, aabbBox, aabbBox + 1, aabbPart, aabb_max_partials(), st);
+        KeyOut ko{keysA, nullptr, (flags & ARSIM_DYNAMIC_AABB) ? aabbBox : nullptr};
+        launch_pose_key(pb, bb, N, ph, sv, mode, ko, false, false, st);
+        void *sortedKeys;
+        unsigned *vA = valsA, *vB = valsB;
+        if (keyBytes == 4) {
+            unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB;
+            if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
+                return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
+            sortedKeys = kA;
+        } else {
+            uint64_t *kA = (uint64_t *)keysA, *kB = (uint64_t *)keysB;
+            if (sort_pairs<uint64_t>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
+                return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
+            sortedKeys = kA;
+        }
+        launch_lbvh_build(sortedKeys, keyBytes, N, S > 1 ? 30 : 0, sceneStartD, sceneCountD, S, tree, st);
+        launch_lbvh_refit((const int *)vA, pos, vel, rb, S > 1 ? scene : nullptr, ph.r, N, tree, st);
+        launch_dem_lbvh((const int *)vA, tree, N, ph, sv, S > 1, o, tu.demBlock, st, overflowD);
+        lastKeys = sortedKeys;
+        lastIdx = vA;
+    }
+    ARSIM_HIP(hipGetLastError());
+    return ARSIM_OK;
+}
+
+}  // namespace arsim
+
+using arsim::Engine;
+
+extern "C" {
+
+int arsim_engine_create(const ArSimSceneHost *scenes, int numScenes, const ArSimTuning *tuning, int device,
+                        arsim_engine **out) {
+    if (!scenes || numScenes <= 0 || !out) return fail(ARSIM_ERR_INVALID_ARG, "engine_create: bad arguments");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(ARSIM_ERR_NO_DEVICE, "no HIP device");
+    std::vector<arsim::SceneSource> src(numScenes);
+    for (int s = 0; s < numScenes; ++s) src[s].host = &scenes[s];
+    arsim_engine *e = new arsim_engine();
+    int rc = e->e.build(src, tuning, device, nullptr);
+    if (rc) { delete e; return rc; }
+    *out = e;
+    return ARSIM_OK;
+}
+
+int arsim_engine_destroy(arsim_engine *e) {
+    if (!e) return ARSIM_OK;
+    hipSetDevice(e->e.device);
+    if (e->e.st) hipStreamSynchronize(e->e.st);
+    delete e;
+    return ARSIM_OK;
+}
+
+int arsim_engine_step(arsim_engine *e, int numSteps) {
+    if (!e || numSteps < 0) return fail(ARSIM_ERR_INVALID_ARG, "engine_step: bad arguments");
+    return e->e.step(numSteps);
+}
+
+int arsim_engine_synchronize(arsim_engine *e) {
+    if (!e) return fail(ARSIM_ERR_INVALID_ARG, "null engine");
+    ARSIM_HIP(hipStreamSynchronize(e->e.st));
+    int ov = 0;
+    if (e->e.overflowD) {
+        ARSIM_HIP(hipMemcpy(&ov, e->e.overflowD, 4, hipMemcpyDeviceToHost));
+        if (ov) return fail(ARSIM_ERR_OVERFLOW, "LBVH traversal stack overflow");
+    }
+    return ARSIM_OK;
+}
+
+int arsim_engine_set_dt(arsim_engine *e, float dt) {
+    if (!e || !(dt > 0.f)) return fail(ARSIM_ERR_INVALID_ARG, "engine_set_dt: bad arguments");
+    if (e->e.ph.dt != dt) {
+        e->e.ph.dt = dt;
+        e->e.c0.dt = dt;
+        e->e.invalidate_graph();   // dt is a kernel argument of the captured launches
+    }
+    return ARSIM_OK;
+}
+
+int arsim_engine_stream(arsim_engine *e, arsim_stream_t *stream) {
+    if (!e || !stream) return fail(ARSIM_ERR_INVALID_ARG, "null argument");
+    *stream = (arsim_stream_t)e->e.st;
+    return ARSIM_OK;
+}
+
+int arsim_engine_num_particles(arsim_engine *e, int sceneIdx) {
+    if (!e) return -1;
+    if (sceneIdx < 0) return e->e.N;
+    if (sceneIdx >= e->e.S) return -1;
+    return e->e.pCount[sceneIdx];
+}
+
+size_t arsim_engine_device_bytes(arsim_engine *e) { return e ? e->e.devBytes : 0; }
+
+int arsim_engine_download(arsim_engine *e, int s, float *pos, float *vel, float *relativePos, float *pForce,
+                          float *pTorque, float *rbPositions, float *rbVelocities, float *rbAngularVelocity,
+                          float *rbQuaternion, float *rbCurrentInertia) {
+    if (!e || s < 0 || s >= e->e.S) return fail(ARSIM_ERR_INVALID_ARG, "engine_download: bad scene");
+    Engine &E = e->e;
+    ARSIM_HIP(hipStreamSynchronize(E.st));
+    const size_t po = E.pStart[s], n = E.pCount[s], bo = E.bStart[s], b = E.bCount[s];
+    if (pos) ARSIM_HIP(hipMemcpy(pos, E.pos + po, n * 16, hipMemcpyDeviceToHost));
+    if (vel) ARSIM_HIP(hipMemcpy(vel, E.vel + po, n * 16, hipMemcpyDeviceToHost));
+    if (relativePos) ARSIM_HIP(hipMemcpy(relativePos, E.relPos + po, n * 16, hipMemcpyDeviceToHost));
+    std::vector<float> f, r;
+    std::vector<int> rbh;
+    if (pForce || pTorque) {
+        f.resize(4 * n);
+        r.resize(4 * n);
+        rbh.resize(n);
+        ARSIM_HIP(hipMemcpy(f.data(), E.pForce + po, n * 16, hipMemcpyDeviceToHost));
+        ARSIM_HIP(hipMemcpy(r.data(), E.relPos + po, n * 16, hipMemcpyDeviceToHost));
+        ARSIM_HIP(hipMemcpy(rbh.data(), E.rb + po, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            if (rbh[i] < 0) { for (int c = 0; c < 4; ++c) f[4 * i + c] = 0.f; }
+        }
+        if (pForce) std::memcpy(pForce, f.data(), n * 16);
+        if (pTorque) {
+            // the contact kernel's per-particle torque, relativePos × pForce
+            for (size_t i = 0; i < n; ++i) {
+                float rx = r[4 * i], ry = r[4 * i + 1], rz = r[4 * i + 2];
+                float fx = f[4 * i], fy = f[4 * i + 1], fz = f[4 * i + 2];
+                bool body = rbh[i] >= 0;
+                pTorque[4 * i] = body ? ry * fz - rz * fy : 0.f;
+                pTorque[4 * i + 1] = body ? rz * fx - rx * fz : 0.f;
+                pTorque[4 * i + 2] = body ? rx * fy - ry * fx : 0.f;
+                pTorque[4 * i + 3] = 0.f;
+            }
+        }
+    }
+    if (b > 0) {
+        if (rbPositions) ARSIM_HIP(hipMemcpy(rbPositions, E.rbPos + bo, b * 16, hipMemcpyDeviceToHost));
+        if (rbVelocities) ARSIM_HIP(hipMemcpy(rbVelocities, E.rbVel + bo, b * 16, hipMemcpyDeviceToHost));
+        if (rbAngularVelocity) ARSIM_HIP(hipMemcpy(rbAngularVelocity, E.rbAng + bo, b * 16, hipMemcpyDeviceToHost));
+        if (rbQuaternion) ARSIM_HIP(hipMemcpy(rbQuaternion, E.qcum + bo, b * 16, hipMemcpyDeviceToHost));
+        if (rbCurrentInertia) ARSIM_HIP(hipMemcpy(rbCurrentInertia, E.Iw + bo * 9, b * 36, hipMemcpyDeviceToHost));
+    }
+    return ARSIM_OK;
+}
+
+int arsim_engine_profile_step(arsim_engine *e, float *ms, const char **names, int maxStages) {
+    if (!e) return fail(ARSIM_ERR_INVALID_ARG, "null engine");
+    Engine &E = e->e;
+    int count = 0;
+    const char *const *nm = Engine::stage_names(E.method, &count);
+    std::vector<hipEvent_t> ev(count + 1);
+    for (auto &x : ev) ARSIM_HIP(hipEventCreate(&x));
+    int ns = 0;
+    int rc = E.enqueue_step(ev.data(), &ns);
+    if (rc == 0) {
+        ARSIM_HIP(hipEventSynchronize(ev[count]));
+        for (int k = 0; k < count && k < maxStages; ++k) {
+            float t = 0.f;
+            hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+            if (ms) ms[k] = t;
+            if (names) names[k] = nm[k];
+        }
+    }
+    for (auto &x : ev) hipEventDestroy(x);
+    E.invalidate_graph();   // captured graphs stay valid; re-capture keeps the state machine simple
+    return rc ? rc : count;
+}
+
+int arsim_engine_debug_buffers(arsim_engine *e, void **sortedKeys, void **sortedIndex, void **cellStart,
+                               void **cellEnd, void **sortedPos, void **sortedVel, void **lbvhLeft, void **lbvhRight,
+                               void **lbvhBounds, void **lbvhCMs) {
+    if (!e) return fail(ARSIM_ERR_INVALID_ARG, "null engine");
+    Engine &E = e->e;
+    if (sortedKeys) *sortedKeys = E.lastKeys;
+    if (sortedIndex) *sortedIndex = E.lastIdx;
+    if (cellStart) *cellStart = E.cellStart;
+    if (cellEnd) *cellEnd = E.cellEnd;
+    if (sortedPos) *sortedPos = E.sortedPos;
+    if (sortedVel) *sortedVel = E.method == ARSIM_METHOD_GRID ? (void *)E.sortedVel : (void *)E.tree.sortedVel;
+    if (lbvhLeft) *lbvhLeft = E.tree.left;
+    if (lbvhRight) *lbvhRight = E.tree.right;
+    if (lbvhBounds) *lbvhBounds = E.tree.bounds;
+    if (lbvhCMs) *lbvhCMs = E.tree.cms;
+    return ARSIM_OK;
+}
+
+int arsim_engine_contact_pairs(arsim_engine *e, int *pairsHost, int *countsHost, int maxPer) {
+    if (!e || !pairsHost || !countsHost || maxPer <= 0) return fail(ARSIM_ERR_INVALID_ARG, "contact_pairs: bad arguments");
+    Engine &E = e->e;
+    int *pd = nullptr, *cd = nullptr;
+    ARSIM_HIP(hipSetDevice(E.device));
+    ARSIM_HIP(hipMalloc(&pd, (size_t)E.N * maxPer * 4));
+    if (hipMalloc(&cd, (size_t)E.N * 4) != hipSuccess) { hipFree(pd); return fail(ARSIM_ERR_OUT_OF_MEMORY, "contact_pairs"); }
+    int rc = E.enqueue_broadphase(pd, cd, maxPer);
+    if (rc == 0) {
+        hipStreamSynchronize(E.st);
+        hipMemcpy(pairsHost, pd, (size_t)E.N * maxPer * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(countsHost, cd, (size_t)E.N * 4, hipMemcpyDeviceToHost);
+        for (int i = 0; i < E.N; ++i)
+            if (countsHost[i] > maxPer) rc = fail(ARSIM_ERR_OVERFLOW, "contact list longer than maxPerParticle");
+    }
+    hipFree(pd);
+    hipFree(cd);
+    return rc;
+}
+
+int arsim_engine_check_finite(arsim_engine *e, int *nonFinite) {
+    if (!e || !nonFinite) return fail(ARSIM_ERR_INVALID_ARG, "null argument");
+    Engine &E = e->e;
+    arsim::launch_fill_u32((unsigned *)E.flagD, 0u, 1, E.st);
+    arsim::launch_check_finite(E.pos, E.N, E.flagD, E.st);
+    arsim::launch_check_finite(E.vel, E.N, E.flagD, E.st);
+    arsim::launch_check_finite(E.rbPos, E.B, E.flagD, E.st);
+    arsim::launch_check_finite(E.rbVel, E.B, E.flagD, E.st);
+    arsim::launch_check_finite(E.rbAng, E.B, E.flagD, E.st);
+    ARSIM_HIP(hipStreamSynchronize(E.st));
+    int f = 0;
+    ARSIM_HIP(hipMemcpy(&f, E.flagD, 4, hipMemcpyDeviceToHost));
+    *nonFinite = f;
+    return f ? fail(ARSIM_ERR_NAN, "non-finite state") : ARSIM_OK;
+}
+
+int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best, float *bestMs) {
+    if (!e || stepsPerTrial <= 0) return fail(ARSIM_ERR_INVALID_ARG, "autotune: bad arguments");
+    Engine &E = e->e;
+    // the trials advance a scratch copy of the state: snapshot everything a step writes
+    struct Snap { void *dst; const void *src; size_t bytes; };
+    std::vector<Snap> snaps = {
+        {nullptr, E.pos, (size_t)E.N * 16},  {nullptr, E.vel, (size_t)E.N * 16},
+        {nullptr, E.relPos, (size_t)E.N * 16}, {nullptr, E.pForce, (size_t)E.N * 16},
+        {nullptr, E.rbPos, (size_t)E.B * 16}, {nullptr, E.rbVel, (size_t)E.B * 16},
+        {nullptr, E.rbAng, (size_t)E.B * 16}, {nullptr, E.qinc, (size_t)E.B * 16},
+        {nullptr, E.qcum, (size_t)E.B * 16}, {nullptr, E.Iw, (size_t)E.B * 36}};
+    int rc = ARSIM_OK;
+    for (auto &s : snaps) {
+        if (s.bytes == 0) continue;
+        if (hipMalloc(&s.dst, s.bytes) != hipSuccess) { rc = fail(ARSIM_ERR_OUT_OF_MEMORY, "autotune snapshot"); break; }
+        hipMemcpyAsync(s.dst, s.src, s.bytes, hipMemcpyDeviceToDevice, E.st);
+    }
+    const ArSimTuning orig = E.tu;
+    ArSimTuning bestT = orig;
+    float bestT_ms = 1e30f;
+    const int blocks[] = {64, 128, 256};
+    const int ipts[] = {8, 12, 16};
+    const int rbits[] = {7, 8};
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    for (int bl : blocks)
+        for (int ip : ipts)
+            for (int rb : rbits) {
+                if (rc) break;
+                if (!arsim::sort_tuning_supported(ip, rb)) continue;
+                for (auto &s : snaps)
+                    if (s.dst) hipMemcpyAsync((void *)s.src, s.dst, s.bytes, hipMemcpyDeviceToDevice, E.st);
+                E.tu = ArSimTuning{bl, ip, rb, orig.useGraph};
+                E.reset_grid_tables();
+                if ((rc = E.step(2)) != 0) break;
+                hipEventRecord(a, E.st);
+                if ((rc = E.step(stepsPerTrial)) != 0) break;
+                hipEventRecord(b, E.st);
+                hipEventSynchronize(b);
+                float t = 0.f;
+                hipEventElapsedTime(&t, a, b);
+                t /= stepsPerTrial;
+                if (t < bestT_ms) { bestT_ms = t; bestT = E.tu; }
+            }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    for (auto &s : snaps)
+        if (s.dst) hipMemcpyAsync((void *)s.src, s.dst, s.bytes, hipMemcpyDeviceToDevice, E.st);
+    hipStreamSynchronize(E.st);
+    for (auto &s : snaps)
+        if (s.dst) hipFree(s.dst);
+    E.tu = rc ? orig : bestT;
+    E.reset_grid_tables();
+    hipStreamSynchronize(E.st);
+    if (rc) return rc;
+    if (best) *best = bestT;
+    if (bestMs) *bestMs = bestT_ms;
+    return ARSIM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// engine.hpp -- the scene-level step engine: S >= 1 independent scenes resident in HBM, one
+// launch per stage per step for all of them (S == 1 is the plain path).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/arsim.h"
+#include "kernels.hpp"
+
+namespace arsim {
+
+// Where a scene's initial state comes from when an engine is (re)built.
+struct SceneSource {
+    const ArSimSceneHost *host = nullptr;   // upload from host arrays, or
+    const struct Engine *from = nullptr;    // copy scene `fromScene` of another engine on the device
+    int fromScene = -1;
+};
+
+struct Engine {
+    int device = 0;
+    hipStream_t st = nullptr;
+    bool ownStream = true;
+    int S = 0, N = 0, B = 0;
+    int method = ARSIM_METHOD_GRID;
+    unsigned flags = 0;
+    ArSimParams p0{};
+    ArSimStepConfig c0{};
+    DevPhys ph{};
+    ArSimTuning tu{128, 16, 8, 1};
+    std::vector<SceneGeom> geoms;
+    std::vector<int> pStart, pCount, bStart, bCount;
+    int cellBits = 0, sceneBits = 0, keyBits = 0, keyBytes = 4;
+    size_t numCells = 0;
+
+    // device buffers
+    std::vector<void *> allocs;
+    size_t devBytes = 0;
+    float4 *pos = nullptr, *vel = nullptr, *relPos = nullptr, *bodyFrame = nullptr, *pForce = nullptr;
+    int *rb = nullptr;
+    uint16_t *scene = nullptr;
+    float4 *rbPos = nullptr, *rbVel = nullptr, *rbAng = nullptr, *qinc = nullptr, *qcum = nullptr;
+    float *Ib = nullptr, *Iw = nullptr, *mass = nullptr;
+    int *pstartD = nullptr, *pcountD = nullptr;
+    SceneGeom *geomD = nullptr;
+    int *sceneStartD = nullptr, *sceneCountD = nullptr;
+    void *keysA = nullptr, *keysB = nullptr;
+    unsigned *valsA = nullptr, *valsB = nullptr;
+    unsigned *cellStart = nullptr, *cellEnd = nullptr;
+    float4 *sortedPos = nullptr, *sortedVel = nullptr;
+    void *sortWs = nullptr;
+    size_t sortWsBytes = 0;
+    Lbvh tree{};
+    int *overflowD = nullptr, *flagD = nullptr;
+    float4 *aabbPart = nullptr, *aabbBox = nullptr;
+
+    // step state
+    bool first = true;
+    void *prevSorted = nullptr;
+    void *lastKeys = nullptr;
+    unsigned *lastIdx = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool graphReady = false;
+
+    ~Engine();
+    int build(const std::vector<SceneSource> &src, const ArSimTuning *tuning, int dev, hipStream_t stream);
+    int enqueue_step(hipEvent_t *ev, int *numStages);
+    int step(int n);
+    int enqueue_broadphase(int *pairsD, int *countsD, int maxPer);
+    void invalidate_graph();
+    int reset_grid_tables();
+    static const char *const *stage_names(int method, int *count);
+    SceneView scene_view() const;
+    BodyBufs body_bufs() const;
+    PartBufs part_bufs() const;
+    size_t scene_bytes_estimate() const;
+
+  private:
+    template <class T> int alloc(T **p, size_t count);
+};
+
+}  // namespace arsim
+
+struct arsim_engine {
+    arsim::Engine e;
+};

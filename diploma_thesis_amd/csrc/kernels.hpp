@@ -1,0 +1,109 @@
+// kernels.hpp -- host-side launch interface of the gfx950 kernels (internal to libarsim).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "arsim_device.hpp"
+
+namespace arsim {
+
+struct SortTuning {
+    int ipt = 16;        // items per thread
+    int radixBits = 8;   // digit width
+};
+
+// ---------------------------------------------------------------- sort (sort.hip)
+int sort_max_passes(int keyBytes);
+size_t sort_workspace_bytes(int n, int keyBytes);
+bool sort_tuning_supported(int ipt, int rbits);
+// Returns 0 on success; swaps *keys/*alt and *vals/*altVals once per pass so that on return
+// *keys / *vals hold the sorted data.  valsIota: the input values are 0..n-1 (not read).
+template <class K>
+int sort_pairs(K **keys, unsigned **vals, K **alt, unsigned **altVals, int n, int beginBit, int endBit,
+               bool valsIota, void *ws, size_t wsBytes, const SortTuning &tu, hipStream_t st);
+
+// ---------------------------------------------------------------- buffers
+struct BodyBufs {
+    float4 *pos, *vel, *ang;      // rbPositions, rbVelocities, rbAngularVelocity
+    float4 *qinc, *qcum;          // rbQuaternion (incremental), cumulative quaternion
+    const float *Ib;              // inverse body-frame inertia, 9 per body, column-major
+    float *Iw;                    // inverse world-frame inertia (rbCurrentInertia)
+    float *model;                 // optional column-major mat4 per body
+    const float *mass;
+    const int *pstart, *pcount;
+};
+
+struct PartBufs {
+    float4 *pos, *vel, *relPos;
+    const float4 *bodyFrame;
+    const int *rb;
+    const uint16_t *scene;        // per-particle scene id (NULL when there is one scene)
+};
+
+enum KeyMode { kKeyNone = 0, kKeyGrid = 1, kKeyMorton30 = 2, kKeyMorton63 = 3, kKeyForest30 = 4 };
+
+struct KeyOut {
+    void *keys;                   // u32 (grid / morton30) or u64 (morton63 / forest30)
+    unsigned *index;              // may be NULL when the sort generates the identity
+    const float4 *box;            // dynamic Morton box (min, max) on the device, or NULL
+};
+
+// ---------------------------------------------------------------- particle / body stages (body.hip)
+void launch_body_integrate(const BodyBufs &b, int numBodies, const DevPhys &ph, const unsigned *prevSortedKeys,
+                           int numParticles, unsigned *cellStart, hipStream_t st);
+// doBodies / doGrains select which particles are updated; mode selects the key written.
+void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,
+                     const SceneView &sv, int mode, const KeyOut &ko, bool doBodies, bool doGrains,
+                     hipStream_t st);
+void launch_reduce_apply(const float4 *pForce, const float4 *pTorque, const float4 *relPos, const BodyBufs &b,
+                         int numBodies, const DevPhys &ph, hipStream_t st);
+void launch_fill_u32(unsigned *p, unsigned value, size_t count, hipStream_t st);
+void launch_aabb(const float4 *pos, int n, float4 *outMin, float4 *outMax, float4 *partials, int maxPartials,
+                 hipStream_t st);
+int  aabb_max_partials();
+void launch_check_finite(const float4 *a, int n, int *flag, hipStream_t st);
+// dst[i] = src[i] + delta for src[i] >= 0, else src[i] (rebases body indices when scenes move).
+void launch_copy_offset(int *dst, const int *src, int n, int delta, hipStream_t st);
+void launch_fill_u16(uint16_t *p, uint16_t value, size_t count, hipStream_t st);
+
+// ---------------------------------------------------------------- grid path (grid.hip)
+// sortedPos.w carries the particle's rbIndex bits; with sceneInVelW sortedVel.w carries its scene id.
+void launch_reorder(const unsigned *sortedKeys, const unsigned *sortedIdx, const float4 *pos, const float4 *vel,
+                    const int *rb, const uint16_t *scene, float4 *sortedPos, float4 *sortedVel,
+                    unsigned *cellStart, unsigned *cellEnd, int n, hipStream_t st);
+struct DemOut {
+    float4 *pForce;               // body particles: Σ F (original index)
+    float4 *pTorque;              // optional: relativePos × Σ F
+    float4 *vel;                  // grains: velocity after the contact impulse
+    const float4 *relPos;
+    int *pairs;                   // pair-emission mode when non-NULL
+    int *counts;
+    int maxPer;
+};
+void launch_dem_grid(const float4 *sortedPos, const float4 *sortedVel, const unsigned *sortedIdx,
+                     const unsigned *cellStart, const unsigned *cellEnd, int n, const DevPhys &ph,
+                     const SceneView &sv, bool sceneInVelW, const DemOut &o, int block, hipStream_t st);
+
+// ---------------------------------------------------------------- LBVH path (lbvh.hip)
+struct Lbvh {
+    int *left, *right, *parent, *minR, *maxR;
+    unsigned char *isLeaf;        // optional
+    float4 *bounds;               // 2 float4 per node: (min, 0), (max, 0)
+    float4 *cms;                  // leaf centres, w = rbIndex bits
+    float4 *sortedVel;            // leaf velocities, w = scene id bits when batched
+    int *counter;                 // N - 1 refit counters
+    int *sceneRoot;               // per scene (batched) or NULL
+};
+// keyBytes 4 or 8; sceneShift > 0: the scene id is key >> sceneShift and equal-key ties use the
+// scene-local leaf index (sceneStart[scene] is subtracted).
+void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShift, const int *sceneStart,
+                       const int *sceneCount, int numScenes, const Lbvh &t, hipStream_t st);
+void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *vel, const int *rb,
+                       const uint16_t *scene, float radius, int n, const Lbvh &t, hipStream_t st);
+// overflow: device int set to 1 when a traversal stack would exceed its LDS depth.
+void launch_dem_lbvh(const int *sortedIdx, const Lbvh &t, int n, const DevPhys &ph, const SceneView &sv,
+                     bool sceneInVelW, const DemOut &o, int block, hipStream_t st, int *overflow);
+
+}  // namespace arsim

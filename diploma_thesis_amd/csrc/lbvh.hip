@@ -1,0 +1,253 @@
+// lbvh.hip -- Karras LBVH: radix-tree build, leaf init + atomic bottom-up refit, and the
+// stack traversal with the DEM contact force.
+#include "kernels.hpp"
+
+namespace arsim {
+
+namespace {
+
+constexpr int kStackDepth = 32;   // per-thread LDS stack entries
+
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int X = 8;
+    int q = nb / X, r = nb % X;
+    int x = b % X, k = b / X;
+    return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ int clz_key(unsigned v) { return __clz((int)v); }
+__device__ __forceinline__ int clz_key(uint64_t v) { return __clzll((long long)v); }
+
+// longestCommonPrefix (BVHSource/BVHAuxiliary.cuh:31-43).  Equal keys always belong to one
+// scene (the scene id is in the high key bits), so the tie-break uses scene-local indices
+// li = i - base, lj = j - base and a batched subtree matches the standalone tree.
+template <class K>
+__device__ __forceinline__ int lcp(const K *__restrict__ codes, int n, K ki, int i, int j, int base) {
+    if (j < 0 || j >= n) return -1;
+    K kj = codes[j];
+    if (ki != kj) return clz_key((K)(ki ^ kj));
+    return (int)sizeof(K) * 8 + __clz((int)((unsigned)(i - base) ^ (unsigned)(j - base)));
+}
+
+// kernelConstructRadixTreeSoA (BVHSource/BVHSoAKernels.cu:7-99).  t = ceil(l / 2^k) is computed
+// with integer shifts instead of the reference's double division (identical for l < 2^31).
+template <class K>
+__global__ void k_karras(const K *__restrict__ codes, int n, int sceneShift, const int *__restrict__ sceneStart,
+                         const int *__restrict__ sceneCount, Lbvh t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) t.parent[n] = -1;
+    if (i >= n - 1) return;
+    t.counter[i] = 0;
+    const K ki = codes[i];
+    const int base = sceneShift > 0 ? sceneStart[(int)(ki >> sceneShift)] : 0;
+    const int dPrev = lcp(codes, n, ki, i, i - 1, base);
+    const int dNext = lcp(codes, n, ki, i, i + 1, base);
+    const int d = (dNext - dPrev) > 0 ? 1 : -1;
+    const int sigMin = lcp(codes, n, ki, i, i - d, base);
+    long long lmax = 2;
+    while (lcp(codes, n, ki, i, (int)(i + lmax * d), base) > sigMin) lmax *= 2;
+    long long l = 0;
+    for (long long s = lmax / 2; s >= 1; s /= 2) {
+        if (lcp(codes, n, ki, i, (int)(i + (l + s) * d), base) > sigMin) l += s;
+        if (s == 1) break;
+    }
+    const int j = (int)(i + l * d);
+    const int sigNode = lcp(codes, n, ki, i, j, base);
+    long long s = 0;
+    for (int k = 1;; ++k) {
+        long long tt = (l + (1ll << k) - 1) >> k;
+        if (lcp(codes, n, ki, i, (int)(i + (s + tt) * d), base) > sigNode) s += tt;
+        if (tt == 1) break;
+    }
+    const int gamma = (int)(i + s * d + (d < 0 ? d : 0));
+    const int node = i + n;
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const int L = (lo == gamma) ? gamma : gamma + n;
+    const int R = (hi == gamma + 1) ? gamma + 1 : gamma + 1 + n;
+    t.left[node] = L;
+    t.right[node] = R;
+    t.parent[L] = node;
+    t.parent[R] = node;
+    if (t.minR) { t.minR[node] = lo; t.maxR[node] = hi; }
+    if (t.isLeaf) t.isLeaf[node] = 0;
+    if (t.sceneRoot && sceneShift > 0) {
+        int sc = (int)(codes[lo] >> sceneShift);
+        if (lo == sceneStart[sc] && hi == sceneStart[sc] + sceneCount[sc] - 1) t.sceneRoot[sc] = node;
+    }
+}
+
+// kernelConstructLeafNodesSoA (BVHSoAKernels.cu:108-138) + kernelConstructInternalNodesSoA
+// (:148-176) in one launch: each leaf writes its centre (w = rbIndex bits), velocity copy and
+// AABB = c ± r, then walks up; the first arrival at a node stops, the second merges the
+// children.  The counter add is acquire-release at agent scope, so the sibling's bounds,
+// written before its add, are visible to the merging thread on any XCD.
+__global__ void k_leaf_refit(const int *__restrict__ sIdx, const float4 *__restrict__ pos,
+                             const float4 *__restrict__ vel, const int *__restrict__ rb,
+                             const uint16_t *__restrict__ scene, float r, int n, Lbvh t) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = sIdx[i];
+    const float4 p = pos[o];
+    t.cms[i] = make_float4(p.x, p.y, p.z, rb ? __int_as_float(rb[o]) : p.w);
+    if (t.sortedVel) {
+        float4 v = vel[o];
+        if (scene) v.w = __int_as_float((int)scene[o]);
+        t.sortedVel[i] = v;
+    }
+    t.bounds[2 * (size_t)i] = make_float4(p.x - r, p.y - r, p.z - r, 0.0f);
+    t.bounds[2 * (size_t)i + 1] = make_float4(p.x + r, p.y + r, p.z + r, 0.0f);
+    if (t.isLeaf) t.isLeaf[i] = 1;
+    if (t.minR) { t.minR[i] = i; t.maxR[i] = i; }
+    if (n < 2) return;
+    int node = t.parent[i];
+    int prev = __hip_atomic_fetch_add(&t.counter[node - n], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    while (prev == 1) {
+        const int L = t.left[node], R = t.right[node];
+        float4 aMin = t.bounds[2 * (size_t)L], aMax = t.bounds[2 * (size_t)L + 1];
+        float4 bMin = t.bounds[2 * (size_t)R], bMax = t.bounds[2 * (size_t)R + 1];
+        float4 mn = make_float4(aMin.x < bMin.x ? aMin.x : bMin.x, aMin.y < bMin.y ? aMin.y : bMin.y,
+                                aMin.z < bMin.z ? aMin.z : bMin.z, 0.0f);
+        float4 mx = make_float4(aMax.x > bMax.x ? aMax.x : bMax.x, aMax.y > bMax.y ? aMax.y : bMax.y,
+                                aMax.z > bMax.z ? aMax.z : bMax.z, 0.0f);
+        t.bounds[2 * (size_t)node] = mn;
+        t.bounds[2 * (size_t)node + 1] = mx;
+        const int up = t.parent[node];
+        if (up < 0) break;
+        node = up;
+        prev = __hip_atomic_fetch_add(&t.counter[node - n], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__device__ __forceinline__ bool overlap(float4 aMin, float4 aMax, float4 bMin, float4 bMax) {
+    // checkOverlap(AABB, AABB) (BVHAuxiliary.cuh:139-152)
+    return (aMin.x <= bMax.x && aMax.x >= bMin.x) && (aMin.y <= bMax.y && aMax.y >= bMin.y) &&
+           (aMin.z <= bMax.z && aMax.z >= bMin.z);
+}
+
+// collideBVHSoARigidBody (Kernels/RB_BVHKernels.cu:100-244): per sorted leaf, traversal from the
+// root (or the leaf's scene root), left child before right, the right child pushed when both
+// are entered.  Leaves are the nodes < n.  The stack lives in LDS, one column per thread.
+template <bool EMIT>
+__global__ void k_dem_lbvh(const int *__restrict__ sIdx, Lbvh t, int n, DevPhys ph, SceneView sv, int sceneInVelW,
+                           DemOut o, int *overflow) {
+    extern __shared__ int sStack[];
+    const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int i = lb * (int)blockDim.x + (int)threadIdx.x;
+    if (i >= n) return;
+    const int bd = (int)blockDim.x, tid = (int)threadIdx.x;
+    const float4 C = t.cms[i];
+    const float4 Vq = t.sortedVel[i];
+    const int ra = __float_as_int(C.w);
+    const int s = sceneInVelW ? __float_as_int(Vq.w) : 0;
+    const float4 qMin = t.bounds[2 * (size_t)i], qMax = t.bounds[2 * (size_t)i + 1];
+    const f3 pa = xyz(C), va = xyz(Vq);
+    const int orig = sIdx[i];
+    int node = t.sceneRoot ? t.sceneRoot[s] : (n >= 2 ? n : -1);
+    f3 F = mk3(0.f, 0.f, 0.f);
+    int cnt = 0, sp = 0;
+    while (node >= n) {
+        const int L = t.left[node], R = t.right[node];
+        const bool oL = L != i && overlap(qMin, qMax, t.bounds[2 * (size_t)L], t.bounds[2 * (size_t)L + 1]);
+        const bool oR = R != i && overlap(qMin, qMax, t.bounds[2 * (size_t)R], t.bounds[2 * (size_t)R + 1]);
+        const bool lL = L < n, lR = R < n;
+        if (oL && lL) {
+            float4 CL = t.cms[L];
+            if (pair_allowed(ra, __float_as_int(CL.w))) {
+                if (EMIT) {
+                    if (in_contact(pa, xyz(CL), ph.r)) {
+                        if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[L];
+                        ++cnt;
+                    }
+                } else {
+                    f3 fj;
+                    if (pair_force(pa, xyz(CL), va, xyz(t.sortedVel[L]), ph, fj)) F = add3(F, fj);
+                }
+            }
+        }
+        if (oR && lR) {
+            float4 CR = t.cms[R];
+            if (pair_allowed(ra, __float_as_int(CR.w))) {
+                if (EMIT) {
+                    if (in_contact(pa, xyz(CR), ph.r)) {
+                        if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[R];
+                        ++cnt;
+                    }
+                } else {
+                    f3 fj;
+                    if (pair_force(pa, xyz(CR), va, xyz(t.sortedVel[R]), ph, fj)) F = add3(F, fj);
+                }
+            }
+        }
+        const bool tL = oL && !lL, tR = oR && !lR;
+        if (!tL && !tR) {
+            if (sp == 0) break;
+            --sp;
+            node = sStack[sp * bd + tid];
+        } else {
+            node = tL ? L : R;
+            if (tL && tR) {
+                if (sp == kStackDepth) {
+                    atomicOr(overflow, 1);
+                    break;
+                }
+                sStack[sp * bd + tid] = R;
+                ++sp;
+            }
+        }
+    }
+    if (EMIT) {
+        o.counts[orig] = cnt;
+        return;
+    }
+    SceneGeom g;
+    load_geom(sv, s, g);
+    if (!((ph.flags & kGrainClamp) && ra < 0)) F = add3(F, wall_force(pa, va, ph, g));
+    if (ra >= 0) {
+        o.pForce[orig] = f4(F, 0.0f);
+        if (o.pTorque) o.pTorque[orig] = f4(cross3(xyz(o.relPos[orig]), F), 0.0f);
+    } else {
+        const float fs = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
+        o.vel[orig] = make_float4(va.x + F.x * fs, va.y + F.y * fs, va.z + F.z * fs, 0.0f);
+    }
+}
+
+}  // namespace
+
+void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShift, const int *sceneStart,
+                       const int *sceneCount, int numScenes, const Lbvh &t, hipStream_t st) {
+    (void)numScenes;
+    if (n <= 0) return;
+    const int block = 256;
+    int threads = n - 1 > 1 ? n - 1 : 1;
+    dim3 grid((threads + block - 1) / block);
+    if (keyBytes == 8)
+        hipLaunchKernelGGL(k_karras<uint64_t>, grid, dim3(block), 0, st, (const uint64_t *)sortedKeys, n, sceneShift,
+                           sceneStart, sceneCount, t);
+    else
+        hipLaunchKernelGGL(k_karras<unsigned>, grid, dim3(block), 0, st, (const unsigned *)sortedKeys, n, sceneShift,
+                           sceneStart, sceneCount, t);
+}
+
+void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *vel, const int *rb,
+                       const uint16_t *scene, float radius, int n, const Lbvh &t, hipStream_t st) {
+    if (n <= 0) return;
+    const int block = 256;
+    hipLaunchKernelGGL(k_leaf_refit, dim3((n + block - 1) / block), dim3(block), 0, st, sortedIdx, pos, vel, rb,
+                       scene, radius, n, t);
+}
+
+void launch_dem_lbvh(const int *sortedIdx, const Lbvh &t, int n, const DevPhys &ph, const SceneView &sv,
+                     bool sceneInVelW, const DemOut &o, int block, hipStream_t st, int *overflow) {
+    if (n <= 0) return;
+    if (block <= 0) block = 128;
+    dim3 grid((n + block - 1) / block);
+    size_t lds = (size_t)kStackDepth * block * sizeof(int);
+    if (o.pairs)
+        hipLaunchKernelGGL(k_dem_lbvh<true>, grid, dim3(block), lds, st, sortedIdx, t, n, ph, sv, sceneInVelW ? 1 : 0,
+                           o, overflow);
+    else
+        hipLaunchKernelGGL(k_dem_lbvh<false>, grid, dim3(block), lds, st, sortedIdx, t, n, ph, sv,
+                           sceneInVelW ? 1 : 0, o, overflow);
+}
+
+}  // namespace arsim

@@ -1,0 +1,324 @@
+// scenegen.cpp -- seeded synthetic scenes in the reference's particle / body layout.
+//
+// After addRigidBody / initObject (RigidBodies/RigidBodyAddition.cpp:20-228, 1486-1903) and
+// addNewSphere (Particles/sphereAddition.cpp:151-301): a body is a set of equal spheres with
+// offsets from its centre of mass, its inverse inertia is the inverse of the point-mass sum
+// Σ (|r|² I - r rᵀ) over its particles (RigidBodyAddition.cpp:1633-1643), body particles come
+// first in body order, free grains (rbIndex -1) follow.  Body mass = particle count.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/arsim.h"
+#include "host_common.hpp"
+
+struct ArSimSceneOwned {
+    int N = 0, B = 0;
+    std::vector<float> pos, vel, bodyFrame, rbPos, rbVel, rbAng, rbQuat, rbInertia, rbMass;
+    std::vector<int> rb, ppo;
+    ArSimParams params{};
+    ArSimStepConfig cfg{};
+};
+
+namespace {
+
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    uint64_t next() {   // SplitMix64
+        uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    }
+    double uni() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    double normal() {   // Box-Muller
+        double u1 = uni(), u2 = uni();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+struct BodyShape {
+    std::vector<double> off;   // xyz per particle, centred
+    double circum = 0.0;       // max |offset| + r
+};
+
+BodyShape cube_shape(int edge, double r) {
+    BodyShape s;
+    double h = (edge - 1) * 0.5;
+    for (int i = 0; i < edge; ++i)
+        for (int j = 0; j < edge; ++j)
+            for (int k = 0; k < edge; ++k) {
+                s.off.push_back((i - h) * 2.0 * r);
+                s.off.push_back((j - h) * 2.0 * r);
+                s.off.push_back((k - h) * 2.0 * r);
+            }
+    s.circum = std::sqrt(3.0) * h * 2.0 * r + r;
+    return s;
+}
+
+// Voxelised ellipsoid: lattice points (spacing 2r) inside (x/a)² + (y/b)² + (z/c)² <= 1.
+BodyShape ellipsoid_shape(double a, double b, double c, double r) {
+    BodyShape s;
+    double sp = 2.0 * r;
+    int na = (int)std::floor(a / sp), nb = (int)std::floor(b / sp), nc = (int)std::floor(c / sp);
+    double cx = 0, cy = 0, cz = 0;
+    for (int i = -na; i <= na; ++i)
+        for (int j = -nb; j <= nb; ++j)
+            for (int k = -nc; k <= nc; ++k) {
+                double x = i * sp, y = j * sp, z = k * sp;
+                if ((x * x) / (a * a) + (y * y) / (b * b) + (z * z) / (c * c) <= 1.0) {
+                    s.off.push_back(x); s.off.push_back(y); s.off.push_back(z);
+                    cx += x; cy += y; cz += z;
+                }
+            }
+    if (s.off.empty()) { s.off = {0.0, 0.0, 0.0}; }
+    size_t n = s.off.size() / 3;
+    cx /= n; cy /= n; cz /= n;
+    double m = 0;
+    for (size_t p = 0; p < n; ++p) {
+        s.off[3 * p] -= cx; s.off[3 * p + 1] -= cy; s.off[3 * p + 2] -= cz;
+        double d = std::sqrt(s.off[3 * p] * s.off[3 * p] + s.off[3 * p + 1] * s.off[3 * p + 1] +
+                             s.off[3 * p + 2] * s.off[3 * p + 2]);
+        m = std::max(m, d);
+    }
+    s.circum = m + r;
+    return s;
+}
+
+// 3x3 inverse (column-major), zero matrix when singular.
+void inv3(const double m[9], double out[9]) {
+    double a = m[0], b = m[3], c = m[6], d = m[1], e = m[4], f = m[7], g = m[2], h = m[5], i = m[8];
+    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    double det = a * A + b * B + c * C;
+    if (std::fabs(det) < 1e-300) { for (int k = 0; k < 9; ++k) out[k] = 0.0; return; }
+    double inv[9];
+    inv[0] = A / det; inv[3] = -(b * i - c * h) / det; inv[6] = (b * f - c * e) / det;
+    inv[1] = B / det; inv[4] = (a * i - c * g) / det;  inv[7] = -(a * f - c * d) / det;
+    inv[2] = C / det; inv[5] = -(a * h - b * g) / det; inv[8] = (a * e - b * d) / det;
+    for (int k = 0; k < 9; ++k) out[k] = inv[k];
+}
+
+void quat_to_mat(const double q[4], double R[9]) {   // q = (x,y,z,w), column-major
+    double x = q[0], y = q[1], z = q[2], w = q[3];
+    R[0] = 1 - 2 * (y * y + z * z); R[1] = 2 * (x * y + w * z);     R[2] = 2 * (x * z - w * y);
+    R[3] = 2 * (x * y - w * z);     R[4] = 1 - 2 * (x * x + z * z); R[5] = 2 * (y * z + w * x);
+    R[6] = 2 * (x * z + w * y);     R[7] = 2 * (y * z - w * x);     R[8] = 1 - 2 * (x * x + y * y);
+}
+
+struct CellKey {
+    size_t operator()(int64_t k) const { return (size_t)(k * 0x9e3779b97f4a7c15ull); }
+};
+
+}  // namespace
+
+extern "C" int arsim_generate_scene(const ArSimGenSpec *spec, ArSimSceneOwned **out) {
+    if (!spec || !out) return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "null argument");
+    if (spec->numCubes < 0 || spec->numGrains < 0 || spec->numEllipsoids < 0 || spec->radius <= 0.f)
+        return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "bad generator counts or radius");
+    for (int a = 0; a < 3; ++a)
+        if (spec->gridLog2[a] < 0 || spec->gridLog2[a] > 12)
+            return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "grid log2 out of range");
+    ArSimSceneOwned *s = new (std::nothrow) ArSimSceneOwned();
+    if (!s) return arsim_host::fail(ARSIM_ERR_OUT_OF_MEMORY, "host allocation failed");
+    Rng rng(spec->seed);
+    const double r = spec->radius;
+    const double box[3] = {spec->box[0], spec->box[1], spec->box[2]};
+
+    // ---- body shapes
+    std::vector<BodyShape> shapes;
+    int cubeEdge = spec->cubeEdge > 0 ? spec->cubeEdge : 4;
+    for (int b = 0; b < spec->numCubes; ++b) shapes.push_back(cube_shape(cubeEdge, r));
+    for (int b = 0; b < spec->numEllipsoids; ++b) {
+        double lo = spec->ellipsoidMinSemi, hi = spec->ellipsoidMaxSemi;
+        double a = lo + (hi - lo) * rng.uni(), bb = lo + (hi - lo) * rng.uni(), c = lo + (hi - lo) * rng.uni();
+        shapes.push_back(ellipsoid_shape(a, bb, c, r));
+    }
+    const int B = (int)shapes.size();
+
+    // ---- jittered lattice placement of bodies (no initial overlap)
+    double maxCirc = 0;
+    for (auto &sh : shapes) maxCirc = std::max(maxCirc, sh.circum);
+    std::vector<double> centres(3 * (size_t)B);
+    if (B > 0) {
+        int n = (int)std::ceil(std::cbrt((double)B) - 1e-9);
+        while ((long long)n * n * n < B) ++n;
+        double pitch[3];
+        int cells[3];
+        double hmax = spec->dropHeightMax > 0.f ? spec->dropHeightMax : box[1];
+        for (int a = 0; a < 3; ++a) {
+            double ext = (a == 1) ? std::min(box[1], hmax) : box[a];
+            cells[a] = n;
+            pitch[a] = ext / n;
+        }
+        // drop the vertical lattice to as few layers as the horizontal ones allow
+        while ((long long)cells[0] * cells[2] * (cells[1] - 1) >= B && cells[1] > 1) --cells[1];
+        pitch[1] = ((spec->dropHeightMax > 0.f) ? std::min(box[1], (double)spec->dropHeightMax) : box[1]) / cells[1];
+        double minPitch = std::min(pitch[0], std::min(pitch[1], pitch[2]));
+        if (minPitch < 2.0 * maxCirc) {
+            delete s;
+            return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "box too small for the bodies");
+        }
+        long long total = (long long)cells[0] * cells[1] * cells[2];
+        std::vector<long long> order(total);
+        for (long long k = 0; k < total; ++k) order[k] = k;
+        for (long long k = total - 1; k > 0; --k) {   // Fisher-Yates
+            long long j = (long long)(rng.uni() * (double)(k + 1));
+            if (j > k) j = k;
+            std::swap(order[k], order[j]);
+        }
+        for (int b = 0; b < B; ++b) {
+            long long c = order[b];
+            int cx = (int)(c % cells[0]), cy = (int)((c / cells[0]) % cells[1]), cz = (int)(c / ((long long)cells[0] * cells[1]));
+            int ci[3] = {cx, cy, cz};
+            for (int a = 0; a < 3; ++a) {
+                double slack = 0.5 * pitch[a] - shapes[b].circum;
+                double jit = slack > 0 ? (2.0 * rng.uni() - 1.0) * slack * 0.9 : 0.0;
+                centres[3 * b + a] = (ci[a] + 0.5) * pitch[a] + jit;
+            }
+        }
+    }
+
+    // ---- particles of bodies
+    std::vector<double> P;   // world positions (double) for overlap checks
+    for (int b = 0; b < B; ++b) {
+        // Shoemake uniform random rotation, (x,y,z,w)
+        double u1 = rng.uni(), u2 = rng.uni(), u3 = rng.uni();
+        double q[4] = {std::sqrt(1 - u1) * std::sin(6.283185307179586 * u2), std::sqrt(1 - u1) * std::cos(6.283185307179586 * u2),
+                       std::sqrt(u1) * std::sin(6.283185307179586 * u3), std::sqrt(u1) * std::cos(6.283185307179586 * u3)};
+        double R[9];
+        quat_to_mat(q, R);
+        double v[3] = {rng.normal() * spec->bodyVelSigma, rng.normal() * spec->bodyVelSigma, rng.normal() * spec->bodyVelSigma};
+        const BodyShape &sh = shapes[b];
+        size_t np = sh.off.size() / 3;
+        double I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (size_t p = 0; p < np; ++p) {
+            float fx = (float)sh.off[3 * p], fy = (float)sh.off[3 * p + 1], fz = (float)sh.off[3 * p + 2];
+            double x = fx, y = fy, z = fz;
+            I[0] += y * y + z * z; I[3] -= x * y; I[6] -= x * z;
+            I[1] -= x * y; I[4] += x * x + z * z; I[7] -= y * z;
+            I[2] -= x * z; I[5] -= y * z; I[8] += x * x + y * y;
+            double w[3] = {R[0] * x + R[3] * y + R[6] * z, R[1] * x + R[4] * y + R[7] * z, R[2] * x + R[5] * y + R[8] * z};
+            double wp[3] = {centres[3 * b] + w[0], centres[3 * b + 1] + w[1], centres[3 * b + 2] + w[2]};
+            s->pos.insert(s->pos.end(), {(float)wp[0], (float)wp[1], (float)wp[2], 0.f});
+            s->vel.insert(s->vel.end(), {(float)v[0], (float)v[1], (float)v[2], 0.f});
+            s->bodyFrame.insert(s->bodyFrame.end(), {fx, fy, fz, 0.f});
+            s->rb.push_back(b);
+            P.insert(P.end(), {wp[0], wp[1], wp[2]});
+        }
+        double Iinv[9];
+        inv3(I, Iinv);
+        for (int k = 0; k < 9; ++k) s->rbInertia.push_back((float)Iinv[k]);
+        s->rbPos.insert(s->rbPos.end(), {(float)centres[3 * b], (float)centres[3 * b + 1], (float)centres[3 * b + 2], 0.f});
+        s->rbVel.insert(s->rbVel.end(), {(float)v[0], (float)v[1], (float)v[2], 0.f});
+        s->rbAng.insert(s->rbAng.end(), {0.f, 0.f, 0.f, 0.f});
+        s->rbQuat.insert(s->rbQuat.end(), {(float)q[0], (float)q[1], (float)q[2], (float)q[3]});
+        s->rbMass.push_back((float)np);
+        s->ppo.push_back((int)np);
+    }
+
+    // ---- free grains: uniform in the box, rejected if closer than 2r to any particle
+    if (spec->numGrains > 0) {
+        const double cell = 2.0 * r;
+        std::unordered_map<int64_t, std::vector<int>, CellKey> grid;
+        grid.reserve((P.size() / 3 + spec->numGrains) * 2);
+        auto key = [&](int x, int y, int z) { return ((int64_t)(x + 1048576) << 42) | ((int64_t)(y + 1048576) << 21) | (int64_t)(z + 1048576); };
+        auto insert = [&](size_t idx) {
+            int cx = (int)std::floor(P[3 * idx] / cell), cy = (int)std::floor(P[3 * idx + 1] / cell), cz = (int)std::floor(P[3 * idx + 2] / cell);
+            grid[key(cx, cy, cz)].push_back((int)idx);
+        };
+        for (size_t k = 0; k < P.size() / 3; ++k) insert(k);
+        int placed = 0;
+        long long attempts = 0, maxAttempts = (long long)spec->numGrains * 200 + 1000;
+        while (placed < spec->numGrains && attempts < maxAttempts) {
+            ++attempts;
+            double x = r + (box[0] - 2 * r) * rng.uni();
+            double y = r + (box[1] - 2 * r) * rng.uni();
+            double z = r + (box[2] - 2 * r) * rng.uni();
+            int cx = (int)std::floor(x / cell), cy = (int)std::floor(y / cell), cz = (int)std::floor(z / cell);
+            bool ok = true;
+            for (int dz = -1; dz <= 1 && ok; ++dz)
+                for (int dy = -1; dy <= 1 && ok; ++dy)
+                    for (int dx = -1; dx <= 1 && ok; ++dx) {
+                        auto it = grid.find(key(cx + dx, cy + dy, cz + dz));
+                        if (it == grid.end()) continue;
+                        for (int j : it->second) {
+                            double ddx = P[3 * (size_t)j] - x, ddy = P[3 * (size_t)j + 1] - y, ddz = P[3 * (size_t)j + 2] - z;
+                            if (ddx * ddx + ddy * ddy + ddz * ddz < 4.0 * r * r * 1.0001) { ok = false; break; }
+                        }
+                    }
+            if (!ok) continue;
+            P.insert(P.end(), {x, y, z});
+            insert(P.size() / 3 - 1);
+            double vs = spec->grainVelSigma;
+            s->pos.insert(s->pos.end(), {(float)x, (float)y, (float)z, 0.f});
+            s->vel.insert(s->vel.end(), {(float)(rng.normal() * vs), (float)(rng.normal() * vs), (float)(rng.normal() * vs), 0.f});
+            s->bodyFrame.insert(s->bodyFrame.end(), {0.f, 0.f, 0.f, 0.f});
+            s->rb.push_back(-1);
+            ++placed;
+        }
+        if (placed < spec->numGrains) {
+            delete s;
+            return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "could not place all grains without overlap");
+        }
+    }
+
+    s->N = (int)s->rb.size();
+    s->B = B;
+    ArSimParams &p = s->params;
+    std::memset(&p, 0, sizeof(p));
+    p.gravity = {0.f, spec->gravityY, 0.f};
+    p.globalDamping = 1.0f;
+    p.particleRadius = spec->radius;
+    p.gridSize = {1u << spec->gridLog2[0], 1u << spec->gridLog2[1], 1u << spec->gridLog2[2]};
+    p.numCells = p.gridSize.x * p.gridSize.y * p.gridSize.z;
+    p.worldOrigin = {0.f, 0.f, 0.f};
+    p.cellSize = {2.0f * spec->radius, 2.0f * spec->radius, 2.0f * spec->radius};
+    p.numBodies = (unsigned)B;
+    p.spring = spec->spring;
+    p.damping = spec->damping;
+    p.shear = spec->shear;
+    p.attraction = 0.0f;
+    p.boundaryDamping = -0.5f;
+    p.Wdamping = 1.0f;
+    p.Vdamping = 1.0f;
+    ArSimStepConfig &c = s->cfg;
+    std::memset(&c, 0, sizeof(c));
+    c.dt = spec->dt;
+    for (int a = 0; a < 3; ++a) { c.boxMin[a] = 0.f; c.boxMax[a] = spec->box[a]; }
+    c.wallMask = ARSIM_WALLS_OPEN_TOP;
+    c.method = spec->method;
+    c.flags = 0;
+    *out = s;
+    return ARSIM_OK;
+}
+
+extern "C" int arsim_scene_view(const ArSimSceneOwned *s, ArSimSceneHost *v) {
+    if (!s || !v) return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "null argument");
+    v->numParticles = s->N;
+    v->numBodies = s->B;
+    v->pos = s->pos.data();
+    v->vel = s->vel.data();
+    v->rbIndices = s->rb.data();
+    v->bodyFramePos = s->bodyFrame.data();
+    v->rbPositions = s->rbPos.data();
+    v->rbVelocities = s->rbVel.data();
+    v->rbAngularVelocity = s->rbAng.data();
+    v->rbQuaternion = s->rbQuat.data();
+    v->rbInertia = s->rbInertia.data();
+    v->rbMass = s->rbMass.data();
+    v->particlesPerObject = s->ppo.data();
+    v->params = s->params;
+    v->cfg = s->cfg;
+    return ARSIM_OK;
+}
+
+extern "C" int arsim_scene_free(ArSimSceneOwned *s) {
+    delete s;
+    return ARSIM_OK;
+}

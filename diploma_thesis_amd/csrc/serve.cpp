@@ -1,0 +1,264 @@
+// serve.cpp -- continuous batching of independent scenes over a Poisson request stream.
+//
+// Requests arrive as scenes (config-0 generator: 4x4x4 cubes with 75 % of the particles plus
+// 25 % free grains, box scaled to the particle count).  Each iteration of the loop:
+//   1. retires scenes that finished their step count (their slots and bytes free up);
+//   2. admits queued arrivals while the slot limit and the HBM budget allow;
+//   3. if the resident set changed, rebuilds the batched engine by copying the surviving scenes'
+//      device state into the new layout (device-to-device) and uploading the new ones;
+//   4. advances every resident scene by one step in the batched launches.
+// Per-scene results do not depend on the batch they run in: every stage is scene-local and
+// order-preserving, which tests/ check by running a scene alone and in a mix.
+// The one-at-a-time mode serves the same stream FIFO on the plain path (one engine per scene).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "engine.hpp"
+#include "host_common.hpp"
+
+using arsim_host::fail;
+
+namespace {
+
+struct Req {
+    double arrival = 0;   // seconds (already divided by the time scale)
+    int n = 0, steps = 0;
+    uint64_t seed = 0;
+    int done = 0;
+    double admitted = -1, finished = -1;
+    size_t bytes = 0;
+    ArSimSceneOwned *scene = nullptr;
+};
+
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    }
+    double uni() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+double pct(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    double idx = q * (v.size() - 1);
+    size_t lo = (size_t)std::floor(idx), hi = (size_t)std::ceil(idx);
+    return v[lo] + (v[hi] - v[lo]) * (idx - lo);
+}
+
+// Bytes a scene of n particles and b bodies occupies in a batched grid engine.
+size_t scene_bytes(int n, int b, int method, int cellBits) {
+    size_t per = 16 * 5 + 4 + 2 + 8 * 2 + 4 * 2 + 16 * 2 + 64;   // particle state, keys, values, sorted copies, sort ws
+    if (method != ARSIM_METHOD_GRID) per += 2 * 32 * 2 + 16 * 2 + 4 * 6;
+    size_t body = 16 * 5 + 36 * 2 + 12;
+    size_t cells = method == ARSIM_METHOD_GRID ? ((size_t)8 << cellBits) : 0;
+    return (size_t)n * per + (size_t)b * body + cells;
+}
+
+}  // namespace
+
+// The config-0 generator scaled to n particles (used by serving and by the batched bench).
+extern "C" int arsim_generate_mix_scene(int n, uint64_t seed, int method, ArSimSceneOwned **out) {
+    ArSimGenSpec g{};
+    g.seed = seed;
+    g.cubeEdge = 4;
+    g.numCubes = (int)std::lround(0.75 * n / 64.0);
+    g.numGrains = n - 64 * g.numCubes;
+    if (g.numGrains < 0) { g.numCubes = n / 64; g.numGrains = n - 64 * g.numCubes; }
+    double L = 64.0 * std::cbrt((double)n / 16384.0);
+    L = std::max(L, 16.0);
+    for (int a = 0; a < 3; ++a) g.box[a] = (float)L;
+    g.radius = 0.5f;
+    g.bodyVelSigma = 0.5f;
+    g.grainVelSigma = 0.5f;
+    int lg = (int)std::ceil(std::log2(L));
+    lg = std::min(std::max(lg, 2), 6);
+    for (int a = 0; a < 3; ++a) g.gridLog2[a] = lg;
+    g.method = method;
+    g.dt = 1e-4f;
+    g.spring = 5e4f;
+    g.damping = 50.f;
+    g.shear = 10.f;
+    g.gravityY = -9.81f;
+    return arsim_generate_scene(&g, out);
+}
+
+extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *rep) {
+    using clk = std::chrono::steady_clock;
+    if (!cfg || !rep || cfg->numRequests <= 0 || cfg->arrivalRate <= 0 || cfg->minParticles <= 0 ||
+        cfg->maxParticles < cfg->minParticles || cfg->minSteps <= 0 || cfg->maxSteps < cfg->minSteps)
+        return fail(ARSIM_ERR_INVALID_ARG, "serve: bad configuration");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(ARSIM_ERR_NO_DEVICE, "no HIP device");
+    std::memset(rep, 0, sizeof(*rep));
+    const double scale = cfg->timeScale > 0 ? cfg->timeScale : 1.0;
+    const int method = cfg->method;
+
+    // ---- request stream
+    Rng rng(cfg->seed);
+    std::vector<Req> reqs(cfg->numRequests);
+    double t = 0;
+    const double lmin = std::log((double)cfg->minParticles), lmax = std::log((double)cfg->maxParticles);
+    for (int i = 0; i < cfg->numRequests; ++i) {
+        double u = rng.uni();
+        t += -std::log(1.0 - u) / cfg->arrivalRate;
+        reqs[i].arrival = t / scale;
+        reqs[i].n = (int)std::lround(std::exp(lmin + (lmax - lmin) * rng.uni()));
+        reqs[i].steps = cfg->minSteps + (int)(rng.uni() * (cfg->maxSteps - cfg->minSteps + 1));
+        if (reqs[i].steps > cfg->maxSteps) reqs[i].steps = cfg->maxSteps;
+        reqs[i].seed = rng.next();
+        reqs[i].bytes = scene_bytes(reqs[i].n, (int)(0.75 * reqs[i].n / 64), method, 18);
+    }
+
+    std::vector<double> stepLat, complLat, waitLat;
+    double updates = 0;
+    int completed = 0, maxResident = 0, rejected = 0;
+    double peakBytes = 0;
+    const auto t0 = clk::now();
+    auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    ArSimTuning tune{128, 16, 8, 0};
+
+    if (cfg->oneAtATime) {
+        for (auto &r : reqs) {
+            while (now() < r.arrival) std::this_thread::sleep_for(std::chrono::microseconds(200));
+            ArSimSceneOwned *sc = nullptr;
+            int rc = arsim_generate_mix_scene(r.n, r.seed, method, &sc);
+            if (rc) return rc;
+            ArSimSceneHost v;
+            arsim_scene_view(sc, &v);
+            r.admitted = now();
+            arsim::Engine eng;
+            std::vector<arsim::SceneSource> src(1);
+            src[0].host = &v;
+            ArSimTuning tg = tune;
+            tg.useGraph = 1;
+            rc = eng.build(src, &tg, 0, nullptr);
+            arsim_scene_free(sc);
+            if (rc) return rc;
+            double s0 = now();
+            if ((rc = eng.step(r.steps)) != 0) return rc;
+            hipStreamSynchronize(eng.st);
+            double s1 = now();
+            r.finished = s1;
+            for (int k = 0; k < r.steps; ++k) stepLat.push_back((s1 - s0) / r.steps * 1e3);
+            complLat.push_back((r.finished - r.arrival) * 1e3);
+            waitLat.push_back((r.admitted - r.arrival) * 1e3);
+            updates += (double)v.numParticles * r.steps;
+            peakBytes = std::max(peakBytes, (double)eng.devBytes);
+            ++completed;
+            maxResident = 1;
+        }
+    } else {
+        std::deque<int> pending;
+        for (int i = 0; i < cfg->numRequests; ++i) pending.push_back(i);
+        std::vector<int> resident;            // request ids, in engine scene order
+        std::unique_ptr<arsim::Engine> eng;
+        size_t residentBytes = 0;
+        std::vector<char> countedReject(cfg->numRequests, 0);
+        while (completed < cfg->numRequests) {
+            bool changed = false;
+            // retire
+            std::vector<int> keep;
+            for (int id : resident) {
+                Req &r = reqs[id];
+                if (r.done >= r.steps) {
+                    complLat.push_back((r.finished - r.arrival) * 1e3);
+                    residentBytes -= r.bytes;
+                    ++completed;
+                    changed = true;
+                } else {
+                    keep.push_back(id);
+                }
+            }
+            // admit
+            std::vector<int> admitted;
+            double tn = now();
+            while (!pending.empty() && reqs[pending.front()].arrival <= tn &&
+                   (int)(keep.size() + admitted.size()) < cfg->maxResident) {
+                Req &r = reqs[pending.front()];
+                if ((double)(residentBytes + r.bytes) > cfg->hbmBudgetBytes) {
+                    if (!countedReject[pending.front()]) { ++rejected; countedReject[pending.front()] = 1; }
+                    break;
+                }
+                int rc = arsim_generate_mix_scene(r.n, r.seed, method, &r.scene);
+                if (rc) return rc;
+                r.admitted = now();
+                waitLat.push_back((r.admitted - r.arrival) * 1e3);
+                residentBytes += r.bytes;
+                admitted.push_back(pending.front());
+                pending.pop_front();
+                changed = true;
+            }
+            if (keep.empty() && admitted.empty()) {
+                eng.reset();
+                resident.clear();
+                if (pending.empty()) break;
+                double wait = reqs[pending.front()].arrival - now();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::microseconds((long long)(wait * 1e6)));
+                continue;
+            }
+            if (changed) {
+                std::vector<arsim::SceneSource> src;
+                std::vector<ArSimSceneHost> views(admitted.size());
+                for (size_t k = 0; k < keep.size(); ++k) {
+                    arsim::SceneSource s;
+                    s.from = eng.get();
+                    s.fromScene = (int)(std::find(resident.begin(), resident.end(), keep[k]) - resident.begin());
+                    src.push_back(s);
+                }
+                for (size_t k = 0; k < admitted.size(); ++k) {
+                    arsim_scene_view(reqs[admitted[k]].scene, &views[k]);
+                    arsim::SceneSource s;
+                    s.host = &views[k];
+                    src.push_back(s);
+                }
+                std::unique_ptr<arsim::Engine> ne(new arsim::Engine());
+                int rc = ne->build(src, &tune, 0, nullptr);
+                for (int id : admitted) { arsim_scene_free(reqs[id].scene); reqs[id].scene = nullptr; }
+                if (rc) return rc;
+                if (eng) hipStreamSynchronize(eng->st);
+                eng = std::move(ne);
+                resident = keep;
+                resident.insert(resident.end(), admitted.begin(), admitted.end());
+                maxResident = std::max(maxResident, (int)resident.size());
+                peakBytes = std::max(peakBytes, (double)eng->devBytes);
+            }
+            double s0 = now();
+            int rc = eng->step(1);
+            if (rc) return rc;
+            hipStreamSynchronize(eng->st);
+            double s1 = now();
+            for (int id : resident) {
+                Req &r = reqs[id];
+                ++r.done;
+                updates += r.n > 0 ? (double)eng->pCount[&id - &resident[0]] : 0.0;
+                stepLat.push_back((s1 - s0) * 1e3);
+                if (r.done >= r.steps) r.finished = s1;
+            }
+        }
+    }
+    rep->completed = completed;
+    rep->wallSeconds = now();
+    rep->particleUpdates = updates;
+    rep->updatesPerSecond = updates / std::max(rep->wallSeconds, 1e-9);
+    rep->stepLatencyP50Ms = pct(stepLat, 0.50);
+    rep->stepLatencyP99Ms = pct(stepLat, 0.99);
+    rep->completionLatencyP50Ms = pct(complLat, 0.50);
+    rep->completionLatencyP99Ms = pct(complLat, 0.99);
+    rep->queueWaitP50Ms = pct(waitLat, 0.50);
+    rep->queueWaitP99Ms = pct(waitLat, 0.99);
+    rep->maxResidentSeen = maxResident;
+    rep->rejectedForBudget = rejected;
+    rep->peakDeviceBytes = peakBytes;
+    return ARSIM_OK;
+}

@@ -1,0 +1,116 @@
+"""Python handle on the C++ step engine (arsim_engine_*), the scene-level ParticleSystem::update API."""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as capi
+
+
+class Engine:
+    """S >= 1 scenes resident on the GPU; step() advances all of them (one launch per stage)."""
+
+    def __init__(self, scenes, dem_block=128, sort_ipt=16, sort_radix_bits=8, use_graph=True, device=0):
+        if not isinstance(scenes, (list, tuple)):
+            scenes = [scenes]
+        self.scenes = list(scenes)
+        self._views = (capi.ArSimSceneHost * len(scenes))(*[s.view() for s in scenes])
+        self.tuning = capi.ArSimTuning(dem_block, sort_ipt, sort_radix_bits, 1 if use_graph else 0)
+        self.h = C.c_void_p()
+        capi.check(capi.lib.arsim_engine_create(self._views, len(scenes), C.byref(self.tuning), device,
+                                                C.byref(self.h)), "engine_create")
+
+    def close(self):
+        if self.h:
+            capi.lib.arsim_engine_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, n=1):
+        capi.check(capi.lib.arsim_engine_step(self.h, int(n)), "engine_step")
+
+    def update(self, dt):
+        """ParticleSystem::update(dt) (Particles/particleSystem.h:288): one step of length dt."""
+        capi.check(capi.lib.arsim_engine_set_dt(self.h, float(dt)), "engine_set_dt")
+        self.step(1)
+
+    def sync(self):
+        capi.check(capi.lib.arsim_engine_synchronize(self.h), "engine_synchronize")
+
+    @property
+    def stream(self):
+        s = C.c_void_p()
+        capi.check(capi.lib.arsim_engine_stream(self.h, C.byref(s)))
+        return s
+
+    @property
+    def num_particles(self):
+        return capi.lib.arsim_engine_num_particles(self.h, -1)
+
+    @property
+    def device_bytes(self):
+        return capi.lib.arsim_engine_device_bytes(self.h)
+
+    def download(self, scene=0):
+        s = self.scenes[scene]
+        n, b = s.n, s.b
+        out = dict(pos=np.zeros((n, 4), np.float32), vel=np.zeros((n, 4), np.float32),
+                   rel=np.zeros((n, 4), np.float32), force=np.zeros((n, 4), np.float32),
+                   torque=np.zeros((n, 4), np.float32), rb_pos=np.zeros((b, 4), np.float32),
+                   rb_vel=np.zeros((b, 4), np.float32), rb_ang=np.zeros((b, 4), np.float32),
+                   rb_quat=np.zeros((b, 4), np.float32), rb_iw=np.zeros((b, 9), np.float32))
+        P = capi.ptr
+        capi.check(capi.lib.arsim_engine_download(self.h, scene, P(out["pos"]), P(out["vel"]), P(out["rel"]),
+                                                  P(out["force"]), P(out["torque"]), P(out["rb_pos"]),
+                                                  P(out["rb_vel"]), P(out["rb_ang"]), P(out["rb_quat"]),
+                                                  P(out["rb_iw"])), "engine_download")
+        return out
+
+    def profile(self):
+        ms = (C.c_float * 16)()
+        names = (C.c_char_p * 16)()
+        k = capi.lib.arsim_engine_profile_step(self.h, ms, names, 16)
+        if k < 0 or k > 16:
+            capi.check(k, "engine_profile_step")
+        return [(names[i].decode(), ms[i]) for i in range(k)]
+
+    def contact_pairs(self, max_per=64):
+        n = self.num_particles
+        pairs = np.zeros((n, max_per), np.int32)
+        counts = np.zeros(n, np.int32)
+        capi.check(capi.lib.arsim_engine_contact_pairs(self.h, capi.ptr(pairs), capi.ptr(counts), max_per),
+                   "engine_contact_pairs")
+        return pairs, counts
+
+    def check_finite(self):
+        flag = C.c_int(0)
+        rc = capi.lib.arsim_engine_check_finite(self.h, C.byref(flag))
+        return rc == capi.ARSIM_OK and flag.value == 0
+
+    def autotune(self, steps_per_trial=5):
+        best = capi.ArSimTuning()
+        ms = C.c_float(0)
+        capi.check(capi.lib.arsim_engine_autotune(self.h, steps_per_trial, C.byref(best), C.byref(ms)),
+                   "engine_autotune")
+        self.tuning = best
+        return (best.demBlock, best.sortItemsPerThread, best.sortRadixBits), ms.value
+
+
+def contact_set(pairs, counts, offset=0):
+    """Set of unordered pairs (i < j) from an (N, max_per) contact list."""
+    out = set()
+    for i in np.nonzero(counts)[0]:
+        for j in pairs[i, :counts[i]]:
+            a, b = int(i) + offset, int(j) + offset
+            out.add((a, b) if a < b else (b, a))
+    return out

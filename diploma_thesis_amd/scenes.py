@@ -1,0 +1,146 @@
+"""Seeded synthetic scenes (host arrays) for the configurations the pipeline is measured on.
+
+The generator itself is C++ (arsim_generate_scene / arsim_generate_mix_scene); this module
+wraps it and copies the result into numpy arrays in the reference layout.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _capi as capi
+
+
+class HostScene:
+    """Host-side scene arrays (numpy) plus SimParams / step config; owns its memory."""
+
+    def __init__(self, view, keep=None):
+        n, b = view.numParticles, view.numBodies
+
+        def arr(p, count, dtype):
+            if count == 0 or not p:
+                return np.zeros(0, dtype)
+            buf = (C.c_char * (count * np.dtype(dtype).itemsize)).from_address(p)
+            return np.frombuffer(buf, dtype=dtype, count=count).copy()
+
+        self.pos = arr(view.pos, 4 * n, np.float32).reshape(n, 4)
+        self.vel = arr(view.vel, 4 * n, np.float32).reshape(n, 4)
+        self.rb = arr(view.rbIndices, n, np.int32)
+        self.body_frame = arr(view.bodyFramePos, 4 * n, np.float32).reshape(n, 4)
+        self.rb_pos = arr(view.rbPositions, 4 * b, np.float32).reshape(b, 4)
+        self.rb_vel = arr(view.rbVelocities, 4 * b, np.float32).reshape(b, 4)
+        self.rb_ang = arr(view.rbAngularVelocity, 4 * b, np.float32).reshape(b, 4)
+        self.rb_quat = arr(view.rbQuaternion, 4 * b, np.float32).reshape(b, 4)
+        self.rb_inertia = arr(view.rbInertia, 9 * b, np.float32).reshape(b, 9)
+        self.rb_mass = arr(view.rbMass, b, np.float32)
+        self.ppo = arr(view.particlesPerObject, b, np.int32)
+        self.params = capi.ArSimParams.from_buffer_copy(view.params)
+        self.cfg = capi.ArSimStepConfig.from_buffer_copy(view.cfg)
+
+    @property
+    def n(self):
+        return int(self.pos.shape[0])
+
+    @property
+    def b(self):
+        return int(self.rb_pos.shape[0])
+
+    @property
+    def pstart(self):
+        return (np.cumsum(self.ppo) - self.ppo).astype(np.int32)
+
+    def view(self):
+        """An ArSimSceneHost pointing at this object's arrays (valid while self lives)."""
+        v = capi.ArSimSceneHost()
+        v.numParticles, v.numBodies = self.n, self.b
+        v.pos, v.vel = capi.ptr(self.pos), capi.ptr(self.vel)
+        v.rbIndices, v.bodyFramePos = capi.ptr(self.rb), capi.ptr(self.body_frame)
+        v.rbPositions, v.rbVelocities = capi.ptr(self.rb_pos), capi.ptr(self.rb_vel)
+        v.rbAngularVelocity, v.rbQuaternion = capi.ptr(self.rb_ang), capi.ptr(self.rb_quat)
+        v.rbInertia, v.rbMass = capi.ptr(self.rb_inertia), capi.ptr(self.rb_mass)
+        v.particlesPerObject = capi.ptr(self.ppo)
+        v.params, v.cfg = self.params, self.cfg
+        return v
+
+    def copy(self):
+        other = object.__new__(HostScene)
+        for k, v in self.__dict__.items():
+            if isinstance(v, np.ndarray):
+                setattr(other, k, v.copy())
+            elif isinstance(v, (capi.ArSimParams, capi.ArSimStepConfig)):
+                setattr(other, k, type(v).from_buffer_copy(v))
+            else:
+                setattr(other, k, v)
+        return other
+
+
+def _from_owned(owned):
+    view = capi.ArSimSceneHost()
+    capi.check(capi.lib.arsim_scene_view(owned, C.byref(view)), "scene_view")
+    s = HostScene(view)
+    capi.lib.arsim_scene_free(owned)
+    return s
+
+
+def generate(spec):
+    owned = C.c_void_p()
+    capi.check(capi.lib.arsim_generate_scene(C.byref(spec), C.byref(owned)), "generate_scene")
+    return _from_owned(owned)
+
+
+def mix_scene(n, seed, method=capi.METHOD_GRID):
+    owned = C.c_void_p()
+    capi.check(capi.lib.arsim_generate_mix_scene(int(n), C.c_uint64(seed), int(method), C.byref(owned)),
+               "generate_mix_scene")
+    return _from_owned(owned)
+
+
+def _spec(seed=0, cubes=0, grains=0, box=(64.0, 64.0, 64.0), grid_log2=(6, 6, 6), method=capi.METHOD_GRID,
+          body_sigma=0.5, grain_sigma=0.5, ellipsoids=0, semi=(1.0, 4.0), drop=0.0):
+    g = capi.ArSimGenSpec()
+    g.seed = seed
+    g.numCubes, g.cubeEdge, g.numEllipsoids = cubes, 4, ellipsoids
+    g.ellipsoidMinSemi, g.ellipsoidMaxSemi = semi
+    g.numGrains = grains
+    for a in range(3):
+        g.box[a] = box[a]
+        g.gridLog2[a] = grid_log2[a]
+    g.radius = 0.5
+    g.bodyVelSigma, g.grainVelSigma = body_sigma, grain_sigma
+    g.dropHeightMax = drop
+    g.method = method
+    g.dt, g.spring, g.damping, g.shear, g.gravityY = 1e-4, 5e4, 50.0, 10.0, -9.81
+    return g
+
+
+def config0(method=capi.METHOD_GRID, seed=0):
+    """256 cubes of 4x4x4 (16,384 particles), r = 0.5, [0,64]^3, grid 64^3, v ~ N(0, 0.5)."""
+    return generate(_spec(seed=seed, cubes=256, method=method))
+
+
+def config1(method=capi.METHOD_GRID, seed=0):
+    """8,192 cubes (524,288 particles) + 524,288 grains in [0,512]^3, grid 256^3 hashed, v ~ N(0,1)."""
+    return generate(_spec(seed=seed, cubes=8192, grains=524288, box=(512.0, 512.0, 512.0), grid_log2=(8, 8, 8),
+                          method=method, body_sigma=1.0, grain_sigma=1.0))
+
+
+def config2(method=capi.METHOD_GRID, seed=0, scale=1.0):
+    """Dense pile: 16,384 voxelised ellipsoids (semi-axes U[2r, 8r]) plus grains to ~4.19 M particles,
+    dropped from heights in [0, 200], v = 0.  scale < 1 shrinks counts and box for tests."""
+    nb = max(1, int(16384 * scale))
+    box = 256.0 * scale ** (1.0 / 3.0)
+    spec = _spec(seed=seed, ellipsoids=nb, semi=(1.0, 4.0), box=(box, 200.0 * max(scale, 0.05) ** (1 / 3), box),
+                 grid_log2=(8, 8, 8), method=method, body_sigma=0.0, grain_sigma=0.0, drop=200.0)
+    probe = generate(spec)
+    target = int(4194304 * scale)
+    grains = max(0, target - probe.n)
+    spec.box[1] = max(spec.box[1], 200.0)
+    spec.numGrains = grains
+    return generate(spec)
+
+
+def batch_mix(num_scenes=512, lo=1024, hi=65536, seed=0, method=capi.METHOD_GRID):
+    """Scenes with particle counts log-uniform in [lo, hi], each from the small-scene generator."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(lo), math.log(hi), size=num_scenes)).round().astype(int)
+    return [mix_scene(int(n), seed * 100003 + k, method) for k, n in enumerate(counts)]

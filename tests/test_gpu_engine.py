@@ -1,0 +1,153 @@
+"""Full-step parity of the engine (scene-level update) against the oracle, determinism, hipGraph
+replay, and batch invariance (a scene gives the same bits alone and inside a batch)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from diploma_thesis_amd import Engine, capi, contact_set, scenes
+from gpu_util import rel_err_vs_scale
+
+pytestmark = pytest.mark.gpu
+
+STATE_KEYS = [("pos", "pos"), ("vel", "vel"), ("rel", "rel"), ("rb_pos", "rb_pos"), ("rb_vel", "rb_vel"),
+              ("rb_ang", "rb_ang"), ("rb_quat", "qcum"), ("rb_iw", "Iw")]
+
+
+def assert_state_equal(gpu, st, what=""):
+    for gk, ok in STATE_KEYS:
+        a, b = gpu[gk], getattr(st, ok)
+        if not np.array_equal(a, b):
+            bad = np.argwhere(a != b)
+            raise AssertionError(f"{what}: {gk} differs at {bad[:5].tolist()} ({len(bad)} entries): "
+                                 f"{a[tuple(bad[0])]} vs {b[tuple(bad[0])]}")
+
+
+@pytest.mark.parametrize("method", [capi.METHOD_GRID, capi.METHOD_LBVH30, capi.METHOD_LBVH63])
+def test_one_step_bit_exact_vs_fp32_mirror(method):
+    print(f"engine: one step method={method}", flush=True)
+    sc = scenes.config0(method)
+    with Engine(sc, use_graph=False) as e:
+        e.step(1)
+        e.sync()
+        g = e.download()
+    st = oracle.State(sc, np.float32)
+    st.step(1)
+    assert_state_equal(g, st, f"method {method}")
+
+
+@pytest.mark.parametrize("method", [capi.METHOD_GRID, capi.METHOD_LBVH30])
+def test_multi_step_with_contacts_vs_mirror_and_fp64(method):
+    print(f"engine: 30 steps dense method={method}", flush=True)
+    sc = scenes.mix_scene(12000, 21, method)
+    # compress the scene towards its centre so bodies and grains start overlapping
+    c = sc.rb_pos[:, :3].mean(0)
+    sc.rb_pos[:, :3] = c + (sc.rb_pos[:, :3] - c) * np.float32(0.62)
+    g_idx = sc.rb < 0
+    sc.pos[g_idx, :3] = c + (sc.pos[g_idx, :3] - c) * np.float32(0.62)
+    with Engine(sc, use_graph=True) as e:
+        e.step(30)
+        e.sync()
+        g = e.download()
+    st = oracle.State(sc, np.float32)
+    st.step(30)
+    assert_state_equal(g, st, "30 steps")
+    s64 = oracle.State(sc, np.float64)
+    s64.step(30)
+    assert rel_err_vs_scale(g["rb_vel"][:, :3], s64.rb_vel[:, :3], 1.0) < 1e-3
+    assert np.max(np.abs(g["pos"][:, :3] - s64.pos[:, :3])) < 1e-3
+
+
+def test_hundred_step_drift_report():
+    print("engine: 100-step drift", flush=True)
+    sc = scenes.config0()
+    with Engine(sc) as e:
+        e.step(100)
+        e.sync()
+        g = e.download()
+        assert e.check_finite()
+    s64 = oracle.State(sc, np.float64)
+    s64.step(100)
+    dpos = float(np.max(np.abs(g["pos"][:, :3] - s64.pos[:, :3])))
+    dvel = float(np.max(np.abs(g["rb_vel"][:, :3] - s64.rb_vel[:, :3])))
+    print(f"100-step drift vs FP64: max |dx| = {dpos:.3e}, max |dv_body| = {dvel:.3e}", flush=True)
+    assert dpos < 1e-3 and dvel < 1e-3
+
+
+@pytest.mark.parametrize("method", [capi.METHOD_GRID, capi.METHOD_LBVH30])
+def test_deterministic_and_graph_equals_eager(method):
+    print(f"engine: determinism method={method}", flush=True)
+    sc = scenes.mix_scene(20000, 9, method)
+    outs = []
+    for graph in (False, True, True):
+        with Engine(sc, use_graph=graph) as e:
+            for _ in range(4):
+                e.step(3)
+                e.sync()          # eager work and syncs between graph replays
+            outs.append(e.download())
+    for o in outs[1:]:
+        for k in outs[0]:
+            assert np.array_equal(o[k], outs[0][k]), k
+
+
+@pytest.mark.parametrize("method", [capi.METHOD_GRID, capi.METHOD_LBVH30])
+def test_batched_scenes_equal_alone(method):
+    print(f"engine: batch invariance method={method}", flush=True)
+    mix = [scenes.mix_scene(n, 100 + k, method) for k, n in enumerate([1500, 7000, 2048, 3333])]
+    with Engine(mix, use_graph=True) as e:
+        e.step(8)
+        e.sync()
+        batched = [e.download(k) for k in range(len(mix))]
+        pairs, counts = e.contact_pairs()
+    offsets = np.cumsum([0] + [s.n for s in mix])
+    for k, sc in enumerate(mix):
+        with Engine(sc, use_graph=False) as e1:
+            e1.step(8)
+            e1.sync()
+            alone = e1.download()
+            p1, c1 = e1.contact_pairs()
+        for key in alone:
+            assert np.array_equal(batched[k][key], alone[key]), (k, key)
+        lo, hi = offsets[k], offsets[k + 1]
+        sub = contact_set(pairs[lo:hi] - lo, counts[lo:hi])
+        assert sub == contact_set(p1, c1)
+
+
+def test_engine_contact_pairs_match_brute():
+    print("engine: contact pairs", flush=True)
+    sc = scenes.config0()
+    with Engine(sc) as e:
+        e.step(5)
+        e.sync()
+        g = e.download()
+        pairs, counts = e.contact_pairs()
+    assert contact_set(pairs, counts) == oracle.contacts_brute(g["pos"], sc.rb, 0.5)
+
+
+def test_profile_and_autotune_leave_state_consistent():
+    print("engine: profile/autotune", flush=True)
+    sc = scenes.mix_scene(30000, 4)
+    with Engine(sc) as e:
+        e.step(2)
+        stages = e.profile()
+        assert [n for n, _ in stages][0] == "body_integrate" and all(t >= 0 for _, t in stages)
+        before = e.download()
+        best, ms = e.autotune(steps_per_trial=2)
+        after = e.download()
+        for k in before:
+            assert np.array_equal(before[k], after[k]), k
+        assert ms > 0
+        e.step(2)
+        e.sync()
+
+
+def test_serve_small_stream():
+    print("serve: small stream", flush=True)
+    for one in (0, 1):
+        cfg = capi.ArSimServeConfig(seed=1, numRequests=6, arrivalRate=200.0, minParticles=2048, maxParticles=6000,
+                                    minSteps=5, maxSteps=20, hbmBudgetBytes=8e9, maxResident=64, oneAtATime=one,
+                                    timeScale=1.0, method=capi.METHOD_GRID)
+        rep = capi.ArSimServeReport()
+        capi.check(capi.lib.arsim_serve_run(C.byref(cfg), C.byref(rep)), "serve_run")
+        assert rep.completed == 6 and rep.particleUpdates > 0 and rep.stepLatencyP99Ms >= rep.stepLatencyP50Ms
