@@ -1,0 +1,190 @@
+"""Benchmark of the gfx950 per-step DEM pipeline.
+
+Headline line (value): particle-updates/s of the plain one-scene path on the large single scene
+(2^20 particles: 8,192 rigid 4x4x4 cubes + 524,288 grains in [0,512]^3, hashed 256^3 grid), with
+inputs resident in HBM.  Beside it on the same JSON line:
+  - p50 / p99 per-step latency of that scene (HIP events, one step each);
+  - roofline of the dominant kernel (algorithmic bytes / live HIP-event duration vs 8 TB/s HBM);
+  - cpu_baseline: the oracle's serial FP32 restatement on a bounded sample of the same scene;
+  - batched: 512 scenes log-uniform in [1024, 65536] particles stepped in one launch per stage,
+    autotuned, vs the plain path looping over the same scenes.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic bytes per particle (or per body) of each stage, compulsory traffic in the reference
+# layout with neighbour re-reads assumed cached
+ALGO_BYTES = {
+    "dem_grid": 120.0,          # sorted pos+vel 32, index 4, rbIndex 4, relativePos 16, pForce/pTorque RMW 64
+    "dem_lbvh": 120.0,
+    "reorder_cellstart": 72.0,  # hash 4, index 4, gathered pos+vel 32, sorted pos+vel 32
+    "pose_key": 68.0 + 24.0,    # body integrate + pose 68, hash 24
+    "reduce_apply": 32.0,
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--method", choices=["grid", "lbvh30", "lbvh63"], default="grid")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-scenes", type=int, default=512)
+    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--cpu-sample-steps", type=int, default=0, help="0: choose ~15 s of CPU work")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this pipeline runs on one GPU; multi-GPU runs are not part of this build "
+                                   "(independent scenes shard across GPUs with no exchange)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    from diploma_thesis_amd import Engine, capi, scenes
+
+    method = {"grid": capi.METHOD_GRID, "lbvh30": capi.METHOD_LBVH30, "lbvh63": capi.METHOD_LBVH63}[args.method]
+    log("generating the 2^20-particle scene")
+    sc = scenes.config1(method)
+    n = sc.n
+    eng = Engine(sc, use_graph=True)
+    log(f"engine ready: {n} particles, {sc.b} bodies, {eng.device_bytes / 2**20:.0f} MiB on device")
+    eng.step(args.warmup)
+    eng.sync()
+
+    # ---- timed region: K steps, inputs resident in HBM
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(args.steps)
+    eng.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = n * args.steps / (t1 - t0)
+    log(f"plain: {ms_per_step:.4f} ms/step, {value:.4e} particle-updates/s")
+    if not eng.check_finite():
+        log("WARNING: non-finite state after the timed steps")
+
+    # ---- per-step latency (one step per event pair) and per-stage times (HIP events on the engine stream)
+    lat = []
+    for _ in range(50):
+        s0 = time.perf_counter()
+        eng.step(1)
+        eng.sync()
+        lat.append((time.perf_counter() - s0) * 1e3)
+    stage_ms = {}
+    reps = 10
+    for _ in range(reps):
+        for name, ms in eng.profile():
+            stage_ms[name] = stage_ms.get(name, 0.0) + ms / reps
+    dom = max((k for k in stage_ms if k != "sort"), key=lambda k: stage_ms[k])
+    per_unit = ALGO_BYTES.get(dom, 0.0)
+    achieved = per_unit * n / (stage_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", f"{dom}_c1_pmc.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "bytes_per_particle": per_unit, "kernel_ms": round(stage_ms[dom], 5)}
+    stage_summary = {k: round(v, 5) for k, v in stage_ms.items()}
+    eng.close()
+
+    # ---- CPU baseline: the oracle's serial FP32 restatement of the same step on the same scene
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        st = oracle.State(sc, np.float32)
+        t0 = time.perf_counter()
+        st.step(1)
+        one = time.perf_counter() - t0
+        k = args.cpu_sample_steps or max(1, min(20, int(15.0 / max(one, 1e-3))))
+        t0 = time.perf_counter()
+        st.step(k)
+        sec = time.perf_counter() - t0
+        cpu = {"value": n * k / sec, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{k} steps of the 2^20-particle scene after 1 untimed step, oracle FP32 serial C++ "
+                         f"(g++ -O2), {sec:.1f} s"}
+        log(f"cpu baseline: {cpu['value']:.3e} particle-updates/s over {k} steps")
+
+    # ---- batched form vs the plain path looping over scenes
+    batched = None
+    if not args.no_batched:
+        log(f"generating {args.batched_scenes} scenes")
+        mix = scenes.batch_mix(args.batched_scenes, seed=0, method=capi.METHOD_GRID)
+        total = sum(s.n for s in mix)
+        be = Engine(mix, use_graph=True)
+        best, best_ms = be.autotune(steps_per_trial=3)
+        be.close()
+        be = Engine(mix, dem_block=best[0], sort_ipt=best[1], sort_radix_bits=best[2], use_graph=True)
+        be.step(3)
+        be.sync()
+        t0 = time.perf_counter()
+        be.step(args.batched_steps)
+        be.sync()
+        bsec = time.perf_counter() - t0
+        blat = []
+        for _ in range(20):
+            s0 = time.perf_counter()
+            be.step(1)
+            be.sync()
+            blat.append((time.perf_counter() - s0) * 1e3)
+        be.close()
+        plain = [Engine(s, use_graph=True) for s in mix]
+        for e in plain:
+            e.step(2)
+        for e in plain:
+            e.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.batched_steps):
+            for e in plain:
+                e.step(1)
+        for e in plain:
+            e.sync()
+        psec = time.perf_counter() - t0
+        for e in plain:
+            e.close()
+        bv, pv = total * args.batched_steps / bsec, total * args.batched_steps / psec
+        batched = {"scenes": len(mix), "particles": total, "steps": args.batched_steps,
+                   "batched_updates_per_s": bv, "plain_loop_updates_per_s": pv, "speedup": bv / pv,
+                   "batched_step_ms_p50": float(np.percentile(blat, 50)),
+                   "batched_step_ms_p99": float(np.percentile(blat, 99)),
+                   "tuning": {"dem_block": best[0], "sort_items_per_thread": best[1], "sort_radix_bits": best[2]}}
+        log(f"batched: {bv:.3e} vs plain loop {pv:.3e} particle-updates/s ({bv / pv:.1f}x)")
+
+    line = {
+        "metric": "particle-updates/s batched vs plain on 1 GPU; p50/p99 per-scene step latency",
+        "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain single large scene: 2^20 particles (8,192 rigid 4x4x4 cubes + 524,288 "
+                               "grains) in [0,512]^3, r=0.5, k=5e4, c=50, shear=10, dt=1e-4, 5 walls",
+                   "method": args.method, "grid": "256^3 hashed" if method == capi.METHOD_GRID else None,
+                   "particles": n, "bodies": sc.b},
+        "step_latency_ms_p50": float(np.percentile(lat, 50)), "step_latency_ms_p99": float(np.percentile(lat, 99)),
+        "stage_ms": stage_summary,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "batched": batched,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

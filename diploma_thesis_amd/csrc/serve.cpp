@@ -238,10 +238,10 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
             if (rc) return rc;
             hipStreamSynchronize(eng->st);
             double s1 = now();
-            for (int id : resident) {
-                Req &r = reqs[id];
+            for (size_t k = 0; k < resident.size(); ++k) {
+                Req &r = reqs[resident[k]];
                 ++r.done;
-                updates += r.n > 0 ? (double)eng->pCount[&id - &resident[0]] : 0.0;
+                updates += (double)eng->pCount[k];
                 stepLat.push_back((s1 - s0) * 1e3);
                 if (r.done >= r.steps) r.finished = s1;
             }

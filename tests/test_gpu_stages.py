@@ -19,10 +19,17 @@ pytestmark = pytest.mark.gpu
 L = capi.lib
 
 
-def jittered_scene(n=None, seed=0, sigma=0.25):
-    """Config-0 scene with its particles jittered so that many pairs overlap."""
+def jittered_scene(n=None, seed=0, sigma=0.25, squeeze=0.55):
+    """Small-scene generator output squeezed towards its centre and jittered, so that many pairs
+    (grain-grain, grain-body, body-body) overlap; positions are used as they are."""
     sc = scenes.config0() if n is None else scenes.mix_scene(n, seed)
     rng = np.random.default_rng(seed)
+    c = sc.pos[:, :3].mean(0)
+    old = sc.rb_pos[:, :3].copy()
+    sc.rb_pos[:, :3] = c + (old - c) * np.float32(squeeze)       # bodies move closer, keep their shape
+    body = sc.rb >= 0
+    sc.pos[body, :3] += sc.rb_pos[sc.rb[body], :3] - old[sc.rb[body]]
+    sc.pos[~body, :3] = c + (sc.pos[~body, :3] - c) * np.float32(squeeze)
     sc.pos[:, :3] += rng.normal(scale=sigma, size=(sc.n, 3)).astype(np.float32)
     sc.vel[:, :3] = rng.normal(scale=0.5, size=(sc.n, 3)).astype(np.float32)
     return sc
