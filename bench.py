@@ -44,7 +44,50 @@ def parse():
     ap.add_argument("--batched-scenes", type=int, default=512)
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--cpu-sample-steps", type=int, default=0, help="0: choose ~15 s of CPU work")
+    ap.add_argument("--quick", action="store_true",
+                    help="timed steps only (no latency / stage loops, no CPU baseline, no batched run): for profilers")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a plain 256 MiB device copy kernel (3 launches) to calibrate HBM byte counters, then exit")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only step the 512-scene batch (default tuning, no autotune): for profilers")
     return ap.parse_args()
+
+
+def batched_only(args):
+    from diploma_thesis_amd import Engine, capi, scenes
+    mix = scenes.batch_mix(args.batched_scenes, seed=0, method=capi.METHOD_GRID)
+    total = sum(s.n for s in mix)
+    be = Engine(mix, use_graph=True)
+    be.step(args.warmup)
+    be.sync()
+    t0 = time.perf_counter()
+    be.step(args.steps)
+    be.sync()
+    sec = time.perf_counter() - t0
+    stage_ms = {}
+    for _ in range(3):
+        for name, ms in be.profile():
+            stage_ms[name] = stage_ms.get(name, 0.0) + ms / 3
+    be.close()
+    print(json.dumps({"batched_only": True, "particles": total, "steps": args.steps,
+                      "updates_per_s": total * args.steps / sec, "ms_per_step": sec * 1e3 / args.steps,
+                      "stage_ms": stage_ms}))
+
+
+def calibrate_copy():
+    """A 4-byte-per-element copy kernel of known size: 64 Mi elements, 256 MiB read + 256 MiB written."""
+    from diploma_thesis_amd import capi
+    n = 64 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    capi.check(capi.lib.arsim_malloc(C.byref(a), 4 * n))
+    capi.check(capi.lib.arsim_malloc(C.byref(b), 4 * n))
+    capi.check(capi.lib.arsim_memset(a, 0, 4 * n, None))
+    for _ in range(3):
+        capi.check(capi.lib.arsim_rebase_indices(b, a, n, 0, None))
+    capi.check(capi.lib.arsim_stream_synchronize(None))
+    capi.lib.arsim_free(a)
+    capi.lib.arsim_free(b)
+    print(json.dumps({"calibration": "copy", "elements": n, "bytes_read": 4 * n, "bytes_written": 4 * n}))
 
 
 def log(msg):
@@ -57,6 +100,12 @@ def main():
         print(json.dumps({"error": "this pipeline runs on one GPU; multi-GPU runs are not part of this build "
                                    "(independent scenes shard across GPUs with no exchange)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    if args.batched_only:
+        batched_only(args)
+        return
     from diploma_thesis_amd import Engine, capi, scenes
 
     method = {"grid": capi.METHOD_GRID, "lbvh30": capi.METHOD_LBVH30, "lbvh63": capi.METHOD_LBVH63}[args.method]
@@ -79,6 +128,11 @@ def main():
     log(f"plain: {ms_per_step:.4f} ms/step, {value:.4e} particle-updates/s")
     if not eng.check_finite():
         log("WARNING: non-finite state after the timed steps")
+    if args.quick:
+        eng.close()
+        print(json.dumps({"metric": "particle-updates/s", "value": value, "ms_per_step": ms_per_step,
+                          "steps": args.steps, "quick": True}))
+        return
 
     # ---- per-step latency (one step per event pair) and per-stage times (HIP events on the engine stream)
     lat = []

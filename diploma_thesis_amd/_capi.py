@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libarsim.so")
+# ARSIM_LIB selects another build of the same library (for A/B timing of build options)
+LIB_PATH = os.environ.get("ARSIM_LIB", os.path.join(HERE, "libarsim.so"))
 
 ARSIM_OK = 0
 ARSIM_ERR_INVALID_ARG = 1
@@ -138,6 +139,7 @@ SIGNATURES = {
     "arsim_lbvh_build": (I, [VP, I, I, VP, VP, VP, VP, VP, VP, VP, VP]),
     "arsim_lbvh_refit": (I, [VP, VP, VP, VP, F, I, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "arsim_dem_lbvh": (I, [VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, I, VP, VP, VP, VP]),
+    "arsim_rebase_indices": (I, [VP, VP, I, I, VP]),
     "arsim_contact_pairs_grid": (I, [VP, VP, VP, VP, VP, I, VP, VP, VP, I, VP]),
     "arsim_contact_pairs_lbvh": (I, [VP, VP, VP, VP, VP, VP, I, VP, VP, VP, VP, I, VP]),
     "arsim_engine_create": (I, [VP, I, VP, I, PVP]),

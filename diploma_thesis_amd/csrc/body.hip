@@ -6,18 +6,19 @@ namespace arsim {
 
 namespace {
 
+constexpr int kMaxHistWords = 8 * 256;   // up to 8 digit passes of up to 8 bits
+
 // GPUintegratorKernel (Kernels/rigidBodyKernels.cu:262-348), with the orientation carried as a
 // cumulative quaternion (q_cum = normalize(q_inc * q_cum)) and I_w^-1 = R_cum I_b^-1 R_cum^T.
-// Blocks past bodyBlocks clear the cellStart entries the previous step's sorted keys set, so the
-// cell table is empty again before this step's reorder without an O(cells) fill.
-__global__ void k_body_integrate(BodyBufs b, int B, DevPhys ph, const unsigned *__restrict__ prevKeys, int N,
-                                 unsigned *__restrict__ cellStart, int bodyBlocks) {
+// Blocks past bodyBlocks zero the given ranges (the next sort's histograms, tile counters and
+// first look-back array), which saves the sort a launch of its own.
+__global__ void k_body_integrate(BodyBufs b, int B, DevPhys ph, ZeroJobs z, int bodyBlocks) {
     if ((int)blockIdx.x >= bodyBlocks) {
-        int i = ((int)blockIdx.x - bodyBlocks) * blockDim.x + threadIdx.x;
-        if (i < N) {
-            unsigned h = prevKeys[i];
-            if (i == 0 || h != prevKeys[i - 1]) cellStart[h] = kEmpty;
-        }
+        const size_t t = (size_t)((int)blockIdx.x - bodyBlocks) * blockDim.x + threadIdx.x;
+        const size_t stride = (size_t)((int)gridDim.x - bodyBlocks) * blockDim.x;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            for (size_t e = t; e < z.count[k]; e += stride) z.p[k][e] = 0u;
         return;
     }
     int bi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -76,8 +77,43 @@ __global__ void k_body_integrate(BodyBufs b, int B, DevPhys ph, const unsigned *
 // (particleKernels.cu:50-129) for grains, then the broadphase key of the updated position:
 // calcHashD (particleKernels.cu:170-189) or generateMortonCodes (BVHkernels.cu:670-716).
 template <int MODE>
-__global__ void k_pose_key(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko, int doBodies,
-                           int doGrains) {
+__device__ __forceinline__ void pose_key_one(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko,
+                                             int doBodies, int doGrains, unsigned *shist);
+
+// Adds 1 to an LDS histogram bin.  Spatially ordered keys often share their high digits across a
+// whole wave; then one lane adds the wave's count instead of 64 lanes hitting one address.
+__device__ __forceinline__ void hist_add(unsigned *h, unsigned d) {
+    const uint64_t active = __ballot(1);
+    const unsigned d0 = __shfl(d, __ffsll((long long)active) - 1, 64);
+    const uint64_t same = __ballot(d == d0);
+    if (same == active) {
+        if ((threadIdx.x & 63) == (unsigned)(__ffsll((long long)active) - 1)) atomicAdd(&h[d], (unsigned)__popcll(active));
+    } else {
+        atomicAdd(&h[d], 1u);
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko,
+                                                  int doBodies, int doGrains) {
+    // digit histograms of this block's keys for every sort pass (when the sort asked for them)
+    __shared__ unsigned shist[kMaxHistWords];
+    const bool hist = MODE != kKeyNone && ko.hist != nullptr;
+    if (hist) {
+        for (int k = threadIdx.x; k < ko.histPasses * 256; k += blockDim.x) shist[k] = 0u;
+        __syncthreads();
+    }
+    pose_key_one<MODE>(p, b, N, ph, sv, ko, doBodies, doGrains, hist ? shist : nullptr);
+    if (hist) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < ko.histPasses * 256; k += blockDim.x)
+            if (shist[k]) atomicAdd(&ko.hist[k], shist[k]);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void pose_key_one(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko,
+                                             int doBodies, int doGrains, unsigned *shist) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int rbi = p.rb ? p.rb[i] : -1;
@@ -127,10 +163,12 @@ __global__ void k_pose_key(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView 
         }
     }
     if (MODE == kKeyNone) return;
+    uint64_t key;
     if (MODE == kKeyGrid) {
         int gx, gy, gz;
         grid_pos(xyz(P), g, gx, gy, gz);
-        ((unsigned *)ko.keys)[i] = g.cellBase + grid_hash(gx, gy, gz, g);
+        key = g.cellBase + grid_hash(gx, gy, gz, g);
+        ((unsigned *)ko.keys)[i] = (unsigned)key;
     } else {
         float lo0 = g.mLo[0], lo1 = g.mLo[1], lo2 = g.mLo[2];
         float e0 = g.mExt[0], e1 = g.mExt[1], e2 = g.mExt[2];
@@ -140,11 +178,26 @@ __global__ void k_pose_key(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView 
             e0 = mx.x - mn.x; e1 = mx.y - mn.y; e2 = mx.z - mn.z;
         }
         float u = mnorm(P.x, lo0, e0), v = mnorm(P.y, lo1, e1), w = mnorm(P.z, lo2, e2);
-        if (MODE == kKeyMorton30) ((unsigned *)ko.keys)[i] = morton30(u, v, w);
-        else if (MODE == kKeyMorton63) ((uint64_t *)ko.keys)[i] = morton63(u, v, w);
-        else ((uint64_t *)ko.keys)[i] = ((uint64_t)(unsigned)s << 30) | (uint64_t)morton30(u, v, w);
+        if (MODE == kKeyMorton30) {
+            key = morton30(u, v, w);
+            ((unsigned *)ko.keys)[i] = (unsigned)key;
+        } else if (MODE == kKeyMorton63) {
+            key = morton63(u, v, w);
+            ((uint64_t *)ko.keys)[i] = key;
+        } else {
+            key = ((uint64_t)(unsigned)s << 30) | (uint64_t)morton30(u, v, w);
+            ((uint64_t *)ko.keys)[i] = key;
+        }
     }
     if (ko.index) ko.index[i] = (unsigned)i;
+    if (shist) {
+        for (int pp = 0; pp < ko.histPasses; ++pp) {
+            const int sh0 = pp * ko.histRbits;
+            const int bits = ko.histEndBit - sh0 < ko.histRbits ? ko.histEndBit - sh0 : ko.histRbits;
+            const unsigned d = (unsigned)(key >> sh0) & ((1u << bits) - 1u);
+            hist_add(shist + pp * 256, d);
+        }
+    }
 }
 
 // Segmented per-body reduction + apply (the intended sum of UniformGridDEM.cu:421-460).
@@ -292,14 +345,21 @@ void launch_fill_u16(uint16_t *p, uint16_t value, size_t count, hipStream_t st) 
     hipLaunchKernelGGL(k_fill_u16, dim3(blocks), dim3(256), 0, st, p, value, count);
 }
 
-void launch_body_integrate(const BodyBufs &b, int numBodies, const DevPhys &ph, const unsigned *prevSortedKeys,
-                           int numParticles, unsigned *cellStart, hipStream_t st) {
+void launch_body_integrate(const BodyBufs &b, int numBodies, const DevPhys &ph, const ZeroJobs *zero,
+                           hipStream_t st) {
     const int block = 128;
     int bodyBlocks = (int)blocks_for(numBodies, block);
-    int clearBlocks = (prevSortedKeys && cellStart) ? (int)blocks_for(numParticles, block) : 0;
-    if (bodyBlocks + clearBlocks == 0) return;
-    hipLaunchKernelGGL(k_body_integrate, dim3(bodyBlocks + clearBlocks), dim3(block), 0, st, b, numBodies, ph,
-                       prevSortedKeys, numParticles, cellStart, bodyBlocks);
+    ZeroJobs z{};
+    size_t zmax = 0;
+    if (zero) {
+        z = *zero;
+        for (int k = 0; k < 3; ++k) zmax = z.count[k] > zmax ? z.count[k] : zmax;
+    }
+    int zeroBlocks = zmax ? (int)blocks_for((long long)zmax, block) : 0;
+    if (zeroBlocks > 512) zeroBlocks = 512;
+    if (bodyBlocks + zeroBlocks == 0) return;
+    hipLaunchKernelGGL(k_body_integrate, dim3(bodyBlocks + zeroBlocks), dim3(block), 0, st, b, numBodies, ph, z,
+                       bodyBlocks);
 }
 
 void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph, const SceneView &sv,

@@ -116,7 +116,7 @@ int arsim_integrate_bodies(float *rbPositions, float *rbVelocities, float *rbAng
     b.pos = (float4 *)rbPositions; b.vel = (float4 *)rbVelocities; b.ang = (float4 *)rbAngularVelocity;
     b.qinc = (float4 *)rbQuaternion; b.qcum = (float4 *)cumulativeQuaternion;
     b.Ib = rbInertia; b.Iw = rbCurrentInertia; b.model = modelMatrix;
-    launch_body_integrate(b, numBodies, make_phys(*params, *cfg), nullptr, 0, nullptr, S(st));
+    launch_body_integrate(b, numBodies, make_phys(*params, *cfg), nullptr, S(st));
     return check_launch("integrate_bodies");
 }
 
@@ -359,6 +359,12 @@ int arsim_dem_lbvh(const int *rbIndices, float *pForce, float *pTorque, float *v
     o.relPos = (const float4 *)relativePos;
     launch_dem_lbvh(sortedIndices, t, n, make_phys(*params, *cfg), sv, false, o, 128, S(st), overflowFlag);
     return check_launch("dem_lbvh");
+}
+
+int arsim_rebase_indices(int *dst, const int *src, int n, int delta, arsim_stream_t st) {
+    if (n < 0 || (n > 0 && (!dst || !src))) return fail(ARSIM_ERR_INVALID_ARG, "rebase_indices: bad arguments");
+    launch_copy_offset(dst, src, n, delta, S(st));
+    return check_launch("rebase_indices");
 }
 
 /* ---------------------------------------------------------------- contact pairs (test mode) */

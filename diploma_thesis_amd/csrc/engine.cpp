@@ -34,6 +34,26 @@ int ceil_log2(long long v) {
 
 bool same_phys(const DevPhys &a, const DevPhys &b) { return std::memcmp(&a, &b, sizeof(DevPhys)) == 0; }
 
+// The ranges a sort needs cleared before its histograms are accumulated.
+ZeroJobs sort_zero_jobs(const SortPlan &pl) {
+    ZeroJobs z{};
+    z.p[0] = pl.hist;
+    z.count[0] = pl.histWords;
+    z.p[1] = pl.counters;
+    z.count[1] = (size_t)pl.passes;
+    z.p[2] = pl.statusA;
+    z.count[2] = pl.statusWords;
+    return z;
+}
+
+// Lets the key kernel accumulate the sort's digit histograms.
+void attach_hist(KeyOut &ko, const SortPlan &pl, const SortTuning &stt, int keyBits) {
+    ko.hist = pl.hist;
+    ko.histPasses = pl.passes;
+    ko.histRbits = stt.radixBits;
+    ko.histEndBit = keyBits;
+}
+
 }  // namespace
 
 template <class T> int Engine::alloc(T **p, size_t count) {
@@ -175,6 +195,9 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         if (keyBits > 32) return fail(ARSIM_ERR_OVERFLOW, "engine: scene id + cell bits exceed 32");
         keyBytes = 4;
         numCells = (size_t)1 << keyBits;
+        segBits = std::min(6, std::max(3, keyBits - 21));
+        if (segBits > keyBits) segBits = keyBits;
+        numSegs = (long long)(numCells >> segBits);
         for (int s = 0; s < S; ++s) geoms[s].cellBase = (unsigned)s << cellBits;
     } else if (method == ARSIM_METHOD_LBVH30 && S == 1) {
         keyBits = 30; keyBytes = 4;
@@ -207,7 +230,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     }
     A(overflowD, 4); A(flagD, 4);
     if (method == ARSIM_METHOD_GRID) {
-        A(cellStart, numCells); A(cellEnd, numCells);
+        A(segStart, (size_t)numSegs + 1);
         A(sortedPos, N); A(sortedVel, N);
     } else {
         size_t nodes = (size_t)2 * N;
@@ -293,7 +316,6 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     ARSIM_HIP(hipMemcpyAsync(sceneCountD, hCount.data(), 4 * (size_t)S, hipMemcpyHostToDevice, st));
     if (tree.sceneRoot) launch_fill_u32((unsigned *)tree.sceneRoot, 0xffffffffu, (size_t)S, st);
     launch_fill_u32((unsigned *)overflowD, 0u, 1, st);
-    if (cellStart) launch_fill_u32(cellStart, kEmpty, numCells, st);
     ARSIM_HIP(hipGetLastError());
     ARSIM_HIP(hipStreamSynchronize(st));
     first = true;
@@ -303,7 +325,6 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
 
 int Engine::reset_grid_tables() {
     invalidate_graph();
-    if (cellStart) launch_fill_u32(cellStart, kEmpty, numCells, st);
     first = true;
     prevSorted = nullptr;
     ARSIM_HIP(hipGetLastError());
@@ -324,20 +345,26 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
     };
     rec();
     if (method == ARSIM_METHOD_GRID) {
-        launch_body_integrate(bb, B, ph, first ? nullptr : (const unsigned *)prevSorted, N, cellStart, st);
+        // the sort's workspace is cleared by the body launch and its histograms are built while the
+        // keys are generated, so the sort itself is only its digit passes
+        const SortPlan pl = sort_plan(sortWs, N, keyBytes, 0, keyBits, stt);
+        const ZeroJobs z = sort_zero_jobs(pl);
+        launch_body_integrate(bb, B, ph, &z, st);
         rec();
         KeyOut ko{keysA, nullptr, nullptr};
+        attach_hist(ko, pl, stt, keyBits);
         launch_pose_key(pb, bb, N, ph, sv, kKeyGrid, ko, true, true, st);
         rec();
         unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
-        if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
+        if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st, true))
             return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
         rec();
-        launch_reorder(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, sortedPos, sortedVel, cellStart, cellEnd, N, st);
+        launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, sortedPos, sortedVel, segStart, segBits,
+                           numSegs, N, st);
         rec();
         DemOut o{};
         o.pForce = pForce; o.vel = vel; o.relPos = relPos;
-        launch_dem_grid(sortedPos, sortedVel, vA, cellStart, cellEnd, N, ph, sv, S > 1, o, tu.demBlock, st);
+        launch_dem_grid_seg(sortedPos, sortedVel, kA, vA, segStart, segBits, N, ph, sv, S > 1, o, tu.demBlock, st);
         rec();
         launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
         rec();
@@ -345,7 +372,10 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         lastKeys = kA;
         lastIdx = vA;
     } else {
-        launch_body_integrate(bb, B, ph, nullptr, 0, nullptr, st);
+        const SortPlan pl = sort_plan(sortWs, N, keyBytes, 0, keyBits, stt);
+        const bool fused = pl.passes <= 8;   // the key kernel holds at most 8 digit histograms
+        const ZeroJobs z = sort_zero_jobs(pl);
+        launch_body_integrate(bb, B, ph, fused ? &z : nullptr, st);
         rec();
         const int mode = S > 1 ? kKeyForest30 : (method == ARSIM_METHOD_LBVH30 ? kKeyMorton30 : kKeyMorton63);
         if (flags & ARSIM_DYNAMIC_AABB) {
@@ -353,9 +383,11 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
             launch_pose_key(pb, bb, N, ph, sv, kKeyNone, none, true, true, st);
             launch_aabb(pos, N, aabbBox, aabbBox + 1, aabbPart, aabb_max_partials(), st);
             KeyOut ko{keysA, nullptr, aabbBox};
+            if (fused) attach_hist(ko, pl, stt, keyBits);
             launch_pose_key(pb, bb, N, ph, sv, mode, ko, false, false, st);
         } else {
             KeyOut ko{keysA, nullptr, nullptr};
+            if (fused) attach_hist(ko, pl, stt, keyBits);
             launch_pose_key(pb, bb, N, ph, sv, mode, ko, true, true, st);
         }
         rec();
@@ -363,12 +395,12 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         unsigned *vA = valsA, *vB = valsB;
         if (keyBytes == 4) {
             unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB;
-            if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
+            if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st, fused))
                 return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
             sortedKeys = kA;
         } else {
             uint64_t *kA = (uint64_t *)keysA, *kB = (uint64_t *)keysB;
-            if (sort_pairs<uint64_t>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
+            if (sort_pairs<uint64_t>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st, fused))
                 return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
             sortedKeys = kA;
         }
@@ -428,14 +460,14 @@ int Engine::enqueue_broadphase(int *pairsD, int *countsD, int maxPer) {
     DemOut o{};
     o.pairs = pairsD; o.counts = countsD; o.maxPer = maxPer;
     if (method == ARSIM_METHOD_GRID) {
-        if (!first) launch_body_integrate(bb, 0, ph, (const unsigned *)prevSorted, N, cellStart, st);
         KeyOut ko{keysA, nullptr, nullptr};
         launch_pose_key(pb, bb, N, ph, sv, kKeyGrid, ko, false, false, st);
         unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
         if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
             return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
-        launch_reorder(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, sortedPos, sortedVel, cellStart, cellEnd, N, st);
-        launch_dem_grid(sortedPos, sortedVel, vA, cellStart, cellEnd, N, ph, sv, S > 1, o, tu.demBlock, st);
+        launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, sortedPos, sortedVel, segStart, segBits,
+                           numSegs, N, st);
+        launch_dem_grid_seg(sortedPos, sortedVel, kA, vA, segStart, segBits, N, ph, sv, S > 1, o, tu.demBlock, st);
         prevSorted = kA;
         lastKeys = kA;
         lastIdx = vA;
@@ -613,8 +645,8 @@ int arsim_engine_debug_buffers(arsim_engine *e, void **sortedKeys, void **sorted
     Engine &E = e->e;
     if (sortedKeys) *sortedKeys = E.lastKeys;
     if (sortedIndex) *sortedIndex = E.lastIdx;
-    if (cellStart) *cellStart = E.cellStart;
-    if (cellEnd) *cellEnd = E.cellEnd;
+    if (cellStart) *cellStart = E.segStart;   // the engine's cell index: CSR offsets over cell segments
+    if (cellEnd) *cellEnd = nullptr;
     if (sortedPos) *sortedPos = E.sortedPos;
     if (sortedVel) *sortedVel = E.method == ARSIM_METHOD_GRID ? (void *)E.sortedVel : (void *)E.tree.sortedVel;
     if (lbvhLeft) *lbvhLeft = E.tree.left;

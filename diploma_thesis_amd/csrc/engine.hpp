@@ -46,7 +46,9 @@ struct Engine {
     int *sceneStartD = nullptr, *sceneCountD = nullptr;
     void *keysA = nullptr, *keysB = nullptr;
     unsigned *valsA = nullptr, *valsB = nullptr;
-    unsigned *cellStart = nullptr, *cellEnd = nullptr;
+    unsigned *segStart = nullptr;           // grid: CSR offsets over segments of 2^segBits cells
+    int segBits = 3;
+    long long numSegs = 0;
     float4 *sortedPos = nullptr, *sortedVel = nullptr;
     void *sortWs = nullptr;
     size_t sortWsBytes = 0;

@@ -100,7 +100,190 @@ __global__ void k_dem_grid(const float4 *__restrict__ sPos, const float4 *__rest
     }
 }
 
+// Writes v to p[a..b) with 16-byte stores for the aligned middle of long runs.
+__device__ __forceinline__ void fill_range(unsigned *p, long long a, long long b, unsigned v) {
+    long long k = a;
+    while (k < b && (k & 3)) p[k++] = v;
+    const uint4 v4 = make_uint4(v, v, v, v);
+    for (; k + 4 <= b; k += 4) *reinterpret_cast<uint4 *>(p + k) = v4;
+    for (; k < b; ++k) p[k] = v;
+}
+
+// Engine form of the reorder: the same gather of the sorted copies, but instead of the two
+// cell-sized tables it writes a dense CSR offset array over segments of 2^segBits cells:
+// segStart[s] = first sorted index whose key >= s << segBits (segStart[numSegs] = n).  Every entry
+// is rewritten each step, so nothing has to be cleared, and the table is 2^segBits x smaller than
+// cellStart + cellEnd.
+__global__ void k_reorder_seg(const unsigned *__restrict__ keys, const unsigned *__restrict__ idx,
+                              const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                              const int *__restrict__ rb, const uint16_t *__restrict__ scene,
+                              float4 *__restrict__ sPos, float4 *__restrict__ sVel, unsigned *__restrict__ segStart,
+                              int segBits, long long numSegs, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned k = keys[i];
+    const long long s = (long long)(k >> segBits);
+    const long long sp = i == 0 ? -1 : (long long)(keys[i - 1] >> segBits);
+    if (s != sp) fill_range(segStart, sp + 1, s + 1, (unsigned)i);
+    if (i == n - 1) fill_range(segStart, s + 1, numSegs + 1, (unsigned)n);
+    const unsigned o = idx[i];
+    float4 p = pos[o];
+    p.w = __int_as_float(rb[o]);
+    sPos[i] = p;
+    float4 v = vel[o];
+    if (scene) v.w = __int_as_float((int)scene[o]);
+    sVel[i] = v;
+}
+
+// Contacts of one sorted particle within the sorted-key run [j, ...) of cells h0..h1 (one row of
+// up to three consecutive cells), cell by cell in ascending order, each cell's contacts summed
+// into a partial that is added to F when the cell ends -- the same grouping and order as the
+// 27-cell loop over cellStart / cellEnd.
+// The sorted run is read in chunks of kChunk keys and positions issued together, so a row costs
+// one or two memory round trips instead of one per candidate.
+#ifndef ARSIM_DEM_CHUNK
+#define ARSIM_DEM_CHUNK 4
+#endif
+constexpr int kChunk = ARSIM_DEM_CHUNK;
+
+template <bool EMIT>
+__device__ __forceinline__ void scan_cells(const unsigned *__restrict__ keys, const float4 *__restrict__ sPos,
+                                           const float4 *__restrict__ sVel, const unsigned *__restrict__ sIdx,
+                                           unsigned j, int n, unsigned h0, unsigned h1, int i, int ra, f3 pa, f3 va,
+                                           const DevPhys &ph, f3 &F, int &cnt, unsigned orig, const DemOut &o) {
+    unsigned cur = 0xffffffffu;
+    f3 csum = mk3(0.f, 0.f, 0.f);
+    bool done = false;
+    while (!done) {
+        unsigned kk[kChunk];
+        float4 qq[kChunk];
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) kk[c] = ((int)j + c < n) ? keys[j + c] : 0xffffffffu;
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) qq[c] = ((int)j + c < n) ? sPos[j + c] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) {
+            if (done) break;
+            const unsigned kj = kk[c];
+            if (kj < h0) continue;
+            if (kj > h1 || (int)j + c >= n) {
+                done = true;
+                break;
+            }
+            if (kj != cur) {
+                if (cur != 0xffffffffu) F = add3(F, csum);
+                csum = mk3(0.f, 0.f, 0.f);
+                cur = kj;
+            }
+            const unsigned jj = j + c;
+            const float4 Q = qq[c];
+            if ((int)jj != i && pair_allowed(ra, __float_as_int(Q.w))) {
+                if (EMIT) {
+                    if (in_contact(pa, xyz(Q), ph.r)) {
+                        if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = (int)sIdx[jj];
+                        ++cnt;
+                    }
+                } else {
+                    f3 fj;
+                    if (pair_force(pa, xyz(Q), va, xyz(sVel[jj]), ph, fj)) csum = add3(csum, fj);
+                }
+            }
+        }
+        j += kChunk;
+    }
+    if (cur != 0xffffffffu) F = add3(F, csum);
+}
+
+// Engine DEM over the segment index: for each of the 9 (dz, dy) rows the three x-cells are
+// consecutive keys unless x wraps, so one segment lookup and one forward scan of the sorted
+// keys visit them in the order x-1, x, x+1.
+template <bool EMIT>
+__global__ void k_dem_grid_seg(const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
+                               const unsigned *__restrict__ keys, const unsigned *__restrict__ sIdx,
+                               const unsigned *__restrict__ segStart, int segBits, int n, DevPhys ph, SceneView sv,
+                               int sceneInVelW, DemOut o) {
+    const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int i = lb * (int)blockDim.x + (int)threadIdx.x;
+    if (i >= n) return;
+    const float4 P = sPos[i];
+    const float4 Vq = sVel[i];
+    const int ra = __float_as_int(P.w);
+    const int s = sceneInVelW ? __float_as_int(Vq.w) : 0;
+    SceneGeom g;
+    load_geom(sv, s, g);
+    const f3 pa = xyz(P), va = xyz(Vq);
+    int gx, gy, gz;
+    grid_pos(pa, g, gx, gy, gz);
+    const unsigned orig = sIdx[i];
+    const unsigned x0 = (unsigned)(gx - 1) & g.gridMask[0], x1 = (unsigned)gx & g.gridMask[0],
+                   x2 = (unsigned)(gx + 1) & g.gridMask[0];
+    const bool contiguous = (x0 + 1u == x1) && (x1 + 1u == x2);
+    f3 F = mk3(0.f, 0.f, 0.f);
+    int cnt = 0;
+    // the nine row starts are independent: issue all their loads before scanning any row
+    auto row_of = [&](int r) {
+        const unsigned uz = (unsigned)(gz + (r / 3) - 1) & g.gridMask[2];
+        const unsigned uy = (unsigned)(gy + (r % 3) - 1) & g.gridMask[1];
+        return g.cellBase + uz * g.gxy + uy * g.gx;
+    };
+    unsigned rowJ[9];
+    if (contiguous) {
+#pragma unroll
+        for (int r = 0; r < 9; ++r) rowJ[r] = segStart[(row_of(r) + x0) >> segBits];
+    }
+    for (int r = 0; r < 9; ++r) {
+        const unsigned row = row_of(r);
+        if (contiguous) {
+            const unsigned h0 = row + x0;
+            scan_cells<EMIT>(keys, sPos, sVel, sIdx, rowJ[r], n, h0, h0 + 2u, i, ra, pa, va, ph, F, cnt, orig, o);
+        } else {
+            const unsigned xs[3] = {x0, x1, x2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const unsigned h = row + xs[k];
+                scan_cells<EMIT>(keys, sPos, sVel, sIdx, segStart[h >> segBits], n, h, h, i, ra, pa, va, ph, F, cnt,
+                                 orig, o);
+            }
+        }
+    }
+    if (EMIT) {
+        o.counts[orig] = cnt;
+        return;
+    }
+    if (!((ph.flags & kGrainClamp) && ra < 0)) F = add3(F, wall_force(pa, va, ph, g));
+    if (ra >= 0) {
+        o.pForce[orig] = f4(F, 0.0f);
+        if (o.pTorque) o.pTorque[orig] = f4(cross3(xyz(o.relPos[orig]), F), 0.0f);
+    } else {
+        const float fs = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
+        o.vel[orig] = make_float4(va.x + F.x * fs, va.y + F.y * fs, va.z + F.z * fs, 0.0f);
+    }
+}
+
 }  // namespace
+
+void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, const float4 *pos, const float4 *vel,
+                        const int *rb, const uint16_t *scene, float4 *sortedPos, float4 *sortedVel,
+                        unsigned *segStart, int segBits, long long numSegs, int n, hipStream_t st) {
+    if (n <= 0) return;
+    const int block = 256;
+    hipLaunchKernelGGL(k_reorder_seg, dim3((n + block - 1) / block), dim3(block), 0, st, sortedKeys, sortedIdx, pos,
+                       vel, rb, scene, sortedPos, sortedVel, segStart, segBits, numSegs, n);
+}
+
+void launch_dem_grid_seg(const float4 *sortedPos, const float4 *sortedVel, const unsigned *sortedKeys,
+                         const unsigned *sortedIdx, const unsigned *segStart, int segBits, int n, const DevPhys &ph,
+                         const SceneView &sv, bool sceneInVelW, const DemOut &o, int block, hipStream_t st) {
+    if (n <= 0) return;
+    if (block <= 0) block = 128;
+    dim3 grid((n + block - 1) / block);
+    if (o.pairs)
+        hipLaunchKernelGGL(k_dem_grid_seg<true>, grid, dim3(block), 0, st, sortedPos, sortedVel, sortedKeys, sortedIdx,
+                           segStart, segBits, n, ph, sv, sceneInVelW ? 1 : 0, o);
+    else
+        hipLaunchKernelGGL(k_dem_grid_seg<false>, grid, dim3(block), 0, st, sortedPos, sortedVel, sortedKeys, sortedIdx,
+                           segStart, segBits, n, ph, sv, sceneInVelW ? 1 : 0, o);
+}
 
 void launch_reorder(const unsigned *sortedKeys, const unsigned *sortedIdx, const float4 *pos, const float4 *vel,
                     const int *rb, const uint16_t *scene, float4 *sortedPos, float4 *sortedVel, unsigned *cellStart,

@@ -22,7 +22,16 @@ bool sort_tuning_supported(int ipt, int rbits);
 // *keys / *vals hold the sorted data.  valsIota: the input values are 0..n-1 (not read).
 template <class K>
 int sort_pairs(K **keys, unsigned **vals, K **alt, unsigned **altVals, int n, int beginBit, int endBit,
-               bool valsIota, void *ws, size_t wsBytes, const SortTuning &tu, hipStream_t st);
+               bool valsIota, void *ws, size_t wsBytes, const SortTuning &tu, hipStream_t st, bool prepared = false);
+// Where the sort keeps its digit histograms, tile counters and look-back words inside the workspace.
+// A caller may zero {hist, counters, statusA} and accumulate the histograms itself (for instance while
+// it generates the keys) and then call sort_pairs(..., prepared = true), which skips both steps.
+struct SortPlan {
+    unsigned *hist, *counters, *statusA, *statusB;
+    int numTiles, passes, radix;
+    size_t histWords, statusWords;
+};
+SortPlan sort_plan(void *ws, int n, int keyBytes, int beginBit, int endBit, const SortTuning &tu);
 
 // ---------------------------------------------------------------- buffers
 struct BodyBufs {
@@ -48,11 +57,19 @@ struct KeyOut {
     void *keys;                   // u32 (grid / morton30) or u64 (morton63 / forest30)
     unsigned *index;              // may be NULL when the sort generates the identity
     const float4 *box;            // dynamic Morton box (min, max) on the device, or NULL
+    unsigned *hist = nullptr;     // optional: the sort's digit histograms, accumulated here
+    int histPasses = 0, histRbits = 8, histEndBit = 32;
+};
+
+// Up to three device ranges zeroed by extra blocks of the body-integration launch.
+struct ZeroJobs {
+    unsigned *p[3] = {nullptr, nullptr, nullptr};
+    size_t count[3] = {0, 0, 0};
 };
 
 // ---------------------------------------------------------------- particle / body stages (body.hip)
-void launch_body_integrate(const BodyBufs &b, int numBodies, const DevPhys &ph, const unsigned *prevSortedKeys,
-                           int numParticles, unsigned *cellStart, hipStream_t st);
+void launch_body_integrate(const BodyBufs &b, int numBodies, const DevPhys &ph, const ZeroJobs *zero,
+                           hipStream_t st);
 // doBodies / doGrains select which particles are updated; mode selects the key written.
 void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,
                      const SceneView &sv, int mode, const KeyOut &ko, bool doBodies, bool doGrains,
@@ -85,6 +102,15 @@ struct DemOut {
 void launch_dem_grid(const float4 *sortedPos, const float4 *sortedVel, const unsigned *sortedIdx,
                      const unsigned *cellStart, const unsigned *cellEnd, int n, const DevPhys &ph,
                      const SceneView &sv, bool sceneInVelW, const DemOut &o, int block, hipStream_t st);
+
+// Engine form of the grid broadphase: a dense CSR offset array over segments of 2^segBits cells
+// (segStart has numSegs + 1 entries) replaces cellStart / cellEnd; the DEM scans the sorted keys.
+void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, const float4 *pos, const float4 *vel,
+                        const int *rb, const uint16_t *scene, float4 *sortedPos, float4 *sortedVel,
+                        unsigned *segStart, int segBits, long long numSegs, int n, hipStream_t st);
+void launch_dem_grid_seg(const float4 *sortedPos, const float4 *sortedVel, const unsigned *sortedKeys,
+                         const unsigned *sortedIdx, const unsigned *segStart, int segBits, int n, const DevPhys &ph,
+                         const SceneView &sv, bool sceneInVelW, const DemOut &o, int block, hipStream_t st);
 
 // ---------------------------------------------------------------- LBVH path (lbvh.hip)
 struct Lbvh {

@@ -115,6 +115,19 @@ struct CellKey {
     size_t operator()(int64_t k) const { return (size_t)(k * 0x9e3779b97f4a7c15ull); }
 };
 
+// 3 x 21-bit Morton code of a point in [0, box] (ordering only).
+uint64_t morton_code(double x, double y, double z, const double box[3]) {
+    auto q = [](double v, double L) {
+        double u = L > 0 ? v / L : 0.0;
+        u = u < 0 ? 0 : (u > 1 ? 1 : u);
+        uint64_t k = (uint64_t)(u * 2097151.0);
+        uint64_t r = 0;
+        for (int b = 0; b < 21; ++b) r |= ((k >> b) & 1ull) << (3 * b);
+        return r;
+    };
+    return (q(x, box[0]) << 2) | (q(y, box[1]) << 1) | q(z, box[2]);
+}
+
 }  // namespace
 
 extern "C" int arsim_generate_scene(const ArSimGenSpec *spec, ArSimSceneOwned **out) {
@@ -184,6 +197,26 @@ extern "C" int arsim_generate_scene(const ArSimGenSpec *spec, ArSimSceneOwned **
         }
     }
 
+    // ---- bodies in Morton order of their centres: particles that are close in space get close
+    // indices, so the broadphase's gathers and scatters by particle index stay coherent
+    if (B > 1) {
+        std::vector<int> order(B);
+        std::vector<uint64_t> code(B);
+        for (int b = 0; b < B; ++b) {
+            order[b] = b;
+            code[b] = morton_code(centres[3 * b], centres[3 * b + 1], centres[3 * b + 2], box);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return code[a] < code[c]; });
+        std::vector<double> c2(3 * (size_t)B);
+        std::vector<BodyShape> s2(B);
+        for (int k = 0; k < B; ++k) {
+            for (int a = 0; a < 3; ++a) c2[3 * k + a] = centres[3 * order[k] + a];
+            s2[k] = std::move(shapes[order[k]]);
+        }
+        centres.swap(c2);
+        shapes.swap(s2);
+    }
+
     // ---- particles of bodies
     std::vector<double> P;   // world positions (double) for overlap checks
     for (int b = 0; b < B; ++b) {
@@ -233,6 +266,9 @@ extern "C" int arsim_generate_scene(const ArSimGenSpec *spec, ArSimSceneOwned **
             grid[key(cx, cy, cz)].push_back((int)idx);
         };
         for (size_t k = 0; k < P.size() / 3; ++k) insert(k);
+        struct GrainRec { uint64_t code; double v[6]; };
+        std::vector<GrainRec> grains;
+        grains.reserve(spec->numGrains);
         int placed = 0;
         long long attempts = 0, maxAttempts = (long long)spec->numGrains * 200 + 1000;
         while (placed < spec->numGrains && attempts < maxAttempts) {
@@ -256,15 +292,22 @@ extern "C" int arsim_generate_scene(const ArSimGenSpec *spec, ArSimSceneOwned **
             P.insert(P.end(), {x, y, z});
             insert(P.size() / 3 - 1);
             double vs = spec->grainVelSigma;
-            s->pos.insert(s->pos.end(), {(float)x, (float)y, (float)z, 0.f});
-            s->vel.insert(s->vel.end(), {(float)(rng.normal() * vs), (float)(rng.normal() * vs), (float)(rng.normal() * vs), 0.f});
-            s->bodyFrame.insert(s->bodyFrame.end(), {0.f, 0.f, 0.f, 0.f});
-            s->rb.push_back(-1);
+            double v0 = rng.normal() * vs, v1 = rng.normal() * vs, v2 = rng.normal() * vs;
+            grains.push_back({morton_code(x, y, z, box), {x, y, z, v0, v1, v2}});
             ++placed;
         }
         if (placed < spec->numGrains) {
             delete s;
             return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "could not place all grains without overlap");
+        }
+        // grains in Morton order, after the bodies
+        std::stable_sort(grains.begin(), grains.end(),
+                         [](const GrainRec &a, const GrainRec &b) { return a.code < b.code; });
+        for (const GrainRec &g : grains) {
+            s->pos.insert(s->pos.end(), {(float)g.v[0], (float)g.v[1], (float)g.v[2], 0.f});
+            s->vel.insert(s->vel.end(), {(float)g.v[3], (float)g.v[4], (float)g.v[5], 0.f});
+            s->bodyFrame.insert(s->bodyFrame.end(), {0.f, 0.f, 0.f, 0.f});
+            s->rb.push_back(-1);
         }
     }
 

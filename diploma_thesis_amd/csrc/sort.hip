@@ -68,12 +68,21 @@ __global__ __launch_bounds__(kBlock) void k_sort_hist(const K *__restrict__ keys
             int sh0 = beginBit + p * RBITS;
             int bits = min(RBITS, endBit - sh0);
             unsigned d = (unsigned)((key >> sh0) & (K)((1u << bits) - 1u));
-            atomicAdd(&sh[p * RADIX + d], 1u);
+            // keys in spatial order often share a digit across the wave: one add for all of them
+            const uint64_t active = __ballot(1);
+            const int first = __ffsll((long long)active) - 1;
+            const unsigned d0 = __shfl(d, first, 64);
+            if (__ballot(d == d0) == active) {
+                if ((int)(threadIdx.x & 63) == first) atomicAdd(&sh[p * RADIX + d], (unsigned)__popcll(active));
+            } else {
+                atomicAdd(&sh[p * RADIX + d], 1u);
+            }
         }
     }
     __syncthreads();
+    // global layout: pass p's histogram starts at p * 256 whatever the digit width
     for (int k = threadIdx.x; k < passes * RADIX; k += kBlock)
-        if (sh[k]) atomicAdd(&hist[k], sh[k]);
+        if (sh[k]) atomicAdd(&hist[(k / RADIX) * 256 + (k % RADIX)], sh[k]);
 }
 
 template <class K, int IPT, int RBITS>
@@ -213,20 +222,21 @@ inline int num_passes(int beginBit, int endBit, int rbits) { return (endBit - be
 template <class K, int IPT, int RBITS>
 void launch_passes(K **keys, unsigned **vals, K **alt, unsigned **altVals, int n, int beginBit, int endBit,
                    bool valsIota, unsigned *hist, unsigned *counters, unsigned *statusA, unsigned *statusB,
-                   int numTiles, hipStream_t st) {
-    constexpr int RADIX = 1 << RBITS;
+                   int numTiles, bool doHist, hipStream_t st) {
     const int passes = num_passes(beginBit, endBit, RBITS);
-    int histBlocks = (n + kBlock * 16 - 1) / (kBlock * 16);
-    histBlocks = histBlocks < 1 ? 1 : (histBlocks > 2048 ? 2048 : histBlocks);
-    hipLaunchKernelGGL((k_sort_hist<K, RBITS>), dim3(histBlocks), dim3(kBlock), 0, st, *keys, n, beginBit, endBit,
-                       passes, hist);
+    if (doHist) {
+        int histBlocks = (n + kBlock * 8 - 1) / (kBlock * 8);
+        histBlocks = histBlocks < 1 ? 1 : (histBlocks > 4096 ? 4096 : histBlocks);
+        hipLaunchKernelGGL((k_sort_hist<K, RBITS>), dim3(histBlocks), dim3(kBlock), 0, st, *keys, n, beginBit, endBit,
+                           passes, hist);
+    }
     for (int p = 0; p < passes; ++p) {
         int shift = beginBit + p * RBITS;
         int bits = endBit - shift < RBITS ? endBit - shift : RBITS;
         unsigned *cur = (p & 1) ? statusB : statusA;
         unsigned *nxt = (p + 1 < passes) ? ((p & 1) ? statusA : statusB) : nullptr;
         hipLaunchKernelGGL((k_onesweep<K, IPT, RBITS>), dim3(numTiles), dim3(kBlock), 0, st, *keys, *vals, *alt,
-                           *altVals, n, shift, bits, (valsIota && p == 0) ? 1 : 0, hist + (size_t)p * RADIX,
+                           *altVals, n, shift, bits, (valsIota && p == 0) ? 1 : 0, hist + (size_t)p * 256,
                            counters + p, cur, nxt);
         K *tk = *keys; *keys = *alt; *alt = tk;
         unsigned *tv = *vals; *vals = *altVals; *altVals = tv;
@@ -247,34 +257,47 @@ size_t sort_workspace_bytes(int n, int keyBytes) {
     return words * sizeof(unsigned) + 256;
 }
 
+SortPlan sort_plan(void *ws, int n, int keyBytes, int beginBit, int endBit, const SortTuning &tu) {
+    SortPlan p{};
+    const int tile = kBlock * tu.ipt;
+    p.numTiles = n > 0 ? (n + tile - 1) / tile : 1;
+    p.passes = num_passes(beginBit, endBit, tu.radixBits);
+    p.radix = 1 << tu.radixBits;
+    unsigned *w = (unsigned *)ws;
+    p.hist = w;
+    p.counters = p.hist + (size_t)sort_max_passes(keyBytes) * 256;
+    p.statusA = p.counters + 64;
+    p.statusB = p.statusA + (size_t)p.numTiles * p.radix;
+    // the per-pass histograms are laid out with a 256-word stride whatever the digit width
+    p.histWords = (size_t)p.passes * 256;
+    p.statusWords = (size_t)p.numTiles * p.radix;
+    return p;
+}
+
 template <class K>
 int sort_pairs(K **keys, unsigned **vals, K **alt, unsigned **altVals, int n, int beginBit, int endBit,
-               bool valsIota, void *ws, size_t wsBytes, const SortTuning &tu, hipStream_t st) {
+               bool valsIota, void *ws, size_t wsBytes, const SortTuning &tu, hipStream_t st, bool prepared) {
     if (n <= 0) return 0;
     if (n >= (1 << 30)) return -1;
     const int keyBits = (int)sizeof(K) * 8;
     if (beginBit < 0 || endBit > keyBits || beginBit >= endBit) return -1;
     if (wsBytes < sort_workspace_bytes(n, (int)sizeof(K))) return -2;
+    if (prepared && beginBit != 0) return -1;
     const int ipt = tu.ipt, rbits = tu.radixBits;
-    const int tile = kBlock * ipt;
-    const int numTiles = (n + tile - 1) / tile;
-    const int passes = num_passes(beginBit, endBit, rbits);
-    const int radix = 1 << rbits;
-    unsigned *w = (unsigned *)ws;
-    unsigned *hist = w;
-    unsigned *counters = hist + (size_t)sort_max_passes((int)sizeof(K)) * 256;
-    unsigned *statusA = counters + 64;
-    unsigned *statusB = statusA + (size_t)numTiles * radix;
-    int histWords = passes * radix;
-    int initWords = numTiles * radix;
-    int initBlocks = (initWords + kBlock - 1) / kBlock;
-    initBlocks = initBlocks < 1 ? 1 : (initBlocks > 1024 ? 1024 : initBlocks);
-    hipLaunchKernelGGL(k_sort_init, dim3(initBlocks), dim3(kBlock), 0, st, hist, histWords, counters, passes,
-                       statusA, initWords);
+    const SortPlan pl = sort_plan(ws, n, (int)sizeof(K), beginBit, endBit, tu);
+    const int numTiles = pl.numTiles;
+    unsigned *hist = pl.hist, *counters = pl.counters, *statusA = pl.statusA, *statusB = pl.statusB;
+    if (!prepared) {
+        int initWords = (int)pl.statusWords;
+        int initBlocks = (initWords + kBlock - 1) / kBlock;
+        initBlocks = initBlocks < 1 ? 1 : (initBlocks > 1024 ? 1024 : initBlocks);
+        hipLaunchKernelGGL(k_sort_init, dim3(initBlocks), dim3(kBlock), 0, st, hist, (int)pl.histWords, counters,
+                           pl.passes, statusA, initWords);
+    }
 #define ARSIM_SORT_CASE(I, R)                                                                                 \
     if (ipt == I && rbits == R) {                                                                             \
         launch_passes<K, I, R>(keys, vals, alt, altVals, n, beginBit, endBit, valsIota, hist, counters,       \
-                               statusA, statusB, numTiles, st);                                               \
+                               statusA, statusB, numTiles, !prepared, st);                                    \
         return 0;                                                                                             \
     }
     ARSIM_SORT_CASE(8, 8)
@@ -291,9 +314,9 @@ int sort_pairs(K **keys, unsigned **vals, K **alt, unsigned **altVals, int n, in
 }
 
 template int sort_pairs<unsigned>(unsigned **, unsigned **, unsigned **, unsigned **, int, int, int, bool, void *,
-                                  size_t, const SortTuning &, hipStream_t);
+                                  size_t, const SortTuning &, hipStream_t, bool);
 template int sort_pairs<uint64_t>(uint64_t **, unsigned **, uint64_t **, unsigned **, int, int, int, bool, void *,
-                                  size_t, const SortTuning &, hipStream_t);
+                                  size_t, const SortTuning &, hipStream_t, bool);
 
 bool sort_tuning_supported(int ipt, int rbits) {
     return (rbits == 8 && (ipt == 4 || ipt == 8 || ipt == 12 || ipt == 16)) ||

@@ -281,6 +281,12 @@ int arsim_contact_pairs_lbvh(const int *rbIndices, const int *sortedIndices,
                              const ArSimParams *params, int *overflowFlag,
                              int *pairs, int *counts, int maxPerParticle, arsim_stream_t stream);
 
+/* Body-index rebasing copy: dst[i] = src[i] + delta where src[i] >= 0, else src[i].  The reference
+ * offsets a new body's rbIndices by the current body count when it appends it
+ * (RigidBodies/RigidBodyAddition.cpp:111-139); the engine does the same when it moves scenes
+ * between batch layouts.  With delta = 0 it is a plain 4-byte copy (used to calibrate counters). */
+int arsim_rebase_indices(int *dst, const int *src, int n, int delta, arsim_stream_t stream);
+
 /* ------------------------------------------------------------------ engine (scene-level API)
  * ParticleSystem::update(dt) (Particles/particleSystem.h:288) for one or many scenes.
  * One engine holds S >= 1 independent scenes resident in HBM; one step advances all of them

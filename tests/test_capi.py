@@ -1,0 +1,111 @@
+"""The C ABI library loads, exports every function include/arsim.h declares, keeps the reference's
+SimParams layout, and rejects bad arguments with status codes before touching a device."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from diploma_thesis_amd import capi, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "arsim.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(arsim_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    names = declared_functions()
+    assert len(names) > 40
+    lib = C.CDLL(capi.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in capi.SIGNATURES, f"{n} has no ctypes signature"
+    assert capi.lib.arsim_abi_version() == 1
+
+
+def test_simparams_layout_matches_reference():
+    # Kernels/particleSystem.cuh:27-55 with float3 / uint3 = 12 bytes, 4-byte aligned
+    P = capi.ArSimParams
+    assert C.sizeof(P) == 128
+    offs = {f[0]: getattr(P, f[0]).offset for f in P._fields_}
+    assert offs["colliderPos"] == 0 and offs["colliderRadius"] == 12
+    assert offs["gravity"] == 16 and offs["globalDamping"] == 28 and offs["particleRadius"] == 32
+    assert offs["gridSize"] == 36 and offs["numCells"] == 48
+    assert offs["worldOrigin"] == 52 and offs["cellSize"] == 64
+    assert offs["numBodies"] == 76 and offs["maxParticlesPerCell"] == 80
+    assert offs["spring"] == 84 and offs["Vdamping"] == 124
+
+
+def test_stage_functions_reject_bad_arguments_without_a_device():
+    p = scenes.config0().params
+    cfg = scenes.config0().cfg
+    L = capi.lib
+    null = None
+    assert L.arsim_calc_hash(null, null, null, 10, C.byref(p), null) == capi.ARSIM_ERR_INVALID_ARG
+    assert L.arsim_calc_hash(null, null, null, -1, C.byref(p), null) == capi.ARSIM_ERR_INVALID_ARG
+    assert L.arsim_calc_hash(null, null, null, 0, C.byref(p), null) == capi.ARSIM_OK       # empty input
+    bad = capi.ArSimParams.from_buffer_copy(p)
+    bad.gridSize.y = 100
+    fake = C.c_void_p(16)   # never dereferenced: the grid check comes first
+    assert L.arsim_calc_hash(fake, fake, fake, 1, C.byref(bad), null) == capi.ARSIM_ERR_NOT_POW2
+    assert L.arsim_dem_grid(null, null, null, null, null, null, null, null, null, null, 5, C.byref(p), C.byref(cfg),
+                            null) == capi.ARSIM_ERR_INVALID_ARG
+    assert L.arsim_segmented_reduce_apply(null, null, null, null, null, null, null, null, null, 3, C.byref(p),
+                                          C.byref(cfg), null) == capi.ARSIM_ERR_INVALID_ARG
+    assert L.arsim_lbvh_build(null, 3, 5, null, null, null, null, null, null, null, null) == capi.ARSIM_ERR_INVALID_ARG
+    k = C.c_void_p(16)
+    assert L.arsim_sort_pairs_u32(C.byref(k), C.byref(k), C.byref(k), C.byref(k), 10, 0, 33, fake, 1 << 20,
+                                  null) == capi.ARSIM_ERR_INVALID_ARG
+    assert L.arsim_sort_pairs_u32(C.byref(k), C.byref(k), C.byref(k), C.byref(k), 10, 0, 32, fake, 8,
+                                  null) == capi.ARSIM_ERR_WORKSPACE
+    assert L.arsim_sort_workspace_bytes(1000, 3) == 0
+    assert L.arsim_sort_workspace_bytes(1000, 4) > 0
+    assert L.arsim_last_error()   # a message was recorded
+
+
+def test_engine_create_without_device_reports_no_device():
+    from conftest import HAS_GPU
+    if HAS_GPU:
+        pytest.skip("a device is present")
+    sc = scenes.config0()
+    v = sc.view()
+    h = C.c_void_p()
+    assert capi.lib.arsim_engine_create(C.byref(v), 1, None, 0, C.byref(h)) == capi.ARSIM_ERR_NO_DEVICE
+    cfg = capi.ArSimServeConfig(seed=1, numRequests=2, arrivalRate=10.0, minParticles=2048, maxParticles=4096,
+                                minSteps=1, maxSteps=2, hbmBudgetBytes=1e9, maxResident=4, oneAtATime=0,
+                                timeScale=1.0, method=0)
+    rep = capi.ArSimServeReport()
+    assert capi.lib.arsim_serve_run(C.byref(cfg), C.byref(rep)) == capi.ARSIM_ERR_NO_DEVICE
+
+
+def test_generator_invariants():
+    sc = scenes.config0()
+    # bodies first, contiguous, in body order; then grains
+    assert np.all(sc.rb[:-1] <= np.where(sc.rb[1:] < 0, np.iinfo(np.int32).max, sc.rb[1:]))
+    assert np.array_equal(np.repeat(np.arange(sc.b), sc.ppo), sc.rb[sc.rb >= 0])
+    # no initial overlap between particles of different bodies
+    import oracle
+    assert len(oracle.contacts_brute(sc.pos, sc.rb, 0.5)) == 0
+    # particles are at CM + R(q) r_body
+    q = sc.rb_quat.astype(np.float64)
+    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y),
+                  2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x),
+                  2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y)], 1).reshape(-1, 3, 3)
+    b = sc.rb
+    world = sc.rb_pos[b, :3] + np.einsum("nji,nj->ni", R[b], sc.body_frame[:, :3].astype(np.float64))
+    assert np.allclose(world, sc.pos[:, :3], atol=1e-4)
+    # inside the box
+    assert sc.pos[:, :3].min() > 0 and sc.pos[:, :3].max() < 64
+
+
+def test_mix_scene_sizes():
+    for n in (1024, 5000, 65536):
+        sc = scenes.mix_scene(n, 1)
+        assert sc.n == n
+        assert abs((sc.rb >= 0).mean() - 0.75) < 0.05
