@@ -113,7 +113,14 @@ def main():
     sc = scenes.config1(method)
     n = sc.n
     eng = Engine(sc, use_graph=True)
-    log(f"engine ready: {n} particles, {sc.b} bodies, {eng.device_bytes / 2**20:.0f} MiB on device")
+    tuning = None
+    if not args.quick:
+        # block size / items per thread / digit width picked by the engine's autotuner (state is restored)
+        best, best_ms = eng.autotune(steps_per_trial=3)
+        eng.close()
+        eng = Engine(sc, dem_block=best[0], sort_ipt=best[1], sort_radix_bits=best[2], use_graph=True)
+        tuning = {"dem_block": best[0], "sort_items_per_thread": best[1], "sort_radix_bits": best[2]}
+    log(f"engine ready: {n} particles, {sc.b} bodies, {eng.device_bytes / 2**20:.0f} MiB on device, tuning {tuning}")
     eng.step(args.warmup)
     eng.sync()
 
@@ -230,7 +237,7 @@ def main():
         "config": {"workload": "plain single large scene: 2^20 particles (8,192 rigid 4x4x4 cubes + 524,288 "
                                "grains) in [0,512]^3, r=0.5, k=5e4, c=50, shear=10, dt=1e-4, 5 walls",
                    "method": args.method, "grid": "256^3 hashed" if method == capi.METHOD_GRID else None,
-                   "particles": n, "bodies": sc.b},
+                   "particles": n, "bodies": sc.b, "tuning": tuning},
         "step_latency_ms_p50": float(np.percentile(lat, 50)), "step_latency_ms_p99": float(np.percentile(lat, 99)),
         "stage_ms": stage_summary,
         "roofline": roofline,

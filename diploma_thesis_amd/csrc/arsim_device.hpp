@@ -36,6 +36,9 @@ struct DevPhys {
     float gx, gy, gz;
     float globalDamping, Wdamping, boundaryDamping;
     unsigned flags;
+    // a float >= (2r)^2 in exact arithmetic: d2 > cd2Reject implies sqrtf(d2) >= 2r, because sqrtf is
+    // correctly rounded and monotone and 2r is a float; the contact test skips the square root then
+    float cd2Reject;
 };
 
 constexpr unsigned kEmpty = 0xffffffffu;
@@ -125,7 +128,9 @@ __device__ __forceinline__ void sincos_det(float x, float *s_out, float *c_out) 
 // Spring-dashpot-shear(-attraction) pair force of collideSpheres (particleKernels.cu:261-299).
 __device__ __forceinline__ bool pair_force(f3 pa, f3 pb, f3 va, f3 vb, const DevPhys &ph, f3 &out) {
     f3 rel = sub3(pb, pa);
-    float dist = sqrtf(dot3(rel, rel));
+    const float d2 = dot3(rel, rel);
+    if (d2 > ph.cd2Reject) return false;   // certainly no contact; same outcome as the test below
+    float dist = sqrtf(d2);
     float cd = 2.0f * ph.r;
     if (!(dist < cd)) return false;
     f3 n = div3(rel, dist);
@@ -141,10 +146,12 @@ __device__ __forceinline__ bool pair_force(f3 pa, f3 pb, f3 va, f3 vb, const Dev
     return true;
 }
 // Contact predicate only (same arithmetic as the distance test above).
-__device__ __forceinline__ bool in_contact(f3 pa, f3 pb, float r) {
+__device__ __forceinline__ bool in_contact(f3 pa, f3 pb, const DevPhys &ph) {
     f3 rel = sub3(pb, pa);
-    float dist = sqrtf(dot3(rel, rel));
-    return dist < 2.0f * r;
+    const float d2 = dot3(rel, rel);
+    if (d2 > ph.cd2Reject) return false;
+    float dist = sqrtf(d2);
+    return dist < 2.0f * ph.r;
 }
 __device__ __forceinline__ bool pair_allowed(int ra, int rb) { return ra != rb || ra == -1; }
 

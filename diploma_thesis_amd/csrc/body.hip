@@ -80,17 +80,19 @@ template <int MODE>
 __device__ __forceinline__ void pose_key_one(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko,
                                              int doBodies, int doGrains, unsigned *shist);
 
-// Adds 1 to an LDS histogram bin.  Spatially ordered keys often share their high digits across a
-// whole wave; then one lane adds the wave's count instead of 64 lanes hitting one address.
+// Adds 1 to an LDS histogram bin for every active lane.  Spatially ordered keys share digits across
+// a wave, so lanes are grouped by digit (8 ballots find the lanes with the same 8-bit digit) and
+// the lowest lane of each group adds the group's size: at most one LDS atomic per distinct digit.
 __device__ __forceinline__ void hist_add(unsigned *h, unsigned d) {
-    const uint64_t active = __ballot(1);
-    const unsigned d0 = __shfl(d, __ffsll((long long)active) - 1, 64);
-    const uint64_t same = __ballot(d == d0);
-    if (same == active) {
-        if ((threadIdx.x & 63) == (unsigned)(__ffsll((long long)active) - 1)) atomicAdd(&h[d], (unsigned)__popcll(active));
-    } else {
-        atomicAdd(&h[d], 1u);
+    uint64_t peers = __ballot(1);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
     }
+    const unsigned lane = threadIdx.x & 63;
+    if ((unsigned)(__ffsll((long long)peers) - 1) == lane) atomicAdd(&h[d], (unsigned)__popcll(peers));
 }
 
 template <int MODE>

@@ -235,6 +235,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     } else {
         size_t nodes = (size_t)2 * N;
         A(tree.left, nodes); A(tree.right, nodes); A(tree.parent, nodes);
+        A(tree.packed, 4 * (size_t)N);
         A(tree.bounds, 2 * nodes); A(tree.cms, N); A(tree.sortedVel, N); A(tree.counter, N);
         if (S > 1) A(tree.sceneRoot, S);
         A(aabbPart, (size_t)2 * aabb_max_partials()); A(aabbBox, 2);
@@ -713,7 +714,7 @@ int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best,
     ArSimTuning bestT = orig;
     float bestT_ms = 1e30f;
     const int blocks[] = {64, 128, 256};
-    const int ipts[] = {8, 12, 16};
+    const int ipts[] = {4, 8, 12, 16};
     const int rbits[] = {7, 8};
     hipEvent_t a, b;
     hipEventCreate(&a);

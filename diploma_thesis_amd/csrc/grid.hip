@@ -75,7 +75,7 @@ __global__ void k_dem_grid(const float4 *__restrict__ sPos, const float4 *__rest
                     float4 Q = sPos[j];
                     if ((int)j == i || !pair_allowed(ra, __float_as_int(Q.w))) continue;
                     if (EMIT) {
-                        if (in_contact(pa, xyz(Q), ph.r)) {
+                        if (in_contact(pa, xyz(Q), ph)) {
                             if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = (int)sIdx[j];
                             ++cnt;
                         }
@@ -139,13 +139,9 @@ __global__ void k_reorder_seg(const unsigned *__restrict__ keys, const unsigned 
 // up to three consecutive cells), cell by cell in ascending order, each cell's contacts summed
 // into a partial that is added to F when the cell ends -- the same grouping and order as the
 // 27-cell loop over cellStart / cellEnd.
-// The sorted run is read in chunks of kChunk keys and positions issued together, so a row costs
-// one or two memory round trips instead of one per candidate.
-#ifndef ARSIM_DEM_CHUNK
-#define ARSIM_DEM_CHUNK 4
-#endif
-constexpr int kChunk = ARSIM_DEM_CHUNK;
-
+// Forward scan of the sorted keys from j (a segment start at or before h0): skips keys below h0,
+// stops past h1.  (Reading several candidates per round trip was measured slower: the extra
+// registers cost more occupancy than the latency it hid.)
 template <bool EMIT>
 __device__ __forceinline__ void scan_cells(const unsigned *__restrict__ keys, const float4 *__restrict__ sPos,
                                            const float4 *__restrict__ sVel, const unsigned *__restrict__ sIdx,
@@ -153,43 +149,26 @@ __device__ __forceinline__ void scan_cells(const unsigned *__restrict__ keys, co
                                            const DevPhys &ph, f3 &F, int &cnt, unsigned orig, const DemOut &o) {
     unsigned cur = 0xffffffffu;
     f3 csum = mk3(0.f, 0.f, 0.f);
-    bool done = false;
-    while (!done) {
-        unsigned kk[kChunk];
-        float4 qq[kChunk];
-#pragma unroll
-        for (int c = 0; c < kChunk; ++c) kk[c] = ((int)j + c < n) ? keys[j + c] : 0xffffffffu;
-#pragma unroll
-        for (int c = 0; c < kChunk; ++c) qq[c] = ((int)j + c < n) ? sPos[j + c] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int c = 0; c < kChunk; ++c) {
-            if (done) break;
-            const unsigned kj = kk[c];
-            if (kj < h0) continue;
-            if (kj > h1 || (int)j + c >= n) {
-                done = true;
-                break;
-            }
-            if (kj != cur) {
-                if (cur != 0xffffffffu) F = add3(F, csum);
-                csum = mk3(0.f, 0.f, 0.f);
-                cur = kj;
-            }
-            const unsigned jj = j + c;
-            const float4 Q = qq[c];
-            if ((int)jj != i && pair_allowed(ra, __float_as_int(Q.w))) {
-                if (EMIT) {
-                    if (in_contact(pa, xyz(Q), ph.r)) {
-                        if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = (int)sIdx[jj];
-                        ++cnt;
-                    }
-                } else {
-                    f3 fj;
-                    if (pair_force(pa, xyz(Q), va, xyz(sVel[jj]), ph, fj)) csum = add3(csum, fj);
-                }
-            }
+    for (; (int)j < n; ++j) {
+        const unsigned kj = keys[j];
+        if (kj > h1) break;
+        if (kj < h0) continue;
+        if (kj != cur) {
+            if (cur != 0xffffffffu) F = add3(F, csum);
+            csum = mk3(0.f, 0.f, 0.f);
+            cur = kj;
         }
-        j += kChunk;
+        const float4 Q = sPos[j];
+        if ((int)j == i || !pair_allowed(ra, __float_as_int(Q.w))) continue;
+        if (EMIT) {
+            if (in_contact(pa, xyz(Q), ph)) {
+                if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = (int)sIdx[j];
+                ++cnt;
+            }
+        } else {
+            f3 fj;
+            if (pair_force(pa, xyz(Q), va, xyz(sVel[j]), ph, fj)) csum = add3(csum, fj);
+        }
     }
     if (cur != 0xffffffffu) F = add3(F, csum);
 }

@@ -121,6 +121,9 @@ struct Lbvh {
     float4 *sortedVel;            // leaf velocities, w = scene id bits when batched
     int *counter;                 // N - 1 refit counters
     int *sceneRoot;               // per scene (batched) or NULL
+    // optional traversal layout: internal node k = node - N as 4 float4
+    // {Lmin.xyz, L}, {Lmax.xyz, R}, {Rmin.xyz, 0}, {Rmax.xyz, 0} (child ids as int bits)
+    float4 *packed;
 };
 // keyBytes 4 or 8; sceneShift > 0: the scene id is key >> sceneShift and equal-key ties use the
 // scene-local leaf index (sceneStart[scene] is subtracted).

@@ -76,11 +76,41 @@ __global__ void k_karras(const K *__restrict__ codes, int n, int sceneShift, con
     }
 }
 
+// Node bounds that other workgroups read during the refit travel as agent-scope atomic stores and
+// loads (coherent across the XCDs' L2s), so the counter add needs no L2 writeback / invalidate: the
+// writer waits for its stores to complete before its add, and the merging thread loads only after
+// its own add has returned the sibling's increment.
+__device__ __forceinline__ uint64_t pack2(float lo, float hi) {
+    return ((uint64_t)__float_as_uint(hi) << 32) | (uint64_t)__float_as_uint(lo);
+}
+__device__ __forceinline__ void store_bounds(float4 *bounds, size_t node, float4 mn, float4 mx) {
+    uint64_t *p = reinterpret_cast<uint64_t *>(bounds + 2 * node);
+    __hip_atomic_store(p + 0, pack2(mn.x, mn.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 1, pack2(mn.z, mn.w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 2, pack2(mx.x, mx.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 3, pack2(mx.z, mx.w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void load_bounds(float4 *bounds, size_t node, float4 &mn, float4 &mx) {
+    uint64_t *p = reinterpret_cast<uint64_t *>(bounds + 2 * node);
+    const uint64_t a = __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t b = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t c = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t d = __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    mn = make_float4(__uint_as_float((unsigned)a), __uint_as_float((unsigned)(a >> 32)), __uint_as_float((unsigned)b),
+                     __uint_as_float((unsigned)(b >> 32)));
+    mx = make_float4(__uint_as_float((unsigned)c), __uint_as_float((unsigned)(c >> 32)), __uint_as_float((unsigned)d),
+                     __uint_as_float((unsigned)(d >> 32)));
+}
+// Waits until this thread's stores have completed (the compiler may not move memory ops across it).
+__device__ __forceinline__ void stores_complete() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // kernelConstructLeafNodesSoA (BVHSoAKernels.cu:108-138) + kernelConstructInternalNodesSoA
 // (:148-176) in one launch: each leaf writes its centre (w = rbIndex bits), velocity copy and
-// AABB = c ± r, then walks up; the first arrival at a node stops, the second merges the
-// children.  The counter add is acquire-release at agent scope, so the sibling's bounds,
-// written before its add, are visible to the merging thread on any XCD.
+// AABB = c ± r, then walks up; the first arrival at a node stops, the second merges the children.
 __global__ void k_leaf_refit(const int *__restrict__ sIdx, const float4 *__restrict__ pos,
                              const float4 *__restrict__ vel, const int *__restrict__ rb,
                              const uint16_t *__restrict__ scene, float r, int n, Lbvh t) {
@@ -94,27 +124,36 @@ __global__ void k_leaf_refit(const int *__restrict__ sIdx, const float4 *__restr
         if (scene) v.w = __int_as_float((int)scene[o]);
         t.sortedVel[i] = v;
     }
-    t.bounds[2 * (size_t)i] = make_float4(p.x - r, p.y - r, p.z - r, 0.0f);
-    t.bounds[2 * (size_t)i + 1] = make_float4(p.x + r, p.y + r, p.z + r, 0.0f);
+    store_bounds(t.bounds, (size_t)i, make_float4(p.x - r, p.y - r, p.z - r, 0.0f),
+                 make_float4(p.x + r, p.y + r, p.z + r, 0.0f));
     if (t.isLeaf) t.isLeaf[i] = 1;
     if (t.minR) { t.minR[i] = i; t.maxR[i] = i; }
     if (n < 2) return;
     int node = t.parent[i];
-    int prev = __hip_atomic_fetch_add(&t.counter[node - n], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    stores_complete();
+    int prev = __hip_atomic_fetch_add(&t.counter[node - n], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     while (prev == 1) {
         const int L = t.left[node], R = t.right[node];
-        float4 aMin = t.bounds[2 * (size_t)L], aMax = t.bounds[2 * (size_t)L + 1];
-        float4 bMin = t.bounds[2 * (size_t)R], bMax = t.bounds[2 * (size_t)R + 1];
+        float4 aMin, aMax, bMin, bMax;
+        load_bounds(t.bounds, (size_t)L, aMin, aMax);
+        load_bounds(t.bounds, (size_t)R, bMin, bMax);
         float4 mn = make_float4(aMin.x < bMin.x ? aMin.x : bMin.x, aMin.y < bMin.y ? aMin.y : bMin.y,
                                 aMin.z < bMin.z ? aMin.z : bMin.z, 0.0f);
         float4 mx = make_float4(aMax.x > bMax.x ? aMax.x : bMax.x, aMax.y > bMax.y ? aMax.y : bMax.y,
                                 aMax.z > bMax.z ? aMax.z : bMax.z, 0.0f);
-        t.bounds[2 * (size_t)node] = mn;
-        t.bounds[2 * (size_t)node + 1] = mx;
+        store_bounds(t.bounds, (size_t)node, mn, mx);
+        if (t.packed) {
+            float4 *q = t.packed + 4 * (size_t)(node - n);
+            q[0] = make_float4(aMin.x, aMin.y, aMin.z, __int_as_float(L));
+            q[1] = make_float4(aMax.x, aMax.y, aMax.z, __int_as_float(R));
+            q[2] = make_float4(bMin.x, bMin.y, bMin.z, 0.0f);
+            q[3] = make_float4(bMax.x, bMax.y, bMax.z, 0.0f);
+        }
         const int up = t.parent[node];
         if (up < 0) break;
         node = up;
-        prev = __hip_atomic_fetch_add(&t.counter[node - n], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        stores_complete();
+        prev = __hip_atomic_fetch_add(&t.counter[node - n], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -146,15 +185,28 @@ __global__ void k_dem_lbvh(const int *__restrict__ sIdx, Lbvh t, int n, DevPhys 
     f3 F = mk3(0.f, 0.f, 0.f);
     int cnt = 0, sp = 0;
     while (node >= n) {
-        const int L = t.left[node], R = t.right[node];
-        const bool oL = L != i && overlap(qMin, qMax, t.bounds[2 * (size_t)L], t.bounds[2 * (size_t)L + 1]);
-        const bool oR = R != i && overlap(qMin, qMax, t.bounds[2 * (size_t)R], t.bounds[2 * (size_t)R + 1]);
+        int L, R;
+        bool oL, oR;
+        if (t.packed) {
+            // one 64-byte node: both children's boxes and ids
+            const float4 *q = t.packed + 4 * (size_t)(node - n);
+            const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3];
+            L = __float_as_int(a0.w);
+            R = __float_as_int(a1.w);
+            oL = L != i && overlap(qMin, qMax, a0, a1);
+            oR = R != i && overlap(qMin, qMax, b0, b1);
+        } else {
+            L = t.left[node];
+            R = t.right[node];
+            oL = L != i && overlap(qMin, qMax, t.bounds[2 * (size_t)L], t.bounds[2 * (size_t)L + 1]);
+            oR = R != i && overlap(qMin, qMax, t.bounds[2 * (size_t)R], t.bounds[2 * (size_t)R + 1]);
+        }
         const bool lL = L < n, lR = R < n;
         if (oL && lL) {
             float4 CL = t.cms[L];
             if (pair_allowed(ra, __float_as_int(CL.w))) {
                 if (EMIT) {
-                    if (in_contact(pa, xyz(CL), ph.r)) {
+                    if (in_contact(pa, xyz(CL), ph)) {
                         if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[L];
                         ++cnt;
                     }
@@ -168,7 +220,7 @@ __global__ void k_dem_lbvh(const int *__restrict__ sIdx, Lbvh t, int n, DevPhys 
             float4 CR = t.cms[R];
             if (pair_allowed(ra, __float_as_int(CR.w))) {
                 if (EMIT) {
-                    if (in_contact(pa, xyz(CR), ph.r)) {
+                    if (in_contact(pa, xyz(CR), ph)) {
                         if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[R];
                         ++cnt;
                     }

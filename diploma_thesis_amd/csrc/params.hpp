@@ -2,6 +2,7 @@
 // parameter structs (host side).
 #pragma once
 
+#include <cmath>
 #include <string>
 
 #include "../../include/arsim.h"
@@ -19,6 +20,12 @@ inline DevPhys make_phys(const ArSimParams &p, const ArSimStepConfig &c) {
     ph.gx = p.gravity.x; ph.gy = p.gravity.y; ph.gz = p.gravity.z;
     ph.globalDamping = p.globalDamping; ph.Wdamping = p.Wdamping; ph.boundaryDamping = p.boundaryDamping;
     ph.flags = c.flags;
+    // smallest float not below (2r)^2 in exact arithmetic (see DevPhys::cd2Reject)
+    const float cd = 2.0f * p.particleRadius;
+    const double cd2 = (double)cd * (double)cd;
+    float t = (float)cd2;
+    if ((double)t < cd2) t = std::nextafter(t, INFINITY);
+    ph.cd2Reject = t;
     return ph;
 }
 

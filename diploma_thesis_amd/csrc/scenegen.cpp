@@ -174,8 +174,19 @@ extern "C" int arsim_generate_scene(const ArSimGenSpec *spec, ArSimSceneOwned **
         pitch[1] = ((spec->dropHeightMax > 0.f) ? std::min(box[1], (double)spec->dropHeightMax) : box[1]) / cells[1];
         double minPitch = std::min(pitch[0], std::min(pitch[1], pitch[2]));
         if (minPitch < 2.0 * maxCirc) {
-            delete s;
-            return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "box too small for the bodies");
+            // dense pile: horizontal layers at the smallest non-overlapping pitch, stacked upwards
+            // from the floor (the container is open at the top)
+            const double need = 2.0 * maxCirc * 1.05;
+            cells[0] = (int)std::floor(box[0] / need);
+            cells[2] = (int)std::floor(box[2] / need);
+            if (cells[0] < 1 || cells[2] < 1) {
+                delete s;
+                return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "box too small for the bodies");
+            }
+            cells[1] = (int)((B + (long long)cells[0] * cells[2] - 1) / ((long long)cells[0] * cells[2]));
+            pitch[0] = box[0] / cells[0];
+            pitch[2] = box[2] / cells[2];
+            pitch[1] = need;
         }
         long long total = (long long)cells[0] * cells[1] * cells[2];
         std::vector<long long> order(total);

@@ -62,6 +62,24 @@ class HostScene:
         v.params, v.cfg = self.params, self.cfg
         return v
 
+    ARRAYS = ("pos", "vel", "rb", "body_frame", "rb_pos", "rb_vel", "rb_ang", "rb_quat", "rb_inertia", "rb_mass", "ppo")
+
+    def to_arrays(self):
+        """Plain arrays (and the raw SimParams / step-config bytes) for storing a scene."""
+        out = {k: getattr(self, k) for k in self.ARRAYS}
+        out["params_bytes"] = np.frombuffer(bytes(self.params), np.uint8).copy()
+        out["cfg_bytes"] = np.frombuffer(bytes(self.cfg), np.uint8).copy()
+        return out
+
+    @classmethod
+    def from_arrays(cls, d):
+        s = object.__new__(cls)
+        for k in cls.ARRAYS:
+            setattr(s, k, np.ascontiguousarray(d[k]))
+        s.params = capi.ArSimParams.from_buffer_copy(bytes(np.asarray(d["params_bytes"], np.uint8)))
+        s.cfg = capi.ArSimStepConfig.from_buffer_copy(bytes(np.asarray(d["cfg_bytes"], np.uint8)))
+        return s
+
     def copy(self):
         other = object.__new__(HostScene)
         for k, v in self.__dict__.items():
