@@ -154,6 +154,14 @@ __device__ __forceinline__ bool in_contact(f3 pa, f3 pb, const DevPhys &ph) {
     return dist < 2.0f * ph.r;
 }
 __device__ __forceinline__ bool pair_allowed(int ra, int rb) { return ra != rb || ra == -1; }
+// True when the sorted copy Q (xyz position, w = rbIndex bits) may be in contact with particle a:
+// not a itself, an allowed pair, and not certainly out of reach.  The three tests are combined
+// without short-circuiting so that all of Q is needed at once (one 16-byte load, not two).
+__device__ __forceinline__ bool candidate(float4 Q, f3 pa, int ra, bool self, const DevPhys &ph) {
+    const f3 rel = sub3(mk3(Q.x, Q.y, Q.z), pa);
+    const bool near = !(dot3(rel, rel) > ph.cd2Reject);
+    return near & !self & pair_allowed(ra, __float_as_int(Q.w));
+}
 
 // Static-plane walls: the static-contact formula with a plane partner, walls in bit order.
 __device__ __forceinline__ f3 wall_force(f3 p, f3 v, const DevPhys &ph, const SceneGeom &g) {

@@ -170,6 +170,8 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     pStart.resize(S); pCount.resize(S); bStart.resize(S); bCount.resize(S);
     long long nTot = 0, bTot = 0;
     int maxCellBits = 0;
+    long long totalCells = 0;
+    std::vector<long long> sceneCells(S, 0);
     for (int s = 0; s < S; ++s) {
         if (d[s].c.method != method) return fail(ARSIM_ERR_INVALID_ARG, "engine: scenes mix methods");
         if (!same_phys(make_phys(d[s].p, d[s].c), ph))
@@ -183,6 +185,8 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
             long long cells = (long long)d[s].p.gridSize.x * d[s].p.gridSize.y * d[s].p.gridSize.z;
             if (cells <= 0) return fail(ARSIM_ERR_INVALID_ARG, "engine: grid has no cells");
             maxCellBits = std::max(maxCellBits, ceil_log2(cells));
+            sceneCells[s] = cells;
+            totalCells += cells;
         }
     }
     if (nTot >= (1ll << 30)) return fail(ARSIM_ERR_INVALID_ARG, "engine: too many particles");
@@ -190,15 +194,21 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     B = (int)bTot;
     sceneBits = S > 1 ? ceil_log2(S) : 0;
     if (method == ARSIM_METHOD_GRID) {
+        // scenes' cell ranges are packed back to back: key = cellBase[scene] + cell hash
         cellBits = maxCellBits;
-        keyBits = cellBits + sceneBits;
-        if (keyBits > 32) return fail(ARSIM_ERR_OVERFLOW, "engine: scene id + cell bits exceed 32");
+        if (totalCells > (1ll << 32)) return fail(ARSIM_ERR_OVERFLOW, "engine: total cells of all scenes exceed 2^32");
+        keyBits = std::max(1, ceil_log2(totalCells));
         keyBytes = 4;
-        numCells = (size_t)1 << keyBits;
-        segBits = std::min(6, std::max(3, keyBits - 21));
-        if (segBits > keyBits) segBits = keyBits;
-        numSegs = (long long)(numCells >> segBits);
-        for (int s = 0; s < S; ++s) geoms[s].cellBase = (unsigned)s << cellBits;
+        numCells = (size_t)totalCells;
+        // one entry per cell (the DEM then reads no keys) up to 2^27 cells = 512 MiB of table;
+        // beyond that, segments of 2^segBits cells keep the table at that size
+        segBits = keyBits > 27 ? keyBits - 27 : 0;
+        numSegs = (long long)((numCells - 1) >> segBits) + 1;
+        long long base = 0;
+        for (int s = 0; s < S; ++s) {
+            geoms[s].cellBase = (unsigned)base;
+            base += sceneCells[s];
+        }
     } else if (method == ARSIM_METHOD_LBVH30 && S == 1) {
         keyBits = 30; keyBytes = 4;
     } else if (method == ARSIM_METHOD_LBVH63) {

@@ -73,7 +73,7 @@ __global__ void k_dem_grid(const float4 *__restrict__ sPos, const float4 *__rest
                 f3 csum = mk3(0.f, 0.f, 0.f);
                 for (unsigned j = st; j < en; ++j) {
                     float4 Q = sPos[j];
-                    if ((int)j == i || !pair_allowed(ra, __float_as_int(Q.w))) continue;
+                    if (!candidate(Q, pa, ra, (int)j == i, ph)) continue;
                     if (EMIT) {
                         if (in_contact(pa, xyz(Q), ph)) {
                             if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = (int)sIdx[j];
@@ -109,23 +109,40 @@ __device__ __forceinline__ void fill_range(unsigned *p, long long a, long long b
     for (; k < b; ++k) p[k] = v;
 }
 
+// Fills p[a, b) with v for every lane that has a range: short ranges by the lane itself, long
+// ones (empty-cell runs, the tail of the table) by the whole wave with coalesced stores.  Every
+// lane of the wave must call it.
+__device__ __forceinline__ void wave_fill(unsigned *p, bool has, unsigned a, unsigned b, unsigned v) {
+    const bool lng = has && b - a > 32u;
+    if (has && !lng) fill_range(p, a, b, v);
+    unsigned long long m = __ballot(lng);
+    const unsigned lane = threadIdx.x & 63u;
+    while (m) {
+        const int l = __builtin_ctzll(m);
+        m &= m - 1;
+        const unsigned la = __shfl(a, l), lb = __shfl(b, l), lv = __shfl(v, l);
+        for (unsigned k = la + lane; k < lb; k += 64u) p[k] = lv;
+    }
+}
+
 // Engine form of the reorder: the same gather of the sorted copies, but instead of the two
 // cell-sized tables it writes a dense CSR offset array over segments of 2^segBits cells:
 // segStart[s] = first sorted index whose key >= s << segBits (segStart[numSegs] = n).  Every entry
-// is rewritten each step, so nothing has to be cleared, and the table is 2^segBits x smaller than
-// cellStart + cellEnd.
+// is rewritten each step, so nothing has to be cleared.  With segBits = 0 it is cellStart with
+// cellEnd[c] = segStart[c + 1], one table instead of two.  numSegs < 2^32.
 __global__ void k_reorder_seg(const unsigned *__restrict__ keys, const unsigned *__restrict__ idx,
                               const float4 *__restrict__ pos, const float4 *__restrict__ vel,
                               const int *__restrict__ rb, const uint16_t *__restrict__ scene,
                               float4 *__restrict__ sPos, float4 *__restrict__ sVel, unsigned *__restrict__ segStart,
                               int segBits, long long numSegs, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const unsigned k = keys[i];
-    const long long s = (long long)(k >> segBits);
-    const long long sp = i == 0 ? -1 : (long long)(keys[i - 1] >> segBits);
-    if (s != sp) fill_range(segStart, sp + 1, s + 1, (unsigned)i);
-    if (i == n - 1) fill_range(segStart, s + 1, numSegs + 1, (unsigned)n);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const unsigned k = valid ? keys[i] : 0u;
+    const unsigned s = k >> segBits;
+    const unsigned sp1 = (valid && i > 0) ? (keys[i - 1] >> segBits) + 1u : 0u;   // first segment not yet started
+    wave_fill(segStart, valid && s + 1u > sp1, sp1, s + 1u, (unsigned)i);
+    wave_fill(segStart, i == n - 1, s + 1u, (unsigned)numSegs + 1u, (unsigned)n);
+    if (!valid) return;
     const unsigned o = idx[i];
     float4 p = pos[o];
     p.w = __int_as_float(rb[o]);
@@ -159,7 +176,7 @@ __device__ __forceinline__ void scan_cells(const unsigned *__restrict__ keys, co
             cur = kj;
         }
         const float4 Q = sPos[j];
-        if ((int)j == i || !pair_allowed(ra, __float_as_int(Q.w))) continue;
+        if (!candidate(Q, pa, ra, (int)j == i, ph)) continue;
         if (EMIT) {
             if (in_contact(pa, xyz(Q), ph)) {
                 if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = (int)sIdx[j];
@@ -173,10 +190,42 @@ __device__ __forceinline__ void scan_cells(const unsigned *__restrict__ keys, co
     if (cur != 0xffffffffu) F = add3(F, csum);
 }
 
+// Contacts of one sorted particle with the sorted run [j, e) that holds up to three consecutive
+// cells, the second starting at b1 and the third at b2 (exact cell offsets: no key is read).
+// A cell's partial is added to F when a non-empty cell ends, as in the 27-cell loop.
+template <bool EMIT>
+__device__ __forceinline__ void scan_run(const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
+                                         const unsigned *__restrict__ sIdx, unsigned j, unsigned b1, unsigned b2,
+                                         unsigned e, int i, int ra, f3 pa, f3 va, const DevPhys &ph, f3 &F, int &cnt,
+                                         unsigned orig, const DemOut &o) {
+    const unsigned j0 = j;
+    f3 csum = mk3(0.f, 0.f, 0.f);
+    for (; j < e; ++j) {
+        if (j != j0 && (j == b1 || j == b2)) {
+            F = add3(F, csum);
+            csum = mk3(0.f, 0.f, 0.f);
+        }
+        const float4 Q = sPos[j];
+        if (!candidate(Q, pa, ra, (int)j == i, ph)) continue;
+        if (EMIT) {
+            if (in_contact(pa, xyz(Q), ph)) {
+                if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = (int)sIdx[j];
+                ++cnt;
+            }
+        } else {
+            f3 fj;
+            if (pair_force(pa, xyz(Q), va, xyz(sVel[j]), ph, fj)) csum = add3(csum, fj);
+        }
+    }
+    if (e > j0) F = add3(F, csum);
+}
+
 // Engine DEM over the segment index: for each of the 9 (dz, dy) rows the three x-cells are
 // consecutive keys unless x wraps, so one segment lookup and one forward scan of the sorted
-// keys visit them in the order x-1, x, x+1.
-template <bool EMIT>
+// keys visit them in the order x-1, x, x+1.  EXACT (segments of one cell, segStart = cellStart
+// with numCells + 1 entries): a row is the run [segStart[h0], segStart[h0 + 3]) with its cell
+// boundaries read from the table, and the keys are not read at all.
+template <bool EMIT, bool EXACT>
 __global__ void k_dem_grid_seg(const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
                                const unsigned *__restrict__ keys, const unsigned *__restrict__ sIdx,
                                const unsigned *__restrict__ segStart, int segBits, int n, DevPhys ph, SceneView sv,
@@ -205,23 +254,42 @@ __global__ void k_dem_grid_seg(const float4 *__restrict__ sPos, const float4 *__
         const unsigned uy = (unsigned)(gy + (r % 3) - 1) & g.gridMask[1];
         return g.cellBase + uz * g.gxy + uy * g.gx;
     };
-    unsigned rowJ[9];
-    if (contiguous) {
+    if (EXACT) {
+        for (int r = 0; r < 9; ++r) {
+            const unsigned row = row_of(r);
+            if (contiguous) {
+                const unsigned h0 = row + x0;
+                scan_run<EMIT>(sPos, sVel, sIdx, segStart[h0], segStart[h0 + 1], segStart[h0 + 2], segStart[h0 + 3], i,
+                               ra, pa, va, ph, F, cnt, orig, o);
+            } else {
+                const unsigned xs[3] = {x0, x1, x2};
 #pragma unroll
-        for (int r = 0; r < 9; ++r) rowJ[r] = segStart[(row_of(r) + x0) >> segBits];
-    }
-    for (int r = 0; r < 9; ++r) {
-        const unsigned row = row_of(r);
+                for (int k = 0; k < 3; ++k) {
+                    const unsigned h = row + xs[k];
+                    const unsigned e = segStart[h + 1];
+                    scan_run<EMIT>(sPos, sVel, sIdx, segStart[h], e, e, e, i, ra, pa, va, ph, F, cnt, orig, o);
+                }
+            }
+        }
+    } else {
+        unsigned rowJ[9];
         if (contiguous) {
-            const unsigned h0 = row + x0;
-            scan_cells<EMIT>(keys, sPos, sVel, sIdx, rowJ[r], n, h0, h0 + 2u, i, ra, pa, va, ph, F, cnt, orig, o);
-        } else {
-            const unsigned xs[3] = {x0, x1, x2};
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const unsigned h = row + xs[k];
-                scan_cells<EMIT>(keys, sPos, sVel, sIdx, segStart[h >> segBits], n, h, h, i, ra, pa, va, ph, F, cnt,
-                                 orig, o);
+            for (int r = 0; r < 9; ++r) rowJ[r] = segStart[(row_of(r) + x0) >> segBits];
+        }
+        for (int r = 0; r < 9; ++r) {
+            const unsigned row = row_of(r);
+            if (contiguous) {
+                const unsigned h0 = row + x0;
+                scan_cells<EMIT>(keys, sPos, sVel, sIdx, rowJ[r], n, h0, h0 + 2u, i, ra, pa, va, ph, F, cnt, orig, o);
+            } else {
+                const unsigned xs[3] = {x0, x1, x2};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const unsigned h = row + xs[k];
+                    scan_cells<EMIT>(keys, sPos, sVel, sIdx, segStart[h >> segBits], n, h, h, i, ra, pa, va, ph, F,
+                                     cnt, orig, o);
+                }
             }
         }
     }
@@ -256,12 +324,22 @@ void launch_dem_grid_seg(const float4 *sortedPos, const float4 *sortedVel, const
     if (n <= 0) return;
     if (block <= 0) block = 128;
     dim3 grid((n + block - 1) / block);
-    if (o.pairs)
-        hipLaunchKernelGGL(k_dem_grid_seg<true>, grid, dim3(block), 0, st, sortedPos, sortedVel, sortedKeys, sortedIdx,
-                           segStart, segBits, n, ph, sv, sceneInVelW ? 1 : 0, o);
-    else
-        hipLaunchKernelGGL(k_dem_grid_seg<false>, grid, dim3(block), 0, st, sortedPos, sortedVel, sortedKeys, sortedIdx,
-                           segStart, segBits, n, ph, sv, sceneInVelW ? 1 : 0, o);
+    const int w = sceneInVelW ? 1 : 0;
+    if (o.pairs) {
+        if (segBits == 0)
+            hipLaunchKernelGGL((k_dem_grid_seg<true, true>), grid, dim3(block), 0, st, sortedPos, sortedVel, sortedKeys,
+                               sortedIdx, segStart, segBits, n, ph, sv, w, o);
+        else
+            hipLaunchKernelGGL((k_dem_grid_seg<true, false>), grid, dim3(block), 0, st, sortedPos, sortedVel,
+                               sortedKeys, sortedIdx, segStart, segBits, n, ph, sv, w, o);
+    } else {
+        if (segBits == 0)
+            hipLaunchKernelGGL((k_dem_grid_seg<false, true>), grid, dim3(block), 0, st, sortedPos, sortedVel,
+                               sortedKeys, sortedIdx, segStart, segBits, n, ph, sv, w, o);
+        else
+            hipLaunchKernelGGL((k_dem_grid_seg<false, false>), grid, dim3(block), 0, st, sortedPos, sortedVel,
+                               sortedKeys, sortedIdx, segStart, segBits, n, ph, sv, w, o);
+    }
 }
 
 void launch_reorder(const unsigned *sortedKeys, const unsigned *sortedIdx, const float4 *pos, const float4 *vel,

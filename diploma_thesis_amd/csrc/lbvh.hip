@@ -204,7 +204,7 @@ __global__ void k_dem_lbvh(const int *__restrict__ sIdx, Lbvh t, int n, DevPhys 
         const bool lL = L < n, lR = R < n;
         if (oL && lL) {
             float4 CL = t.cms[L];
-            if (pair_allowed(ra, __float_as_int(CL.w))) {
+            if (candidate(CL, pa, ra, false, ph)) {
                 if (EMIT) {
                     if (in_contact(pa, xyz(CL), ph)) {
                         if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[L];
@@ -218,7 +218,7 @@ __global__ void k_dem_lbvh(const int *__restrict__ sIdx, Lbvh t, int n, DevPhys 
         }
         if (oR && lR) {
             float4 CR = t.cms[R];
-            if (pair_allowed(ra, __float_as_int(CR.w))) {
+            if (candidate(CR, pa, ra, false, ph)) {
                 if (EMIT) {
                     if (in_contact(pa, xyz(CR), ph)) {
                         if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[R];

@@ -77,3 +77,30 @@ def test_ellipsoid_pile_steps(method):
     s64 = oracle.State(sc, np.float64)
     s64.step(10)
     assert np.max(np.abs(g["rb_ang"][:, :3] - s64.rb_ang[:, :3])) < 1e-4 * max(1.0, np.abs(s64.rb_ang).max())
+
+
+def test_engine_cell_index_is_cell_start():
+    """The engine's cell table after a step: scenes' cell ranges packed back to back, and
+    table[c] = first sorted index with key >= c for every cell c and c = numCells (exact CSR)."""
+    print("engine: cell index", flush=True)
+    mix = [scenes.mix_scene(n, 70 + n, capi.METHOD_GRID) for n in (1500, 9000, 3000, 40000)]
+    cells = [s.params.gridSize.x * s.params.gridSize.y * s.params.gridSize.z for s in mix]
+    assert len(set(cells)) > 1
+    total = int(sum(cells))
+    n = sum(s.n for s in mix)
+    with Engine(mix, use_graph=False) as e:
+        e.step(1)
+        e.sync()
+        kp, tp = C.c_void_p(), C.c_void_p()
+        nul = None
+        capi.check(L.arsim_engine_debug_buffers(e.h, C.byref(kp), nul, C.byref(tp), nul, nul, nul, nul, nul, nul, nul))
+        keys = np.zeros(n, np.uint32)
+        table = np.zeros(total + 1, np.uint32)
+        capi.check(L.arsim_memcpy_dtoh(capi.ptr(keys), kp, keys.nbytes, None))
+        capi.check(L.arsim_memcpy_dtoh(capi.ptr(table), tp, table.nbytes, None))
+    assert np.all(keys[1:] >= keys[:-1]) and int(keys[-1]) < total
+    base = np.concatenate([[0], np.cumsum(cells)])
+    counts = np.histogram(keys, bins=base)[0]
+    assert counts.tolist() == [s.n for s in mix]
+    expect = np.searchsorted(keys, np.arange(total + 1, dtype=np.int64), "left").astype(np.uint32)
+    assert np.array_equal(table, expect)
