@@ -7,6 +7,7 @@ namespace arsim {
 namespace {
 
 constexpr int kStackDepth = 32;   // per-thread LDS stack entries
+constexpr int kWaveStackDepth = 64;   // per-wave LDS stack entries of the packet traversal
 
 __device__ __forceinline__ int xcd_block(int b, int nb) {
     const int X = 8;
@@ -263,6 +264,111 @@ __global__ void k_dem_lbvh(const int *__restrict__ sIdx, Lbvh t, int n, DevPhys 
     }
 }
 
+// Packet form of the same traversal: the 64 lanes of a wave (64 consecutive leaves in Morton
+// order, so spatially close) walk the tree together.  The wave visits a node when any of its lanes'
+// boxes overlaps it; the node id, the stack and the packed node loads are wave-uniform.  A lane
+// handles a leaf only when its own box overlaps the leaf's, and every ancestor's box contains the
+// leaf's box, so each lane meets exactly the leaves of its own traversal, in the same depth-first
+// order (left before right): the per-lane force sums are bit-identical.  Lanes of one wave that
+// start from different scene roots (batched forests) are served one root after another.
+template <bool EMIT>
+__global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__restrict__ packed,
+                                const float4 *__restrict__ cms, const float4 *__restrict__ sVel,
+                                const float4 *__restrict__ bounds, const int *__restrict__ sceneRoot, int n, DevPhys ph,
+                                SceneView sv, int sceneInVelW, DemOut o, int *overflow) {
+    extern __shared__ int wStack[];
+    const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int i = lb * (int)blockDim.x + (int)threadIdx.x;
+    const bool valid = i < n;
+    const int ii = valid ? i : n - 1;
+    int *stack = wStack + ((int)threadIdx.x >> 6) * kWaveStackDepth;
+    const float4 C = cms[ii];
+    const float4 Vq = sVel[ii];
+    const int ra = __float_as_int(C.w);
+    const int s = sceneInVelW ? __float_as_int(Vq.w) : 0;
+    const float4 qMin = bounds[2 * (size_t)ii], qMax = bounds[2 * (size_t)ii + 1];
+    const f3 pa = xyz(C), va = xyz(Vq);
+    const int orig = sIdx[ii];
+    const int myRoot = sceneRoot ? sceneRoot[s] : (n >= 2 ? n : -1);
+    f3 F = mk3(0.f, 0.f, 0.f);
+    int cnt = 0;
+    unsigned long long pending = __ballot(valid && myRoot >= n);
+    while (pending) {
+        const int root = __builtin_amdgcn_readfirstlane(__shfl(myRoot, __builtin_ctzll(pending)));
+        const bool act = valid && myRoot == root;
+        pending &= ~__ballot(act);
+        int node = root, sp = 0;
+        while (true) {
+            const float4 *q = packed + 4 * (size_t)(node - n);
+            const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3];
+            const int L = __builtin_amdgcn_readfirstlane(__float_as_int(a0.w));
+            const int R = __builtin_amdgcn_readfirstlane(__float_as_int(a1.w));
+            const bool oL = act && L != i && overlap(qMin, qMax, a0, a1);
+            const bool oR = act && R != i && overlap(qMin, qMax, b0, b1);
+            if (L < n && oL) {
+                const float4 CL = cms[L];
+                if (candidate(CL, pa, ra, false, ph)) {
+                    if (EMIT) {
+                        if (in_contact(pa, xyz(CL), ph)) {
+                            if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[L];
+                            ++cnt;
+                        }
+                    } else {
+                        f3 fj;
+                        if (pair_force(pa, xyz(CL), va, xyz(sVel[L]), ph, fj)) F = add3(F, fj);
+                    }
+                }
+            }
+            if (R < n && oR) {
+                const float4 CR = cms[R];
+                if (candidate(CR, pa, ra, false, ph)) {
+                    if (EMIT) {
+                        if (in_contact(pa, xyz(CR), ph)) {
+                            if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[R];
+                            ++cnt;
+                        }
+                    } else {
+                        f3 fj;
+                        if (pair_force(pa, xyz(CR), va, xyz(sVel[R]), ph, fj)) F = add3(F, fj);
+                    }
+                }
+            }
+            const bool tL = L >= n && __ballot(oL) != 0ull;
+            const bool tR = R >= n && __ballot(oR) != 0ull;
+            if (!tL && !tR) {
+                if (sp == 0) break;
+                --sp;
+                node = __builtin_amdgcn_readfirstlane(stack[sp]);
+            } else {
+                node = tL ? L : R;
+                if (tL && tR) {
+                    if (sp == kWaveStackDepth) {
+                        if (act) atomicOr(overflow, 1);
+                        break;
+                    }
+                    stack[sp] = R;
+                    ++sp;
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    if (EMIT) {
+        o.counts[orig] = cnt;
+        return;
+    }
+    SceneGeom g;
+    load_geom(sv, s, g);
+    if (!((ph.flags & kGrainClamp) && ra < 0)) F = add3(F, wall_force(pa, va, ph, g));
+    if (ra >= 0) {
+        o.pForce[orig] = f4(F, 0.0f);
+        if (o.pTorque) o.pTorque[orig] = f4(cross3(xyz(o.relPos[orig]), F), 0.0f);
+    } else {
+        const float fs = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
+        o.vel[orig] = make_float4(va.x + F.x * fs, va.y + F.y * fs, va.z + F.z * fs, 0.0f);
+    }
+}
+
 }  // namespace
 
 void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShift, const int *sceneStart,
@@ -293,6 +399,19 @@ void launch_dem_lbvh(const int *sortedIdx, const Lbvh &t, int n, const DevPhys &
     if (n <= 0) return;
     if (block <= 0) block = 128;
     dim3 grid((n + block - 1) / block);
+    if (t.packed) {
+        block = (block + 63) & ~63;
+        grid = dim3((n + block - 1) / block);
+        const size_t wl = (size_t)kWaveStackDepth * (block / 64) * sizeof(int);
+        const int w = sceneInVelW ? 1 : 0;
+        if (o.pairs)
+            hipLaunchKernelGGL(k_dem_lbvh_wave<true>, grid, dim3(block), wl, st, sortedIdx, t.packed, t.cms,
+                               t.sortedVel, t.bounds, t.sceneRoot, n, ph, sv, w, o, overflow);
+        else
+            hipLaunchKernelGGL(k_dem_lbvh_wave<false>, grid, dim3(block), wl, st, sortedIdx, t.packed, t.cms,
+                               t.sortedVel, t.bounds, t.sceneRoot, n, ph, sv, w, o, overflow);
+        return;
+    }
     size_t lds = (size_t)kStackDepth * block * sizeof(int);
     if (o.pairs)
         hipLaunchKernelGGL(k_dem_lbvh<true>, grid, dim3(block), lds, st, sortedIdx, t, n, ph, sv, sceneInVelW ? 1 : 0,
