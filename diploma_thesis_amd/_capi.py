@@ -161,6 +161,14 @@ SIGNATURES = {
     "arsim_scene_view": (I, [VP, VP]),
     "arsim_scene_free": (I, [VP]),
     "arsim_serve_run": (I, [VP, VP]),
+    "arsim_template_suffix": (C.c_char_p, [F]),
+    "arsim_template_load": (I, [C.c_char_p, C.c_char_p, F, F, PVP]),
+    "arsim_template_info": (I, [VP, C.POINTER(I), VP, VP, C.POINTER(F), C.POINTER(F)]),
+    "arsim_template_free": (I, [VP]),
+    "arsim_scene_create": (I, [VP, VP, PVP]),
+    "arsim_scene_add_body": (I, [VP, VP, VP, VP, VP]),
+    "arsim_scene_add_grains": (I, [VP, I, VP, VP]),
+    "arsim_engine_snapshot": (I, [VP, I, PVP]),
 }
 
 

@@ -17,7 +17,7 @@ struct SceneGeom {
     float cellSize[3];
     unsigned gridMask[3];    // gridSize - 1 per axis (sizes are powers of two)
     unsigned gx, gxy;        // gridSize.x, gridSize.x * gridSize.y
-    unsigned cellBase;       // first global cell of this scene (scene << cellBits)
+    unsigned cellBase;       // first global cell of this scene (scenes' cell ranges are packed)
     unsigned wallMask;
     float boxMin[3], boxMax[3];
     float mLo[3], mExt[3];   // Morton normalisation: lo and (hi - lo), computed in FP32 on the host

@@ -14,11 +14,13 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
 #include "host_common.hpp"
 #include "params.hpp"
+#include "scene_owned.hpp"
 
 using arsim_host::fail;
 
@@ -623,6 +625,62 @@ int arsim_engine_download(arsim_engine *e, int s, float *pos, float *vel, float 
         if (rbQuaternion) ARSIM_HIP(hipMemcpy(rbQuaternion, E.qcum + bo, b * 16, hipMemcpyDeviceToHost));
         if (rbCurrentInertia) ARSIM_HIP(hipMemcpy(rbCurrentInertia, E.Iw + bo * 9, b * 36, hipMemcpyDeviceToHost));
     }
+    return ARSIM_OK;
+}
+
+// A host copy of scene s that rebuilds the same state: the per-step state (particle pos / vel, body
+// position, velocity, angular velocity, cumulative orientation) plus the scene's fixed data (body
+// frames, inverse body inertia, masses, particle counts, grid, walls, the current dt).  Quantities a
+// step recomputes before reading them (relativePos, the world inertia, the incremental quaternion,
+// forces) are not part of it.
+int arsim_engine_snapshot(arsim_engine *e, int s, ArSimSceneOwned **out) {
+    if (!e || !out || s < 0 || s >= e->e.S) return fail(ARSIM_ERR_INVALID_ARG, "engine_snapshot: bad arguments");
+    *out = nullptr;
+    Engine &E = e->e;
+    ARSIM_HIP(hipSetDevice(E.device));
+    ARSIM_HIP(hipStreamSynchronize(E.st));
+    const size_t po = E.pStart[s], n = E.pCount[s], bo = E.bStart[s], b = E.bCount[s];
+    ArSimSceneOwned *o = new (std::nothrow) ArSimSceneOwned();
+    if (!o) return fail(ARSIM_ERR_OUT_OF_MEMORY, "engine_snapshot");
+    o->N = (int)n;
+    o->B = (int)b;
+    o->pos.resize(4 * n); o->vel.resize(4 * n); o->bodyFrame.resize(4 * n); o->rb.resize(n);
+    o->rbPos.resize(4 * b); o->rbVel.resize(4 * b); o->rbAng.resize(4 * b); o->rbQuat.resize(4 * b);
+    o->rbInertia.resize(9 * b); o->rbMass.resize(b); o->ppo.resize(b);
+    auto get = [&](void *dst, const void *src, size_t bytes) {
+        return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) : hipSuccess;
+    };
+    hipError_t err = hipSuccess;
+    const struct { void *dst; const void *src; size_t bytes; } copies[] = {
+        {o->pos.data(), E.pos + po, n * 16}, {o->vel.data(), E.vel + po, n * 16},
+        {o->bodyFrame.data(), E.bodyFrame + po, n * 16}, {o->rb.data(), E.rb + po, n * 4},
+        {o->rbPos.data(), E.rbPos + bo, b * 16}, {o->rbVel.data(), E.rbVel + bo, b * 16},
+        {o->rbAng.data(), E.rbAng + bo, b * 16}, {o->rbQuat.data(), E.qcum + bo, b * 16},
+        {o->rbInertia.data(), E.Ib + bo * 9, b * 36}, {o->rbMass.data(), E.mass + bo, b * 4},
+        {o->ppo.data(), E.pcountD + bo, b * 4}};
+    for (const auto &c : copies)
+        if (err == hipSuccess) err = get(c.dst, c.src, c.bytes);
+    if (err != hipSuccess) {
+        delete o;
+        return fail(ARSIM_ERR_HIP, std::string("engine_snapshot: ") + hipGetErrorString(err));
+    }
+    for (int &v : o->rb)
+        if (v >= 0) v -= (int)bo;
+    const arsim::SceneGeom &g = E.geoms[s];
+    o->params = E.p0;
+    o->params.gridSize = {g.gridMask[0] + 1u, g.gridMask[1] + 1u, g.gridMask[2] + 1u};
+    o->params.numCells = o->params.gridSize.x * o->params.gridSize.y * o->params.gridSize.z;
+    o->params.worldOrigin = {g.origin[0], g.origin[1], g.origin[2]};
+    o->params.cellSize = {g.cellSize[0], g.cellSize[1], g.cellSize[2]};
+    o->params.numBodies = (unsigned)b;
+    o->cfg = E.c0;
+    o->cfg.dt = E.ph.dt;
+    for (int a = 0; a < 3; ++a) {
+        o->cfg.boxMin[a] = g.boxMin[a];
+        o->cfg.boxMax[a] = g.boxMax[a];
+    }
+    o->cfg.wallMask = g.wallMask;
+    *out = o;
     return ARSIM_OK;
 }
 

@@ -15,14 +15,7 @@
 
 #include "../../include/arsim.h"
 #include "host_common.hpp"
-
-struct ArSimSceneOwned {
-    int N = 0, B = 0;
-    std::vector<float> pos, vel, bodyFrame, rbPos, rbVel, rbAng, rbQuat, rbInertia, rbMass;
-    std::vector<int> rb, ppo;
-    ArSimParams params{};
-    ArSimStepConfig cfg{};
-};
+#include "scene_owned.hpp"
 
 namespace {
 

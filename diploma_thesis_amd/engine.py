@@ -76,6 +76,13 @@ class Engine:
                                                   P(out["rb_iw"])), "engine_download")
         return out
 
+    def snapshot(self, scene=0):
+        """A HostScene that rebuilds scene `scene` in its current state (arsim_engine_snapshot)."""
+        from .scenes import _from_owned
+        owned = C.c_void_p()
+        capi.check(capi.lib.arsim_engine_snapshot(self.h, int(scene), C.byref(owned)), "engine_snapshot")
+        return _from_owned(owned)
+
     def profile(self):
         ms = (C.c_float * 16)()
         names = (C.c_char_p * 16)()

@@ -162,3 +162,118 @@ def batch_mix(num_scenes=512, lo=1024, hi=65536, seed=0, method=capi.METHOD_GRID
     rng = np.random.default_rng(seed)
     counts = np.exp(rng.uniform(math.log(lo), math.log(hi), size=num_scenes)).round().astype(int)
     return [mix_scene(int(n), seed * 100003 + k, method) for k, n in enumerate(counts)]
+
+
+class Template:
+    """A particle template loaded as the reference's initObject does (arsim_template_load)."""
+
+    def __init__(self, data_dir, model, scale, radius):
+        self.h = C.c_void_p()
+        capi.check(capi.lib.arsim_template_load(str(data_dir).encode(), model.encode(), float(scale), float(radius),
+                                                C.byref(self.h)), "template_load")
+        n = C.c_int(0)
+        capi.check(capi.lib.arsim_template_info(self.h, C.byref(n), None, None, None, None), "template_info")
+        self.n = n.value
+        self.rel = np.zeros((self.n, 4), np.float32)
+        self.inv_inertia = np.zeros(9, np.float32)
+        rad, mass = C.c_float(0), C.c_float(0)
+        capi.check(capi.lib.arsim_template_info(self.h, None, capi.ptr(self.rel), capi.ptr(self.inv_inertia),
+                                                C.byref(rad), C.byref(mass)), "template_info")
+        self.radius, self.mass = rad.value, mass.value
+
+    def close(self):
+        if self.h:
+            capi.lib.arsim_template_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def template_suffix(scale):
+    return capi.lib.arsim_template_suffix(float(scale)).decode()
+
+
+class SceneBuilder:
+    """An empty scene filled with template bodies and free grains (arsim_scene_create / add_*)."""
+
+    def __init__(self, params, cfg):
+        self.h = C.c_void_p()
+        capi.check(capi.lib.arsim_scene_create(C.byref(params), C.byref(cfg), C.byref(self.h)), "scene_create")
+
+    def add_body(self, template, pos, vel=(0.0, 0.0, 0.0), ang=(0.0, 0.0, 0.0)):
+        f3 = lambda v: np.ascontiguousarray(v, np.float32)
+        p, v, w = f3(pos), f3(vel), f3(ang)
+        capi.check(capi.lib.arsim_scene_add_body(self.h, template.h, capi.ptr(p), capi.ptr(v), capi.ptr(w)),
+                   "scene_add_body")
+
+    def add_grains(self, pos4, vel4):
+        pos4 = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+        vel4 = np.ascontiguousarray(vel4, np.float32).reshape(-1, 4)
+        assert pos4.shape == vel4.shape
+        capi.check(capi.lib.arsim_scene_add_grains(self.h, pos4.shape[0], capi.ptr(pos4), capi.ptr(vel4)),
+                   "scene_add_grains")
+
+    def build(self):
+        """A HostScene copy of the current contents (the builder stays usable)."""
+        view = capi.ArSimSceneHost()
+        capi.check(capi.lib.arsim_scene_view(self.h, C.byref(view)), "scene_view")
+        return HostScene(view)
+
+    def close(self):
+        if self.h:
+            capi.lib.arsim_scene_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def template_pile(data_dir, models=("bunny", "teapot", "banana", "cube"), scale=1.0, per_model=4, grains=0,
+                  method=capi.METHOD_GRID, seed=0, spacing=0.6):
+    """Template bodies of the reference's models (radius 1/64, the templates' own scale) on a jittered
+    lattice with small random velocities, plus free grains above them, in a walled box."""
+    r = 1.0 / 64.0
+    temps = [Template(data_dir, m, scale, r) for m in models]
+    ext = max(float(np.max(np.abs(t.rel[:, :3]))) for t in temps) + r
+    pitch = 2.0 * ext * spacing
+    nb = len(temps) * per_model
+    side = int(math.ceil(nb ** (1.0 / 3.0)))
+    box = side * pitch + 4 * r
+    g = _spec(seed=seed, cubes=1, box=(box, box * 2.0, box), grid_log2=(6, 7, 6), method=method)
+    g.radius = r
+    g.spring, g.damping, g.shear, g.dt = 2000.0, 2.0, 0.5, 2e-5
+    base = generate(g)
+    params, cfg = base.params, base.cfg
+    params.particleRadius = r
+    params.cellSize.x = params.cellSize.y = params.cellSize.z = 2.0 * r
+    b = SceneBuilder(params, cfg)
+    rng = np.random.default_rng(seed)
+    k = 0
+    for ix in range(side):
+        for iy in range(side):
+            for iz in range(side):
+                if k >= nb:
+                    break
+                c = (np.array([ix, iy, iz]) + 0.5) * pitch + 2 * r + rng.uniform(-0.05, 0.05, 3) * pitch
+                b.add_body(temps[k % len(temps)], c, rng.normal(scale=0.2, size=3), rng.normal(scale=2.0, size=3))
+                k += 1
+    if grains:
+        gp = np.zeros((grains, 4), np.float32)
+        gp[:, 0] = rng.uniform(2 * r, box - 2 * r, grains)
+        gp[:, 1] = side * pitch + 4 * r + np.arange(grains) * (2.5 * r) / max(1, int(grains ** 0.5))
+        gp[:, 2] = rng.uniform(2 * r, box - 2 * r, grains)
+        gv = np.zeros((grains, 4), np.float32)
+        gv[:, :3] = rng.normal(scale=0.1, size=(grains, 3))
+        b.add_grains(gp, gv)
+    sc = b.build()
+    b.close()
+    for t in temps:
+        t.close()
+    return sc

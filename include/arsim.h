@@ -378,6 +378,35 @@ int arsim_generate_mix_scene(int n, uint64_t seed, int method, ArSimSceneOwned *
 int arsim_scene_view(const ArSimSceneOwned *s, ArSimSceneHost *view);
 int arsim_scene_free(ArSimSceneOwned *s);
 
+/* ------------------------------------------------------------------ scene I/O
+ * Particle templates in the reference's OBJparticles format, loaded as ParticleSystem::initObject
+ * does (RigidBodies/RigidBodyAddition.cpp:1486-1756): <dataDir>/<model>/<model>_<suffix>.txt, a
+ * particle count line then one "x y z" line per particle; points centred on their FP32 mean and
+ * moved by the per-model offset; inverse inertia from the identity plus the point-mass sum, or the
+ * reference's hard-coded tensors for banana / bunny / teapot / cube scales; radius = the largest
+ * coordinate; mass 1. */
+typedef struct ArSimTemplate ArSimTemplate;
+/* The file suffix initObject picks for a scale ("1_0", "1_5", "2_0" or "2_5"; RigidBodyAddition.cpp:1494-1503). */
+const char *arsim_template_suffix(float scale);
+int arsim_template_load(const char *dataDir, const char *modelName, float scale, float particleRadius,
+                        ArSimTemplate **out);
+/* relPos (float4 per particle) and invInertia (9, column-major) may be NULL. */
+int arsim_template_info(const ArSimTemplate *t, int *numParticles, float *relPos, float *invInertia,
+                        float *radius, float *mass);
+int arsim_template_free(ArSimTemplate *t);
+/* An empty scene to be filled with template bodies and grains (ParticleSystem's add paths). */
+int arsim_scene_create(const ArSimParams *params, const ArSimStepConfig *cfg, ArSimSceneOwned **out);
+/* addRigidBody of a template instance (RigidBodyAddition.cpp:1760-1903): identity orientation,
+ * particles at pos + rel with the body velocity; inserted before the scene's free grains. */
+int arsim_scene_add_body(ArSimSceneOwned *s, const ArSimTemplate *t, const float pos[3], const float vel[3],
+                         const float angularVelocity[3]);
+/* Free grains (rbIndex -1): pos4 / vel4 hold a float4 per grain. */
+int arsim_scene_add_grains(ArSimSceneOwned *s, int n, const float *pos4, const float *vel4);
+/* A host scene that rebuilds scene s of an engine in its current state (for eviction / restore and
+ * fixtures): particle pos / vel, body position, velocity, angular velocity and orientation, plus the
+ * scene's fixed data and current dt.  arsim_engine_create on it continues the run bit for bit. */
+int arsim_engine_snapshot(arsim_engine *e, int scene, ArSimSceneOwned **out);
+
 /* ------------------------------------------------------------------ serving
  * Continuous batching over a Poisson request stream: requests are admitted into the running
  * batch as slots and the HBM budget allow, their state stays on the device, every engine
