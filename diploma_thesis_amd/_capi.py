@@ -169,6 +169,7 @@ SIGNATURES = {
     "arsim_scene_add_body": (I, [VP, VP, VP, VP, VP]),
     "arsim_scene_add_grains": (I, [VP, I, VP, VP]),
     "arsim_engine_snapshot": (I, [VP, I, PVP]),
+    "arsim_engine_render_state": (I, [VP, I, VP, VP]),
 }
 
 

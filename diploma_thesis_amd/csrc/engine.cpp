@@ -94,7 +94,7 @@ SceneView Engine::scene_view() const {
 BodyBufs Engine::body_bufs() const {
     BodyBufs b{};
     b.pos = rbPos; b.vel = rbVel; b.ang = rbAng; b.qinc = qinc; b.qcum = qcum;
-    b.Ib = Ib; b.Iw = Iw; b.model = nullptr; b.mass = mass; b.pstart = pstartD; b.pcount = pcountD;
+    b.Ib = Ib; b.Iw = Iw; b.model = model; b.mass = mass; b.pstart = pstartD; b.pcount = pcountD;
     return b;
 }
 
@@ -226,7 +226,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     A(rb, N); A(scene, N);
     size_t nb = std::max(B, 1);
     A(rbPos, nb); A(rbVel, nb); A(rbAng, nb); A(qinc, nb); A(qcum, nb);
-    A(Ib, nb * 9); A(Iw, nb * 9); A(mass, nb); A(pstartD, nb); A(pcountD, nb);
+    A(Ib, nb * 9); A(Iw, nb * 9); A(mass, nb); A(pstartD, nb); A(pcountD, nb); A(model, nb * 16);
     A(geomD, (size_t)S); A(sceneStartD, (size_t)S); A(sceneCountD, (size_t)S);
     {
         unsigned char *ka = nullptr, *kb = nullptr;
@@ -625,6 +625,21 @@ int arsim_engine_download(arsim_engine *e, int s, float *pos, float *vel, float 
         if (rbQuaternion) ARSIM_HIP(hipMemcpy(rbQuaternion, E.qcum + bo, b * 16, hipMemcpyDeviceToHost));
         if (rbCurrentInertia) ARSIM_HIP(hipMemcpy(rbCurrentInertia, E.Iw + bo * 9, b * 36, hipMemcpyDeviceToHost));
     }
+    return ARSIM_OK;
+}
+
+// Render state of scene s after the last step: per-body model matrices (column-major mat4, the
+// rotation of the cumulative quaternion and the position, rigidBodyKernels.cu:322-332) and the
+// cumulative quaternions.  Either pointer may be NULL.
+int arsim_engine_render_state(arsim_engine *e, int s, float *modelMatrices, float *cumulativeQuaternions) {
+    if (!e || s < 0 || s >= e->e.S) return fail(ARSIM_ERR_INVALID_ARG, "engine_render_state: bad scene");
+    Engine &E = e->e;
+    ARSIM_HIP(hipSetDevice(E.device));
+    ARSIM_HIP(hipStreamSynchronize(E.st));
+    const size_t bo = E.bStart[s], b = E.bCount[s];
+    if (b == 0) return ARSIM_OK;
+    if (modelMatrices) ARSIM_HIP(hipMemcpy(modelMatrices, E.model + bo * 16, b * 64, hipMemcpyDeviceToHost));
+    if (cumulativeQuaternions) ARSIM_HIP(hipMemcpy(cumulativeQuaternions, E.qcum + bo, b * 16, hipMemcpyDeviceToHost));
     return ARSIM_OK;
 }
 

@@ -41,6 +41,7 @@ struct Engine {
     uint16_t *scene = nullptr;
     float4 *rbPos = nullptr, *rbVel = nullptr, *rbAng = nullptr, *qinc = nullptr, *qcum = nullptr;
     float *Ib = nullptr, *Iw = nullptr, *mass = nullptr;
+    float *model = nullptr;       // per-body column-major mat4 (render state), written each step
     int *pstartD = nullptr, *pcountD = nullptr;
     SceneGeom *geomD = nullptr;
     int *sceneStartD = nullptr, *sceneCountD = nullptr;

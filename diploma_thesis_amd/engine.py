@@ -83,6 +83,15 @@ class Engine:
         capi.check(capi.lib.arsim_engine_snapshot(self.h, int(scene), C.byref(owned)), "engine_snapshot")
         return _from_owned(owned)
 
+    def render_state(self, scene=0):
+        """Per-body model matrices (B, 4, 4) in column-major storage order and cumulative quaternions."""
+        b = self.scenes[scene].b
+        m = np.zeros((b, 16), np.float32)
+        q = np.zeros((b, 4), np.float32)
+        capi.check(capi.lib.arsim_engine_render_state(self.h, int(scene), capi.ptr(m), capi.ptr(q)),
+                   "engine_render_state")
+        return m, q
+
     def profile(self):
         ms = (C.c_float * 16)()
         names = (C.c_char_p * 16)()

@@ -406,6 +406,10 @@ int arsim_scene_add_grains(ArSimSceneOwned *s, int n, const float *pos4, const f
  * fixtures): particle pos / vel, body position, velocity, angular velocity and orientation, plus the
  * scene's fixed data and current dt.  arsim_engine_create on it continues the run bit for bit. */
 int arsim_engine_snapshot(arsim_engine *e, int scene, ArSimSceneOwned **out);
+/* Headless render state after the last step (the per-body modelMatrixGPU / cumulativeQuaternionGPU
+ * outputs of GPUintegratorKernel, Kernels/rigidBodyKernels.cu:322-332): 16 floats (column-major mat4)
+ * and 4 floats (x,y,z,w) per body of the scene.  Either pointer may be NULL. */
+int arsim_engine_render_state(arsim_engine *e, int scene, float *modelMatrices, float *cumulativeQuaternions);
 
 /* ------------------------------------------------------------------ serving
  * Continuous batching over a Poisson request stream: requests are admitted into the running

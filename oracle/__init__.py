@@ -205,7 +205,7 @@ def wall_force_f64(p, v, params, cfg):
 class _StateF(C.Structure):
     _fields_ = [("N", C.c_int), ("B", C.c_int)] + [(k, VP) for k in (
         "pos", "vel", "relPos", "bodyFrame", "pForce", "pTorque", "rb", "rbPos", "rbVel", "rbAng", "qinc", "qcum",
-        "Ib", "Iw", "mass", "pstart", "pcount")]
+        "Ib", "Iw", "mass", "pstart", "pcount", "model")]
 
 
 class State:
@@ -232,6 +232,7 @@ class State:
         self.mass = scene.rb_mass.astype(t).copy()
         self.pstart = scene.pstart.copy()
         self.pcount = scene.ppo.astype(np.int32).copy()
+        self.model = np.zeros((self.B, 16), t)     # written by each body integration
         self.params = scene.params
         self.cfg = scene.cfg
 
@@ -242,7 +243,7 @@ class State:
                      ("pForce", self.force), ("pTorque", self.torque), ("rb", self.rb), ("rbPos", self.rb_pos),
                      ("rbVel", self.rb_vel), ("rbAng", self.rb_ang), ("qinc", self.qinc), ("qcum", self.qcum),
                      ("Ib", self.Ib), ("Iw", self.Iw), ("mass", self.mass), ("pstart", self.pstart),
-                     ("pcount", self.pcount)):
+                     ("pcount", self.pcount), ("model", self.model)):
             setattr(s, k, _p(a))
         return s
 

@@ -467,6 +467,7 @@ template <class T> struct State {
     T *Iw;                                                 // 9 per body
     const T *mass;
     const int *pstart, *pcount;
+    T *model;                                              // optional: 16 per body (column-major mat4)
 };
 
 template <class T> inline void integrate_bodies(State<T> &s, const Phys<T> &f) {
@@ -495,6 +496,16 @@ template <class T> inline void integrate_bodies(State<T> &s, const Phys<T> &f) {
         mm(R, s.Ib + 9 * (long)b, tmp);
         mm(tmp, RT, Iw);
         for (int k = 0; k < 9; ++k) s.Iw[9 * (long)b + k] = Iw[k];
+        if (s.model) {
+            // modelMatrix of rigidBodyKernels.cu:324-331: rotation of the cumulative quaternion, then the
+            // updated position in the last column
+            T *m = s.model + 16 * (long)b;
+            for (int c = 0; c < 3; ++c) {
+                for (int r = 0; r < 3; ++r) m[4 * c + r] = R[3 * c + r];
+                m[4 * c + 3] = T(0);
+            }
+            m[12] = x.x; m[13] = x.y; m[14] = x.z; m[15] = T(1);
+        }
         V3<T> gs = ref ? f.g : mk<T>(f.g.x * f.dt, f.g.y * f.dt, f.g.z * f.dt);
         v = add(v, gs);
         w = mk<T>(w.x * f.Wdamping, w.y * f.Wdamping, w.z * f.Wdamping);
@@ -762,6 +773,7 @@ typedef struct OrcStateF {
     float *Iw;
     const float *mass;
     const int *pstart, *pcount;
+    float *model;
 } OrcStateF;
 typedef struct OrcStateD {
     int N, B;
@@ -772,6 +784,7 @@ typedef struct OrcStateD {
     double *Iw;
     const double *mass;
     const int *pstart, *pcount;
+    double *model;
 } OrcStateD;
 
 }  // extern "C"
@@ -783,6 +796,7 @@ template <class T, class S> static State<T> to_state(S *o) {
     s.pForce = o->pForce; s.pTorque = o->pTorque; s.rb = o->rb;
     s.rbPos = o->rbPos; s.rbVel = o->rbVel; s.rbAng = o->rbAng; s.qinc = o->qinc; s.qcum = o->qcum;
     s.Ib = o->Ib; s.Iw = o->Iw; s.mass = o->mass; s.pstart = o->pstart; s.pcount = o->pcount;
+    s.model = o->model;
     return s;
 }
 

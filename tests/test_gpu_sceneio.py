@@ -64,3 +64,21 @@ def test_snapshot_of_batched_scene_continues_alone():
         resumed = b.download()
     for k in ("pos", "vel", "rb_pos", "rb_vel", "rb_ang", "rb_quat"):
         assert np.array_equal(full[k], resumed[k]), k
+
+
+@pytest.mark.parametrize("method", [capi.METHOD_GRID, capi.METHOD_LBVH30])
+def test_render_state_matches_oracle(method):
+    """Model matrices of every body after a run equal the oracle's bit for bit (rotation of the
+    cumulative quaternion, position in the last column), and the rotation is orthonormal."""
+    print(f"render state: method={method}", flush=True)
+    sc = scenes.template_pile(DATA, grains=100, method=method)
+    with Engine(sc) as e:
+        e.step(25)
+        m, q = e.render_state()
+    st = oracle.State(sc, np.float32)
+    st.step(25)
+    assert np.array_equal(m, st.model) and np.array_equal(q, st.qcum)
+    M = m.reshape(-1, 4, 4).transpose(0, 2, 1).astype(np.float64)   # row-major view of the mat4
+    R = M[:, :3, :3]
+    assert np.allclose(R @ R.transpose(0, 2, 1), np.eye(3), atol=1e-5)
+    assert np.array_equal(m.reshape(-1, 4, 4)[:, 3, :3], st.rb_pos[:, :3])
