@@ -145,6 +145,28 @@ __device__ __forceinline__ bool pair_force(f3 pa, f3 pb, f3 va, f3 vb, const Dev
     out = F;
     return true;
 }
+// Force on a particle from a static scene point (FindAndHandleAugmentedRealityCollisionsUniformGridCell,
+// RigidBodies/UniformGridDEM.cu:243-301): the pair formula with the normal taken as minus the scene
+// point's normal and a partner at rest (relVel = 0 - va).
+__device__ __forceinline__ bool static_force(f3 pa, f3 ps, f3 ns, f3 va, const DevPhys &ph, f3 &out) {
+    f3 rel = sub3(ps, pa);
+    const float d2 = dot3(rel, rel);
+    if (d2 > ph.cd2Reject) return false;
+    float dist = sqrtf(d2);
+    float cd = 2.0f * ph.r;
+    if (!(dist < cd)) return false;
+    f3 n = mk3(-ns.x, -ns.y, -ns.z);
+    f3 rv = sub3(mk3(0.f, 0.f, 0.f), va);
+    float vn = dot3(rv, n);
+    f3 tv = sub3(rv, scl3(vn, n));
+    float sp = -ph.k * (cd - dist);
+    f3 F = scl3(sp, n);
+    F = add3(F, scl3(ph.c, rv));
+    F = add3(F, scl3(ph.s, tv));
+    F = add3(F, scl3(ph.a, rel));
+    out = F;
+    return true;
+}
 // Contact predicate only (same arithmetic as the distance test above).
 __device__ __forceinline__ bool in_contact(f3 pa, f3 pb, const DevPhys &ph) {
     f3 rel = sub3(pb, pa);

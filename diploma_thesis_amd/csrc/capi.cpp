@@ -3,6 +3,7 @@
 // Each stage function validates its arguments, converts the reference's SimParams into the
 // kernels' parameter structs and enqueues the kernels on the caller's stream.  None of them
 // allocates device memory or waits for the device.
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -408,6 +409,42 @@ int arsim_contact_pairs_lbvh(const int *rbIndices, const int *sortedIndices, con
     o.pairs = pairs; o.counts = counts; o.maxPer = maxPer;
     launch_dem_lbvh(sortedIndices, t, n, make_phys(*params, cfg), sv, false, o, 128, S(st), overflowFlag);
     return check_launch("contact_pairs_lbvh");
+}
+
+/* ---------------------------------------------------------------- static collider set */
+int arsim_range_image_load(const uint16_t *depth, int width, int height, const float *cameraTransformation,
+                           float particleRadius, float *staticPos, float *staticNorm, float *staticRadii,
+                           float *vaoData, arsim_stream_t st) {
+    if (width <= 2 || height <= 2 || !depth || !cameraTransformation || !staticPos || !staticNorm)
+        return fail(ARSIM_ERR_INVALID_ARG, "range_image_load: bad arguments");
+    RangeCamera cam{};
+    std::memcpy(cam.m, cameraTransformation, sizeof(cam.m));
+    const double g0 = 0.478957, g1 = -8.010560, g2 = 4.166928;   // customRangeImageLoader.cu:76-78
+    const double gl = std::sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+    cam.g[0] = (float)(g0 / gl); cam.g[1] = (float)(g1 / gl); cam.g[2] = (float)(g2 / gl);
+    launch_range_image(depth, width, height, cam, particleRadius, (float4 *)staticPos, (float4 *)staticNorm,
+                       staticRadii, vaoData, S(st));
+    return check_launch("range_image_load");
+}
+
+int arsim_dem_static_grid(float *pForce, float *pTorque, const float *sortedPos, const float *sortedVel,
+                          const float *relativePos, const float *staticSortedPos, const float *staticNorm,
+                          const unsigned int *sortedIndex, const unsigned int *staticSortedIndex,
+                          const unsigned int *staticCellStart, const unsigned int *staticCellEnd, int numParticles,
+                          const ArSimParams *params, const ArSimStepConfig *cfg, arsim_stream_t st) {
+    if (numParticles < 0 || !params || !cfg) return fail(ARSIM_ERR_INVALID_ARG, "dem_static_grid: bad arguments");
+    if (numParticles == 0) return ARSIM_OK;
+    if (!pForce || !pTorque || !sortedPos || !sortedVel || !relativePos || !staticSortedPos || !staticNorm ||
+        !sortedIndex || !staticSortedIndex || !staticCellStart || !staticCellEnd)
+        return fail(ARSIM_ERR_INVALID_ARG, "dem_static_grid: null buffer");
+    SceneGeom g{};
+    int rc = make_geom(*params, *cfg, 0, numParticles, g);
+    if (rc) return rc;
+    launch_dem_static((const float4 *)sortedPos, (const float4 *)sortedVel, sortedIndex, numParticles,
+                      (const float4 *)staticSortedPos, (const float4 *)staticNorm, staticSortedIndex, staticCellStart,
+                      staticCellEnd, make_phys(*params, *cfg), g, (float4 *)pForce, (float4 *)pTorque,
+                      (const float4 *)relativePos, true, S(st));
+    return check_launch("dem_static_grid");
 }
 
 }  // extern "C"

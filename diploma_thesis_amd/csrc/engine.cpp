@@ -72,9 +72,72 @@ template <class T> int Engine::alloc(T **p, size_t count) {
 }
 
 Engine::~Engine() {
+    free_static();
     if (gexec) hipGraphExecDestroy(gexec);
     for (void *p : allocs) hipFree(p);
     if (st && ownStream) hipStreamDestroy(st);
+}
+
+void Engine::free_static() {
+    if (stPos) hipFree(stPos);
+    if (stNorm) hipFree(stNorm);
+    if (stTable) hipFree(stTable);
+    stPos = stNorm = nullptr;
+    stTable = nullptr;
+    numStatic = 0;
+}
+
+// Indexes a static point set once: cell keys, a stable sort by key, sorted copies of the points
+// and normals, and the exact cell table over the scene's grid.
+int Engine::set_static(const float *pos4, const float *norm4, int n) {
+    if (S != 1) return fail(ARSIM_ERR_UNSUPPORTED, "engine_set_static: one-scene engines only");
+    if (n < 0 || (n > 0 && (!pos4 || !norm4))) return fail(ARSIM_ERR_INVALID_ARG, "engine_set_static: bad arguments");
+    ARSIM_HIP(hipSetDevice(device));
+    ARSIM_HIP(hipStreamSynchronize(st));
+    invalidate_graph();
+    free_static();
+    if (n == 0) return ARSIM_OK;
+    const SceneGeom &g = geoms[0];
+    const unsigned cells = (g.gridMask[0] + 1u) * (g.gridMask[1] + 1u) * (g.gridMask[2] + 1u);
+    const size_t wsBytes = sort_workspace_bytes(n, 4);
+    void *tmp = nullptr;
+    const size_t v4 = (size_t)n * 16, v1 = (size_t)n * 4;
+    const size_t tmpBytes = 2 * v4 + 4 * v1 + wsBytes + 256;
+    ARSIM_HIP(hipMalloc(&tmp, tmpBytes));
+    char *c = (char *)tmp;
+    float4 *dPos = (float4 *)c; c += v4;
+    float4 *dNorm = (float4 *)c; c += v4;
+    unsigned *kA = (unsigned *)c; c += v1;
+    unsigned *kB = (unsigned *)c; c += v1;
+    unsigned *vA = (unsigned *)c; c += v1;
+    unsigned *vB = (unsigned *)c; c += v1;
+    void *ws = (void *)(((uintptr_t)c + 255) & ~(uintptr_t)255);
+    int rc = ARSIM_OK;
+    hipError_t e = hipMalloc(&stPos, v4);
+    if (e == hipSuccess) e = hipMalloc(&stNorm, v4);
+    if (e == hipSuccess) e = hipMalloc(&stTable, ((size_t)cells + 1) * 4);
+    if (e == hipSuccess) e = hipMemcpy(dPos, pos4, v4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dNorm, norm4, v4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_static_hash(dPos, n, g, kA, st);
+        int bits = 1;
+        while ((1ull << bits) < (unsigned long long)cells) ++bits;
+        SortTuning stt{};
+        if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, n, 0, bits, true, ws, wsBytes, stt, st))
+            rc = fail(ARSIM_ERR_INVALID_ARG, "engine_set_static: sort rejected");
+        else
+            launch_static_reorder(kA, vA, dPos, dNorm, n, cells, stPos, stNorm, stTable, st);
+        e = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = hipGetLastError();
+    }
+    hipFree(tmp);
+    if (e != hipSuccess) rc = fail(ARSIM_ERR_HIP, std::string("engine_set_static: ") + hipGetErrorString(e));
+    if (rc) {
+        free_static();
+        return rc;
+    }
+    numStatic = n;
+    return ARSIM_OK;
 }
 
 void Engine::invalidate_graph() {
@@ -378,6 +441,9 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         DemOut o{};
         o.pForce = pForce; o.vel = vel; o.relPos = relPos;
         launch_dem_grid_seg(sortedPos, sortedVel, kA, vA, segStart, segBits, N, ph, sv, S > 1, o, tu.demBlock, st);
+        if (numStatic)
+            launch_dem_static(sortedPos, sortedVel, vA, N, stPos, stNorm, nullptr, stTable, nullptr, ph, geoms[0],
+                              pForce, nullptr, relPos, false, st);
         rec();
         launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
         rec();
@@ -425,6 +491,9 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         DemOut o{};
         o.pForce = pForce; o.vel = vel; o.relPos = relPos;
         launch_dem_lbvh((const int *)vA, tree, N, ph, sv, S > 1, o, tu.demBlock, st, overflowD);
+        if (numStatic)
+            launch_dem_static(tree.cms, tree.sortedVel, vA, N, stPos, stNorm, nullptr, stTable, nullptr, ph,
+                              geoms[0], pForce, nullptr, relPos, false, st);
         rec();
         launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
         rec();
@@ -626,6 +695,11 @@ int arsim_engine_download(arsim_engine *e, int s, float *pos, float *vel, float 
         if (rbCurrentInertia) ARSIM_HIP(hipMemcpy(rbCurrentInertia, E.Iw + bo * 9, b * 36, hipMemcpyDeviceToHost));
     }
     return ARSIM_OK;
+}
+
+int arsim_engine_set_static(arsim_engine *e, const float *pos4, const float *norm4, int n) {
+    if (!e) return fail(ARSIM_ERR_INVALID_ARG, "null engine");
+    return e->e.set_static(pos4, norm4, n);
 }
 
 // Render state of scene s after the last step: per-body model matrices (column-major mat4, the

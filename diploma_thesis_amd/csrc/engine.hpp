@@ -56,6 +56,10 @@ struct Engine {
     Lbvh tree{};
     int *overflowD = nullptr, *flagD = nullptr;
     float4 *aabbPart = nullptr, *aabbBox = nullptr;
+    // static collider set (one-scene engines): points and normals sorted by cell, exact cell table
+    int numStatic = 0;
+    float4 *stPos = nullptr, *stNorm = nullptr;
+    unsigned *stTable = nullptr;
 
     // step state
     bool first = true;
@@ -71,6 +75,8 @@ struct Engine {
     int step(int n);
     int enqueue_broadphase(int *pairsD, int *countsD, int maxPer);
     void invalidate_graph();
+    int set_static(const float *pos4, const float *norm4, int n);
+    void free_static();
     int reset_grid_tables();
     static const char *const *stage_names(int method, int *count);
     SceneView scene_view() const;

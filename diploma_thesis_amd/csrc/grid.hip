@@ -1,4 +1,5 @@
 // grid.hip -- uniform-grid broadphase: cell bounds + reorder, and the 27-cell DEM kernel.
+#include "csr_fill.hpp"
 #include "kernels.hpp"
 
 namespace arsim {
@@ -97,31 +98,6 @@ __global__ void k_dem_grid(const float4 *__restrict__ sPos, const float4 *__rest
     } else {
         const float fs = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
         o.vel[orig] = make_float4(va.x + F.x * fs, va.y + F.y * fs, va.z + F.z * fs, 0.0f);
-    }
-}
-
-// Writes v to p[a..b) with 16-byte stores for the aligned middle of long runs.
-__device__ __forceinline__ void fill_range(unsigned *p, long long a, long long b, unsigned v) {
-    long long k = a;
-    while (k < b && (k & 3)) p[k++] = v;
-    const uint4 v4 = make_uint4(v, v, v, v);
-    for (; k + 4 <= b; k += 4) *reinterpret_cast<uint4 *>(p + k) = v4;
-    for (; k < b; ++k) p[k] = v;
-}
-
-// Fills p[a, b) with v for every lane that has a range: short ranges by the lane itself, long
-// ones (empty-cell runs, the tail of the table) by the whole wave with coalesced stores.  Every
-// lane of the wave must call it.
-__device__ __forceinline__ void wave_fill(unsigned *p, bool has, unsigned a, unsigned b, unsigned v) {
-    const bool lng = has && b - a > 32u;
-    if (has && !lng) fill_range(p, a, b, v);
-    unsigned long long m = __ballot(lng);
-    const unsigned lane = threadIdx.x & 63u;
-    while (m) {
-        const int l = __builtin_ctzll(m);
-        m &= m - 1;
-        const unsigned la = __shfl(a, l), lb = __shfl(b, l), lv = __shfl(v, l);
-        for (unsigned k = la + lane; k < lb; k += 64u) p[k] = lv;
     }
 }
 

@@ -135,4 +135,23 @@ void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *ve
 void launch_dem_lbvh(const int *sortedIdx, const Lbvh &t, int n, const DevPhys &ph, const SceneView &sv,
                      bool sceneInVelW, const DemOut &o, int block, hipStream_t st, int *overflow);
 
+// ---------------------------------------------------------------- static collider set (static.hip)
+struct RangeCamera {
+    float m[16];                  // camera transformation, column-major
+    float g[3];                   // normalised accelerometer direction
+};
+// RGBDloader + computeNormals: pos / norm hold a float4 per pixel; radii and vao may be NULL.
+void launch_range_image(const uint16_t *img, int W, int H, const RangeCamera &cam, float radius, float4 *pos,
+                        float4 *norm, float *radii, float *vao, hipStream_t st);
+void launch_static_hash(const float4 *pos, int m, const SceneGeom &g, unsigned *keys, hipStream_t st);
+void launch_static_reorder(const unsigned *keys, const unsigned *idx, const float4 *pos, const float4 *norm, int m,
+                           unsigned numCells, float4 *sPos, float4 *sNorm, unsigned *table, hipStream_t st);
+// refLayout: cs / ce are the reference's static cellStart / cellEnd, normals are read through stIdx and
+// every particle gets pForce += F, pTorque += relPos x F; otherwise cs is the exact static cell table
+// (ce unused), normals are sorted and only body particles get pForce += F.
+void launch_dem_static(const float4 *sPos, const float4 *sVel, const unsigned *sIdx, int n, const float4 *stPos,
+                       const float4 *stNorm, const unsigned *stIdx, const unsigned *cs, const unsigned *ce,
+                       const DevPhys &ph, const SceneGeom &g, float4 *pForce, float4 *pTorque,
+                       const float4 *relPos, bool refLayout, hipStream_t st);
+
 }  // namespace arsim

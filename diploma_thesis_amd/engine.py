@@ -83,6 +83,17 @@ class Engine:
         capi.check(capi.lib.arsim_engine_snapshot(self.h, int(scene), C.byref(owned)), "engine_snapshot")
         return _from_owned(owned)
 
+    def set_static(self, pos4, norm4):
+        """Static collider points and normals (float4 each) for a one-scene engine; None clears them."""
+        if pos4 is None:
+            capi.check(capi.lib.arsim_engine_set_static(self.h, None, None, 0), "engine_set_static")
+            return
+        pos4 = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+        norm4 = np.ascontiguousarray(norm4, np.float32).reshape(-1, 4)
+        assert pos4.shape == norm4.shape
+        capi.check(capi.lib.arsim_engine_set_static(self.h, capi.ptr(pos4), capi.ptr(norm4), pos4.shape[0]),
+                   "engine_set_static")
+
     def render_state(self, scene=0):
         """Per-body model matrices (B, 4, 4) in column-major storage order and cumulative quaternions."""
         b = self.scenes[scene].b

@@ -281,6 +281,28 @@ int arsim_contact_pairs_lbvh(const int *rbIndices, const int *sortedIndices,
                              const ArSimParams *params, int *overflowFlag,
                              int *pairs, int *counts, int maxPerParticle, arsim_stream_t stream);
 
+/* Static (augmented-reality) collider set.
+ * loadRangeImage (Kernels/customRangeImageLoader.cu:165-216 = RGBDloader :14-109 + computeNormals
+ * :118-162): a W x H depth image (uint16, device) -> one scene point per pixel (float4, w = 0) with its
+ * normal (float4), through the reference's hard-coded intrinsics and accelerometer rotation and the
+ * host-side camera transformation (16 floats, column-major, host memory).  Pixels past (W-1)(H-1)
+ * and the border's normals are not written, as in the reference.  staticRadii (float per pixel) and
+ * vaoData (5 floats per pixel: position, x / W, y / H) may be NULL. */
+int arsim_range_image_load(const uint16_t *depth, int width, int height, const float *cameraTransformation,
+                           float particleRadius, float *staticPos, float *staticNorm, float *staticRadii,
+                           float *vaoData, arsim_stream_t stream);
+/* FindAndHandleAugmentedRealityCollisionsUniformGridWrapper (RigidBodies/UniformGridDEM.cu:382-417):
+ * contact forces of the sorted dynamic particles against the static points (sorted by cell, with
+ * their cellStart / cellEnd from arsim_calc_hash + sort + arsim_reorder_find_cell_start on the static
+ * set, and their unsorted normals read through staticSortedIndex), ADDED to pForce, with
+ * relativePos x F added to pTorque, for every particle. */
+int arsim_dem_static_grid(float *pForce, float *pTorque, const float *sortedPos, const float *sortedVel,
+                          const float *relativePos, const float *staticSortedPos, const float *staticNorm,
+                          const unsigned int *sortedIndex, const unsigned int *staticSortedIndex,
+                          const unsigned int *staticCellStart, const unsigned int *staticCellEnd,
+                          int numParticles, const ArSimParams *params, const ArSimStepConfig *cfg,
+                          arsim_stream_t stream);
+
 /* Body-index rebasing copy: dst[i] = src[i] + delta where src[i] >= 0, else src[i].  The reference
  * offsets a new body's rbIndices by the current body count when it appends it
  * (RigidBodies/RigidBodyAddition.cpp:111-139); the engine does the same when it moves scenes
@@ -410,6 +432,12 @@ int arsim_engine_snapshot(arsim_engine *e, int scene, ArSimSceneOwned **out);
  * outputs of GPUintegratorKernel, Kernels/rigidBodyKernels.cu:322-332): 16 floats (column-major mat4)
  * and 4 floats (x,y,z,w) per body of the scene.  Either pointer may be NULL. */
 int arsim_engine_render_state(arsim_engine *e, int scene, float *modelMatrices, float *cumulativeQuaternions);
+/* Static collider set of a one-scene engine (host float4 arrays of points and normals, e.g. from
+ * arsim_range_image_load): indexed once on the device, then every step adds the static contact
+ * forces of the body particles to their particle forces after the DEM (the reference's
+ * Find_And_Handle_Augmented_Reality_Collisions_Uniform_Grid_DEM, UniformGridUpdate.cpp:880-935).
+ * n = 0 removes the set.  Not available for batched engines. */
+int arsim_engine_set_static(arsim_engine *e, const float *pos4, const float *norm4, int n);
 
 /* ------------------------------------------------------------------ serving
  * Continuous batching over a Poisson request stream: requests are admitted into the running

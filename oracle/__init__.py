@@ -65,6 +65,9 @@ for name, res, args in [
     ("orc_pair_force_f32", C.c_int, [VP, VP, VP, VP, VP, VP, VP]),
     ("orc_wall_force_f64", None, [VP, VP, VP, VP, VP]),
     ("orc_sincos", None, [VP, C.c_int, VP, VP]),
+    ("orc_range_image", None, [VP, C.c_int, C.c_int, VP, C.c_float, VP, VP, VP, VP]),
+    ("orc_dem_static_f32", C.c_int, [VP, VP, VP, VP, VP, VP, VP, C.c_int]),
+    ("orc_dem_static_f64", C.c_int, [VP, VP, VP, VP, VP, VP, VP, C.c_int]),
 ]:
     f = getattr(lib, name)
     f.restype = res
@@ -72,6 +75,20 @@ for name, res, args in [
 
 
 # ----------------------------------------------------------------------------- stage wrappers
+def range_image(depth, cam, radius):
+    """Static scene points / normals (float4 per pixel) from a uint16 depth image (H, W) and a
+    column-major camera matrix; pixels the reference leaves unwritten stay NaN here."""
+    depth = np.ascontiguousarray(depth, np.uint16)
+    H, W = depth.shape
+    pos = np.full((H * W, 4), np.nan, np.float32)
+    norm = np.full((H * W, 4), np.nan, np.float32)
+    radii = np.full(H * W, np.nan, np.float32)
+    vao = np.full((H * W, 5), np.nan, np.float32)
+    lib.orc_range_image(_p(depth), W, H, _p(np.ascontiguousarray(cam, np.float32).reshape(16)), float(radius),
+                        _p(pos), _p(norm), _p(radii), _p(vao))
+    return pos, norm, radii, vao
+
+
 def calc_hash(pos, params):
     n = pos.shape[0]
     h = np.zeros(n, np.uint32)
@@ -205,7 +222,7 @@ def wall_force_f64(p, v, params, cfg):
 class _StateF(C.Structure):
     _fields_ = [("N", C.c_int), ("B", C.c_int)] + [(k, VP) for k in (
         "pos", "vel", "relPos", "bodyFrame", "pForce", "pTorque", "rb", "rbPos", "rbVel", "rbAng", "qinc", "qcum",
-        "Ib", "Iw", "mass", "pstart", "pcount", "model")]
+        "Ib", "Iw", "mass", "pstart", "pcount", "model", "staticPos", "staticNorm")] + [("numStatic", C.c_int)]
 
 
 class State:
@@ -233,6 +250,7 @@ class State:
         self.pstart = scene.pstart.copy()
         self.pcount = scene.ppo.astype(np.int32).copy()
         self.model = np.zeros((self.B, 16), t)     # written by each body integration
+        self.static_pos = self.static_norm = None  # optional static scene points (float32, float4 each)
         self.params = scene.params
         self.cfg = scene.cfg
 
@@ -243,9 +261,27 @@ class State:
                      ("pForce", self.force), ("pTorque", self.torque), ("rb", self.rb), ("rbPos", self.rb_pos),
                      ("rbVel", self.rb_vel), ("rbAng", self.rb_ang), ("qinc", self.qinc), ("qcum", self.qcum),
                      ("Ib", self.Ib), ("Iw", self.Iw), ("mass", self.mass), ("pstart", self.pstart),
-                     ("pcount", self.pcount), ("model", self.model)):
+                     ("pcount", self.pcount), ("model", self.model), ("staticPos", self.static_pos),
+                     ("staticNorm", self.static_norm)):
             setattr(s, k, _p(a))
+        s.numStatic = 0 if self.static_pos is None else int(self.static_pos.shape[0])
         return s
+
+    def set_static(self, pos4, norm4):
+        """Static scene points and normals; each step adds their contact forces to the body particles
+        after the DEM (rebuilding their grid index every step, as the reference does)."""
+        if pos4 is None or len(pos4) == 0:
+            self.static_pos = self.static_norm = None
+            return
+        self.static_pos = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+        self.static_norm = np.ascontiguousarray(norm4, np.float32).reshape(-1, 4)
+        assert self.static_pos.shape == self.static_norm.shape
+
+    def dem_static(self, cell_start, cell_end, sorted_static_pos, sorted_static_norm, ref=False):
+        self._call("orc_dem_static", _p(np.ascontiguousarray(cell_start, np.uint32)),
+                   _p(np.ascontiguousarray(cell_end, np.uint32)),
+                   _p(np.ascontiguousarray(sorted_static_pos, np.float32)),
+                   _p(np.ascontiguousarray(sorted_static_norm, np.float32)), 1 if ref else 0)
 
     def _call(self, base, *extra):
         s = self._struct()

@@ -158,6 +158,28 @@ inline bool pair_force(V3<T> pa, V3<T> pb, V3<T> va, V3<T> vb, const Phys<T> &f,
     return true;
 }
 
+// DEM force against a static scene point (FindAndHandleAugmentedRealityCollisionsUniformGridCell,
+// RigidBodies/UniformGridDEM.cu:243-301): contact when |ps - pa| < 2r; the normal is minus the
+// scene point's normal, the partner velocity is zero (relVel = 0 - va), the rest is the pair formula.
+template <class T>
+inline bool static_force(V3<T> pa, V3<T> ps, V3<T> ns, V3<T> va, const Phys<T> &f, V3<T> *out) {
+    V3<T> rel = sub(ps, pa);
+    T dist = vsqrt(dot(rel, rel));
+    T cd = T(2) * f.r;
+    if (!(dist < cd)) return false;
+    V3<T> n = mk<T>(-ns.x, -ns.y, -ns.z);
+    V3<T> rv = sub(mk<T>(0, 0, 0), va);
+    T vn = dot(rv, n);
+    V3<T> tv = sub(rv, scl(vn, n));
+    T sp = -f.k * (cd - dist);
+    V3<T> F = scl(sp, n);
+    F = add(F, scl(f.c, rv));
+    F = add(F, scl(f.s, tv));
+    F = add(F, scl(f.a, rel));
+    *out = F;
+    return true;
+}
+
 // Static-plane wall term: the static-contact formula (UniformGridDEM.cu:281-300,
 // particleKernels.cu:408-445) with the wall as the static partner: relVel = -v, contact
 // distance r, overlap r - d, normal = -(inward wall normal).  Walls in bit order.
@@ -468,6 +490,8 @@ template <class T> struct State {
     const T *mass;
     const int *pstart, *pcount;
     T *model;                                              // optional: 16 per body (column-major mat4)
+    const float *staticPos, *staticNorm;                   // optional static scene points (float4 each)
+    int numStatic;
 };
 
 template <class T> inline void integrate_bodies(State<T> &s, const Phys<T> &f) {
@@ -602,6 +626,47 @@ void dem_grid(State<T> &s, const Phys<T> &f, const ArSimParams *p, const uint32_
     }
 }
 
+// Static scene DEM (FindAndHandleAugmentedRealityCollisionsUniformGridKernel, UniformGridDEM.cu:306-378)
+// for the body particles: the 27 cells of the static points' grid in z, y, x order, each cell's
+// contacts summed into a partial that is added to the total for every cell (empty ones too, as the
+// reference does), then pForce += total and the particle torque becomes relativePos x pForce.
+// Static points come sorted by cell with their normals (ssPos / ssNorm) and the static cellStart /
+// cellEnd of the same grid.
+// ref = true: the stage as the reference wrapper runs it: every particle, pForce += F and
+// pTorque += relativePos x F.
+template <class T>
+void dem_static(State<T> &s, const Phys<T> &f, const ArSimParams *p, const uint32_t *cellStart,
+                const uint32_t *cellEnd, const float *ssPos, const float *ssNorm, bool ref = false) {
+    for (int i = 0; i < s.N; ++i) {
+        if (!ref && s.rb[i] < 0) continue;
+        V3<T> pa = ld3(s.pos, i), va = ld3(s.vel, i);
+        int gx = (int)std::floor(((float)pa.x - p->worldOrigin.x) / p->cellSize.x);
+        int gy = (int)std::floor(((float)pa.y - p->worldOrigin.y) / p->cellSize.y);
+        int gz = (int)std::floor(((float)pa.z - p->worldOrigin.z) / p->cellSize.z);
+        V3<T> F = mk<T>(0, 0, 0);
+        for (int z = -1; z <= 1; ++z)
+            for (int y = -1; y <= 1; ++y)
+                for (int x = -1; x <= 1; ++x) {
+                    uint32_t h = grid_hash_cell(gx + x, gy + y, gz + z, p);
+                    V3<T> cs = mk<T>(0, 0, 0);
+                    uint32_t st = cellStart[h];
+                    if (st != 0xffffffffu) {
+                        for (uint32_t j = st; j < cellEnd[h]; ++j) {
+                            V3<T> ps = mk<T>((T)ssPos[4 * j], (T)ssPos[4 * j + 1], (T)ssPos[4 * j + 2]);
+                            V3<T> ns = mk<T>((T)ssNorm[4 * j], (T)ssNorm[4 * j + 1], (T)ssNorm[4 * j + 2]);
+                            V3<T> fj;
+                            if (static_force(pa, ps, ns, va, f, &fj)) cs = add(cs, fj);
+                        }
+                    }
+                    F = add(F, cs);
+                }
+        V3<T> tot = add(ld3(s.pForce, i), F);
+        st4(s.pForce, i, tot, T(0));
+        if (ref) st4(s.pTorque, i, add(ld3(s.pTorque, i), cross(ld3(s.relPos, i), F)), T(0));
+        else st4(s.pTorque, i, cross(ld3(s.relPos, i), tot), T(0));
+    }
+}
+
 // LBVH traversal DEM (collideBVHSoARigidBody, RB_BVHKernels.cu:100-244): stack traversal from the
 // root, left child tested before right, the right child deferred when both are entered.
 template <class T>
@@ -684,6 +749,8 @@ template <class T> struct Scratch {
     std::vector<unsigned char> isLeaf;
     std::vector<float> bounds, cmsF, posF;
     std::vector<T> cms;
+    std::vector<uint32_t> stH, stI, stSH, stSI, stCs, stCe;
+    std::vector<float> stPos, stNorm;
 };
 
 template <class T>
@@ -755,6 +822,22 @@ int step_impl(State<T> &s, const ArSimParams *p, const ArSimStepConfig *cfg, int
             dem_lbvh(s, f, N, w.sIdxI.data(), w.left.data(), w.right.data(), w.bounds.data(), w.cms.data(),
                      w.sVel.data(), w.sRb.data());
         }
+        if (s.numStatic > 0) {
+            // the static points' grid index (rebuilt every step, as UniformGridUpdate.cpp:880-912 does)
+            const int M = s.numStatic;
+            w.stH.resize(M); w.stI.resize(M); w.stSH.resize(M); w.stSI.resize(M);
+            orc_calc_hash(s.staticPos, M, p, w.stH.data(), w.stI.data());
+            orc_sort_pairs_u32(w.stH.data(), w.stI.data(), M, 0, 32, w.stSH.data(), w.stSI.data());
+            w.stCs.resize(p->numCells); w.stCe.resize(p->numCells);
+            orc_cell_bounds(w.stSH.data(), M, p->numCells, w.stCs.data(), w.stCe.data());
+            w.stPos.resize(4 * (size_t)M); w.stNorm.resize(4 * (size_t)M);
+            for (int j = 0; j < M; ++j)
+                for (int c = 0; c < 4; ++c) {
+                    w.stPos[4 * j + c] = s.staticPos[4 * (size_t)w.stSI[j] + c];
+                    w.stNorm[4 * j + c] = s.staticNorm[4 * (size_t)w.stSI[j] + c];
+                }
+            dem_static(s, f, p, w.stCs.data(), w.stCe.data(), w.stPos.data(), w.stNorm.data());
+        }
         reduce_apply(s, f);
     }
     return 0;
@@ -774,6 +857,8 @@ typedef struct OrcStateF {
     const float *mass;
     const int *pstart, *pcount;
     float *model;
+    const float *staticPos, *staticNorm;
+    int numStatic;
 } OrcStateF;
 typedef struct OrcStateD {
     int N, B;
@@ -785,6 +870,8 @@ typedef struct OrcStateD {
     const double *mass;
     const int *pstart, *pcount;
     double *model;
+    const float *staticPos, *staticNorm;
+    int numStatic;
 } OrcStateD;
 
 }  // extern "C"
@@ -797,6 +884,7 @@ template <class T, class S> static State<T> to_state(S *o) {
     s.rbPos = o->rbPos; s.rbVel = o->rbVel; s.rbAng = o->rbAng; s.qinc = o->qinc; s.qcum = o->qcum;
     s.Ib = o->Ib; s.Iw = o->Iw; s.mass = o->mass; s.pstart = o->pstart; s.pcount = o->pcount;
     s.model = o->model;
+    s.staticPos = o->staticPos; s.staticNorm = o->staticNorm; s.numStatic = o->numStatic;
     return s;
 }
 
@@ -875,6 +963,81 @@ void orc_wall_force_f64(const double *p3, const double *v3, const ArSimParams *p
 }
 void orc_sincos(const float *x, int n, float *s, float *c) {
     for (int i = 0; i < n; ++i) sincos_det(x[i], s + i, c + i);
+}
+
+/* Range image -> static scene points (RGBDloader, Kernels/customRangeImageLoader.cu:14-109) and
+ * normals (computeNormals, :118-162), FP32 in the device kernel's operation order.
+ *  - depth d = image / 5000 (d < 0.01 -> -1000), X = d (x - cx) / fx, Y = d (y - cy) / fy with the
+ *    hard-coded intrinsics fx 517.3, fy 516.5, cx 318.6, cy 255.3;
+ *  - p = transpose(cam) * (X, Y, d, 0), then the fixed gravity rotation G (columns (1,0,0),
+ *    -g, (0, g.z, -g.y)) with g the normalised accelerometer vector (0.478957, -8.010560,
+ *    4.166928); a 4x4 matrix-vector product sums (c0 v0 + c1 v1) + (c2 v2 + c3 v3) by columns, as
+ *    GLM 0.9.7 does;
+ *  - the point is (p.x, -p.y, -p.z); pixels with y*W + x > (W-1)(H-1) are not written (the
+ *    reference's bound);
+ *  - normal of an interior pixel = normalize(cross(P[y+1][x] - P[y-1][x], P[y][x+1] - P[y][x-1])),
+ *    the border is not written; normalize = v * (1 / sqrt(v.v)).
+ * cam is column-major.  vao (5 floats per pixel: position, x / W, y / H) and radii may be NULL.
+ * The reference is built with nvcc's default multiply-add contraction, so its values can differ
+ * from these (and from the device kernel's) in the last bits; g and the normalisation use a
+ * correctly rounded square root where the reference uses an approximate reciprocal square root. */
+void orc_range_image(const uint16_t *img, int W, int H, const float *cam, float radius, float *pos, float *norm,
+                     float *radii, float *vao) {
+    const float fx = 517.3f, fy = 516.5f, cx = 318.6f, cy = 255.3f;
+    const double g0 = 0.478957, g1 = -8.010560, g2 = 4.166928;
+    const double gl = std::sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+    const float gx = (float)(g0 / gl), gy = (float)(g1 / gl), gz = (float)(g2 / gl);
+    const long bound = (long)(W - 1) * (long)(H - 1);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const long idx = (long)y * W + x;
+            if (idx > bound) continue;
+            float d = (float)img[idx];
+            d = d / 5000.f;
+            if (radii) radii[idx] = radius;
+            if ((double)d < 0.01) d = -1000.f;   // compared in double, as the reference's literal is
+            const float X = d * ((float)x - cx) / fx;
+            const float Y = d * ((float)y - cy) / fy;
+            const float v[4] = {X, Y, d, 0.f};
+            float t[4];
+            for (int r = 0; r < 4; ++r)   // transpose(cam): element (r, c) = cam[4 r + c]
+                t[r] = (cam[4 * r + 0] * v[0] + cam[4 * r + 1] * v[1]) + (cam[4 * r + 2] * v[2] + cam[4 * r + 3] * v[3]);
+            const float G[16] = {1.f, 0.f, 0.f, 0.f, -gx, -gy, -gz, 0.f, 0.f, gz, -gy, 0.f, 0.f, 0.f, 0.f, 1.f};
+            float u[4];
+            for (int r = 0; r < 4; ++r)
+                u[r] = (G[r] * t[0] + G[4 + r] * t[1]) + (G[8 + r] * t[2] + G[12 + r] * t[3]);
+            const float px = u[0], py = -u[1], pz = -u[2];
+            if (vao) {
+                vao[5 * idx] = px; vao[5 * idx + 1] = py; vao[5 * idx + 2] = pz;
+                vao[5 * idx + 3] = (float)x / (float)W;
+                vao[5 * idx + 4] = (float)y / (float)H;
+            }
+            pos[4 * idx] = px; pos[4 * idx + 1] = py; pos[4 * idx + 2] = pz; pos[4 * idx + 3] = 0.f;
+        }
+    for (int y = 1; y <= H - 2; ++y)
+        for (int x = 1; x <= W - 2; ++x) {
+            const long idx = (long)y * W + x;
+            auto P = [&](int yy, int xx, int c) { return pos[4 * ((long)yy * W + xx) + c]; };
+            const float dx[3] = {P(y, x + 1, 0) - P(y, x - 1, 0), P(y, x + 1, 1) - P(y, x - 1, 1), P(y, x + 1, 2) - P(y, x - 1, 2)};
+            const float dy[3] = {P(y + 1, x, 0) - P(y - 1, x, 0), P(y + 1, x, 1) - P(y - 1, x, 1), P(y + 1, x, 2) - P(y - 1, x, 2)};
+            const float n0 = dy[1] * dx[2] - dy[2] * dx[1];
+            const float n1 = dy[2] * dx[0] - dy[0] * dx[2];
+            const float n2 = dy[0] * dx[1] - dy[1] * dx[0];
+            const float inv = 1.0f / std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+            norm[4 * idx] = n0 * inv; norm[4 * idx + 1] = n1 * inv; norm[4 * idx + 2] = n2 * inv; norm[4 * idx + 3] = 0.f;
+        }
+}
+
+/* Static scene DEM as a stage on a state (sorted static points, their normals, static cell bounds). */
+int orc_dem_static_f32(OrcStateF *o, const ArSimParams *p, const ArSimStepConfig *cfg, const uint32_t *cs,
+                       const uint32_t *ce, const float *ssPos, const float *ssNorm, int ref) {
+    State<float> s = to_state<float>(o); Phys<float> f = mkphys<float>(p, cfg);
+    dem_static(s, f, p, cs, ce, ssPos, ssNorm, ref != 0); return 0;
+}
+int orc_dem_static_f64(OrcStateD *o, const ArSimParams *p, const ArSimStepConfig *cfg, const uint32_t *cs,
+                       const uint32_t *ce, const float *ssPos, const float *ssNorm, int ref) {
+    State<double> s = to_state<double>(o); Phys<double> f = mkphys<double>(p, cfg);
+    dem_static(s, f, p, cs, ce, ssPos, ssNorm, ref != 0); return 0;
 }
 
 }  // extern "C"
