@@ -77,7 +77,7 @@ __global__ void k_body_integrate(BodyBufs b, int B, DevPhys ph, ZeroJobs z, int 
 // (particleKernels.cu:50-129) for grains, then the broadphase key of the updated position:
 // calcHashD (particleKernels.cu:170-189) or generateMortonCodes (BVHkernels.cu:670-716).
 template <int MODE>
-__device__ __forceinline__ void pose_key_one(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko,
+__device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko,
                                              int doBodies, int doGrains, unsigned *shist);
 
 // Adds 1 to an LDS histogram bin for every active lane.  Spatially ordered keys share digits across
@@ -105,7 +105,11 @@ __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N,
         for (int k = threadIdx.x; k < ko.histPasses * 256; k += blockDim.x) shist[k] = 0u;
         __syncthreads();
     }
-    pose_key_one<MODE>(p, b, N, ph, sv, ko, doBodies, doGrains, hist ? shist : nullptr);
+    // a block handles several 256-particle chunks (grid-stride), so that fewer block histograms
+    // are flushed into the global ones: the flush is a global atomic per non-empty bin
+    const int stride = (int)(gridDim.x * blockDim.x);
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < N; i += stride)
+        pose_key_one<MODE>(i, p, b, N, ph, sv, ko, doBodies, doGrains, hist ? shist : nullptr);
     if (hist) {
         __syncthreads();
         for (int k = threadIdx.x; k < ko.histPasses * 256; k += blockDim.x)
@@ -114,9 +118,8 @@ __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N,
 }
 
 template <int MODE>
-__device__ __forceinline__ void pose_key_one(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko,
+__device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko,
                                              int doBodies, int doGrains, unsigned *shist) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int rbi = p.rb ? p.rb[i] : -1;
     const int s = p.scene ? (int)p.scene[i] : 0;
@@ -368,7 +371,9 @@ void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, con
                      int mode, const KeyOut &ko, bool doBodies, bool doGrains, hipStream_t st) {
     if (numParticles <= 0) return;
     const int block = 256;
-    dim3 grid(blocks_for(numParticles, block));
+    // with digit histograms, at most 2048 blocks (8 per CU): each flushes its LDS histogram once
+    const long long need = blocks_for(numParticles, block);
+    dim3 grid((unsigned)(ko.hist && mode != kKeyNone && need > 2048 ? 2048 : need));
     int db = doBodies ? 1 : 0, dg = doGrains ? 1 : 0;
     switch (mode) {
         case kKeyGrid:

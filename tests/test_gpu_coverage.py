@@ -46,9 +46,12 @@ def test_reference_compat_flags_step(method):
     print(f"engine: compat flags method={method}", flush=True)
     sc = scenes.mix_scene(8000, 3, method)
     sc.cfg.flags = capi.COMPAT_REF_UNITS | capi.COMPAT_DROP_FIRST | capi.COMPAT_GRAIN_CLAMP
-    # push grains through the walls so the clamp acts
-    g_idx = sc.rb < 0
-    sc.vel[g_idx, :3] = np.float32(300.0) * np.sign(sc.vel[g_idx, :3])
+    # forces act as F / m per step in the reference units: a soft spring keeps 4 steps finite
+    sc.params.spring, sc.params.damping, sc.params.shear = 20.0, 0.2, 0.05
+    # grains outside the walls, so that the clamp acts on the first step
+    g_idx = np.where(sc.rb < 0)[0]
+    sc.pos[g_idx[:50], 0] = sc.cfg.boxMax[0] + 0.3
+    sc.pos[g_idx[50:100], 2] = sc.cfg.boxMin[2] - 0.3
     sc.rb_ang[:, :3] = np.float32(2.0)
     with Engine(sc, use_graph=False) as e:
         e.step(4)
@@ -56,6 +59,8 @@ def test_reference_compat_flags_step(method):
         g = e.download()
     st = oracle.State(sc, np.float32)
     st.step(4)
+    assert np.isfinite(st.pos).all() and np.isfinite(st.vel).all()
+    assert st.pos[g_idx[:50], 0].max() < sc.cfg.boxMax[0]
     assert_state_equal(g, st, "compat")
 
 
