@@ -11,6 +11,7 @@
 // order-preserving, which tests/ check by running a scene alone and in a mix.
 // The one-at-a-time mode serves the same stream FIFO on the plain path (one engine per scene).
 #include <algorithm>
+#include <cstdio>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -127,6 +128,15 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
     const auto t0 = clk::now();
     auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
     ArSimTuning tune{128, 16, 8, 0};
+    double lastReport = 0.0;
+    auto progress = [&](int done) {   // a line on stderr every 10 s of wall time
+        const double t = now();
+        if (t - lastReport >= 10.0) {
+            lastReport = t;
+            std::fprintf(stderr, "[serve] %.0f s: %d / %d requests completed\n", t, done, cfg->numRequests);
+            std::fflush(stderr);
+        }
+    };
 
     if (cfg->oneAtATime) {
         for (auto &r : reqs) {
@@ -157,6 +167,7 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
             peakBytes = std::max(peakBytes, (double)eng.devBytes);
             ++completed;
             maxResident = 1;
+            progress(completed);
         }
     } else {
         std::deque<int> pending;
@@ -233,6 +244,7 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
                 maxResident = std::max(maxResident, (int)resident.size());
                 peakBytes = std::max(peakBytes, (double)eng->devBytes);
             }
+            progress(completed);
             double s0 = now();
             int rc = eng->step(1);
             if (rc) return rc;
