@@ -170,6 +170,7 @@ SIGNATURES = {
     "arsim_scene_add_grains": (I, [VP, I, VP, VP]),
     "arsim_engine_snapshot": (I, [VP, I, PVP]),
     "arsim_engine_render_state": (I, [VP, I, VP, VP]),
+    "arsim_engine_rebuild": (I, [VP, VP, VP, I, VP, I]),
     "arsim_engine_set_static": (I, [VP, VP, VP, I]),
     "arsim_range_image_load": (I, [VP, I, I, VP, F, VP, VP, VP, VP, VP]),
     "arsim_dem_static_grid": (I, [VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, I, VP, VP, VP]),

@@ -336,6 +336,54 @@ __global__ void k_fill_u16(uint16_t *p, uint16_t v, size_t n) {
 
 inline unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Last range whose start is <= i (ranges sorted by start, tiling the index space).
+template <class F>
+__device__ __forceinline__ int range_of(const CarryRange *__restrict__ tab, int R, int i, F start) {
+    int lo = 0, hi = R;   // invariant: start(lo) <= i < start(hi)
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start(tab[mid]) <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_carry_particles(const CarryRange *__restrict__ tab, int R, int carriedN, StateBufs src,
+                                  StateBufs dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= carriedN) return;
+    const CarryRange r = tab[range_of(tab, R, i, [](const CarryRange &c) { return c.dstP; })];
+    const int j = r.srcP + (i - r.dstP);
+    dst.pos[i] = src.pos[j];
+    dst.vel[i] = src.vel[j];
+    dst.bodyFrame[i] = src.bodyFrame[j];
+    const int v = src.rb[j];
+    dst.rb[i] = v >= 0 ? v - r.srcB + r.dstB : v;
+    dst.scene[i] = (uint16_t)r.scene;
+}
+
+__global__ void k_carry_bodies(const CarryRange *__restrict__ tab, int R, int carriedB, StateBufs src,
+                               StateBufs dst) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= carriedB) return;
+    const CarryRange r = tab[range_of(tab, R, k, [](const CarryRange &c) { return c.dstB; })];
+    const int j = r.srcB + (k - r.dstB);
+    dst.rbPos[k] = src.rbPos[j];
+    dst.rbVel[k] = src.rbVel[j];
+    dst.rbAng[k] = src.rbAng[j];
+    dst.qinc[k] = src.qinc[j];
+    dst.qcum[k] = src.qcum[j];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        dst.Ib[9 * (size_t)k + c] = src.Ib[9 * (size_t)j + c];
+        dst.Iw[9 * (size_t)k + c] = src.Iw[9 * (size_t)j + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) dst.model[16 * (size_t)k + c] = src.model[16 * (size_t)j + c];
+    dst.mass[k] = src.mass[j];
+    dst.pstart[k] = src.pstart[j] - r.srcP + r.dstP;
+    dst.pcount[k] = src.pcount[j];
+}
+
 }  // namespace
 
 void launch_copy_offset(int *dst, const int *src, int n, int delta, hipStream_t st) {
@@ -425,4 +473,17 @@ void launch_check_finite(const float4 *a, int n, int *flag, hipStream_t st) {
     hipLaunchKernelGGL(k_check_finite, dim3(blocks_for(n, 256)), dim3(256), 0, st, a, n, flag);
 }
 
+}  // namespace arsim
+
+namespace arsim {
+void launch_carry(const CarryRange *tab, int numRanges, int carriedN, int carriedB, const StateBufs &src,
+                  const StateBufs &dst, hipStream_t st) {
+    if (numRanges <= 0) return;
+    if (carriedN > 0)
+        hipLaunchKernelGGL(k_carry_particles, dim3(blocks_for(carriedN, 256)), dim3(256), 0, st, tab, numRanges,
+                           carriedN, src, dst);
+    if (carriedB > 0)
+        hipLaunchKernelGGL(k_carry_bodies, dim3(blocks_for(carriedB, 256)), dim3(256), 0, st, tab, numRanges,
+                           carriedB, src, dst);
+}
 }  // namespace arsim

@@ -58,14 +58,36 @@ void attach_hist(KeyOut &ko, const SortPlan &pl, const SortTuning &stt, int keyB
 
 }  // namespace
 
+// Makes *p hold at least count elements.  A buffer that is large enough is kept (its contents are
+// not preserved); a smaller one is replaced by one 25 % larger than asked, so that an engine that is
+// rebuilt for a growing scene set reallocates rarely.
 template <class T> int Engine::alloc(T **p, size_t count) {
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    for (auto &a : allocs) {
+        if (a.first != (void **)p) continue;
+        if (a.second >= bytes) return ARSIM_OK;
+        hipFree(*p);
+        *p = nullptr;
+        devBytes -= a.second;
+        const size_t grown = bytes + bytes / 4;
+        void *d = nullptr;
+        hipError_t e = hipMalloc(&d, grown);
+        if (e != hipSuccess) {
+            a.second = 0;
+            return fail(e == hipErrorOutOfMemory ? ARSIM_ERR_OUT_OF_MEMORY : ARSIM_ERR_HIP,
+                        std::string("engine allocation: ") + hipGetErrorString(e));
+        }
+        a.second = grown;
+        devBytes += grown;
+        *p = (T *)d;
+        return ARSIM_OK;
+    }
     void *d = nullptr;
-    size_t bytes = std::max<size_t>(count * sizeof(T), 16);
     hipError_t e = hipMalloc(&d, bytes);
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? ARSIM_ERR_OUT_OF_MEMORY : ARSIM_ERR_HIP,
                     std::string("engine allocation: ") + hipGetErrorString(e));
-    allocs.push_back(d);
+    allocs.push_back({(void **)p, bytes});
     devBytes += bytes;
     *p = (T *)d;
     return ARSIM_OK;
@@ -74,7 +96,8 @@ template <class T> int Engine::alloc(T **p, size_t count) {
 Engine::~Engine() {
     free_static();
     if (gexec) hipGraphExecDestroy(gexec);
-    for (void *p : allocs) hipFree(p);
+    for (auto &a : allocs)
+        if (*a.first) hipFree(*a.first);
     if (st && ownStream) hipStreamDestroy(st);
 }
 
@@ -154,6 +177,14 @@ SceneView Engine::scene_view() const {
     return sv;
 }
 
+StateBufs Engine::state_bufs() const {
+    StateBufs b{};
+    b.pos = pos; b.vel = vel; b.bodyFrame = bodyFrame; b.rb = rb; b.scene = scene;
+    b.rbPos = rbPos; b.rbVel = rbVel; b.rbAng = rbAng; b.qinc = qinc; b.qcum = qcum;
+    b.Ib = Ib; b.Iw = Iw; b.mass = mass; b.model = model; b.pstart = pstartD; b.pcount = pcountD;
+    return b;
+}
+
 BodyBufs Engine::body_bufs() const {
     BodyBufs b{};
     b.pos = rbPos; b.vel = rbVel; b.ang = rbAng; b.qinc = qinc; b.qcum = qcum;
@@ -185,8 +216,17 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     if (src.size() > 65535) return fail(ARSIM_ERR_INVALID_ARG, "engine: at most 65535 scenes");
     device = dev;
     ARSIM_HIP(hipSetDevice(device));
-    if (stream) { st = stream; ownStream = false; }
-    else ARSIM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    if (stream) {
+        if (st && ownStream) hipStreamDestroy(st);
+        st = stream;
+        ownStream = false;
+    } else if (!st) {
+        ARSIM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        ownStream = true;
+    }
+    // a rebuild starts from a clean step state; buffers are kept where they are large enough
+    invalidate_graph();
+    free_static();
     if (tuning) tu = *tuning;
     if (tu.demBlock <= 0 || tu.demBlock > 1024 || (tu.demBlock & 63)) return fail(ARSIM_ERR_INVALID_ARG, "bad demBlock");
     if (!sort_tuning_supported(tu.sortItemsPerThread, tu.sortRadixBits))
@@ -209,6 +249,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
             d[s] = Desc{h->numParticles, h->numBodies, h->params, h->cfg};
         } else if (src[s].from) {
             const Engine *o = src[s].from;
+            if (o == this) return fail(ARSIM_ERR_INVALID_ARG, "engine: a rebuild cannot read its own buffers");
             int k = src[s].fromScene;
             if (k < 0 || k >= o->S) return fail(ARSIM_ERR_INVALID_ARG, "engine: bad source scene");
             d[s] = Desc{o->pCount[k], o->bCount[k], o->p0, o->c0};
@@ -290,7 +331,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     size_t nb = std::max(B, 1);
     A(rbPos, nb); A(rbVel, nb); A(rbAng, nb); A(qinc, nb); A(qcum, nb);
     A(Ib, nb * 9); A(Iw, nb * 9); A(mass, nb); A(pstartD, nb); A(pcountD, nb); A(model, nb * 16);
-    A(geomD, (size_t)S); A(sceneStartD, (size_t)S); A(sceneCountD, (size_t)S);
+    A(geomD, (size_t)S); A(sceneStartD, (size_t)S); A(sceneCountD, (size_t)S); A(carryTab, (size_t)S);
     {
         unsigned char *ka = nullptr, *kb = nullptr;
         A(ka, (size_t)N * keyBytes); A(kb, (size_t)N * keyBytes);
@@ -312,7 +353,8 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         A(tree.left, nodes); A(tree.right, nodes); A(tree.parent, nodes);
         A(tree.packed, 4 * (size_t)N);
         A(tree.bounds, 2 * nodes); A(tree.cms, N); A(tree.sortedVel, N); A(tree.counter, N);
-        if (S > 1) A(tree.sceneRoot, S);
+        if (S > 1) A(sceneRootBuf, S);
+        tree.sceneRoot = S > 1 ? sceneRootBuf : nullptr;
         A(aabbPart, (size_t)2 * aabb_max_partials()); A(aabbBox, 2);
     }
 #undef A
@@ -320,8 +362,23 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     // ---- fill
     std::vector<int> hStart(S), hCount(S);
     for (int s = 0; s < S; ++s) { hStart[s] = pStart[s]; hCount[s] = pCount[s]; }
-    std::vector<float4> zeroN;
-    for (int s = 0; s < S; ++s) {
+    // Scenes carried over from one other engine that form a prefix of the new layout move in one
+    // launch per array group (a rebuild for a new scene set); others are copied scene by scene.
+    int bulk = 0;
+    const Engine *bulkFrom = S > 0 ? src[0].from : nullptr;
+    while (bulk < S && bulkFrom && src[bulk].from == bulkFrom) ++bulk;
+    if (bulk > 0) {
+        std::vector<CarryRange> tab(bulk);
+        for (int s = 0; s < bulk; ++s) {
+            const int k = src[s].fromScene;
+            tab[s] = CarryRange{pStart[s], bulkFrom->pStart[k], pCount[s], bStart[s], bulkFrom->bStart[k], bCount[s], s};
+        }
+        ARSIM_HIP(hipMemcpyAsync(carryTab, tab.data(), sizeof(CarryRange) * bulk, hipMemcpyHostToDevice, st));
+        const int carriedN = pStart[bulk - 1] + pCount[bulk - 1], carriedB = bStart[bulk - 1] + bCount[bulk - 1];
+        launch_carry(carryTab, bulk, carriedN, carriedB, bulkFrom->state_bufs(), state_bufs(), st);
+        ARSIM_HIP(hipStreamSynchronize(st));   // tab is a host temporary
+    }
+    for (int s = bulk; s < S; ++s) {
         const size_t po = pStart[s], bo = bStart[s], n = pCount[s], b = bCount[s];
         if (src[s].host) {
             const ArSimSceneHost *h = src[s].host;
@@ -600,6 +657,27 @@ int arsim_engine_create(const ArSimSceneHost *scenes, int numScenes, const ArSim
     if (rc) { delete e; return rc; }
     *out = e;
     return ARSIM_OK;
+}
+
+int arsim_engine_rebuild(arsim_engine *e, arsim_engine *from, const int *keepScenes, int numKeep,
+                         const ArSimSceneHost *added, int numAdded) {
+    if (!e || numKeep < 0 || numAdded < 0 || numKeep + numAdded <= 0 || (numKeep > 0 && (!from || !keepScenes)) ||
+        (numAdded > 0 && !added) || from == e)
+        return fail(ARSIM_ERR_INVALID_ARG, "engine_rebuild: bad arguments");
+    std::vector<arsim::SceneSource> src(numKeep + numAdded);
+    for (int k = 0; k < numKeep; ++k) {
+        if (keepScenes[k] < 0 || keepScenes[k] >= from->e.S)
+            return fail(ARSIM_ERR_INVALID_ARG, "engine_rebuild: bad scene index");
+        src[k].from = &from->e;
+        src[k].fromScene = keepScenes[k];
+    }
+    for (int k = 0; k < numAdded; ++k) src[numKeep + k].host = &added[k];
+    if (from) {
+        ARSIM_HIP(hipSetDevice(from->e.device));
+        ARSIM_HIP(hipStreamSynchronize(from->e.st));
+    }
+    const ArSimTuning tu = e->e.tu;
+    return e->e.build(src, &tu, e->e.device, nullptr);
 }
 
 int arsim_engine_destroy(arsim_engine *e) {

@@ -33,9 +33,12 @@ struct Engine {
     int cellBits = 0, sceneBits = 0, keyBits = 0, keyBytes = 4;
     size_t numCells = 0;
 
-    // device buffers
-    std::vector<void *> allocs;
+    // device buffers: each member pointer keeps its allocation (and capacity) across builds, so an
+    // engine object can be rebuilt for a new scene set without new allocations when it fits
+    std::vector<std::pair<void **, size_t>> allocs;
     size_t devBytes = 0;
+    int *sceneRootBuf = nullptr;            // tree.sceneRoot points here when S > 1
+    CarryRange *carryTab = nullptr;         // scene ranges of a bulk carry-over
     float4 *pos = nullptr, *vel = nullptr, *relPos = nullptr, *bodyFrame = nullptr, *pForce = nullptr;
     int *rb = nullptr;
     uint16_t *scene = nullptr;
@@ -81,6 +84,7 @@ struct Engine {
     static const char *const *stage_names(int method, int *count);
     SceneView scene_view() const;
     BodyBufs body_bufs() const;
+    StateBufs state_bufs() const;
     PartBufs part_bufs() const;
     size_t scene_bytes_estimate() const;
 

@@ -85,6 +85,23 @@ void launch_check_finite(const float4 *a, int n, int *flag, hipStream_t st);
 void launch_copy_offset(int *dst, const int *src, int n, int delta, hipStream_t st);
 void launch_fill_u16(uint16_t *p, uint16_t value, size_t count, hipStream_t st);
 
+// Scene carry-over between two engines' buffers (a rebuild for a new scene set): destination
+// scenes [dstP, dstP + n) / [dstB, dstB + b) come from [srcP, ...) / [srcB, ...), ranges in
+// ascending dstP / dstB order and tiling [0, carriedN) / [0, carriedB).
+struct CarryRange {
+    int dstP, srcP, n, dstB, srcB, b, scene;
+};
+struct StateBufs {
+    float4 *pos, *vel, *bodyFrame;
+    int *rb;
+    uint16_t *scene;
+    float4 *rbPos, *rbVel, *rbAng, *qinc, *qcum;
+    float *Ib, *Iw, *mass, *model;
+    int *pstart, *pcount;
+};
+void launch_carry(const CarryRange *tab, int numRanges, int carriedN, int carriedB, const StateBufs &src,
+                  const StateBufs &dst, hipStream_t st);
+
 // ---------------------------------------------------------------- grid path (grid.hip)
 // sortedPos.w carries the particle's rbIndex bits; with sceneInVelW sortedVel.w carries its scene id.
 void launch_reorder(const unsigned *sortedKeys, const unsigned *sortedIdx, const float4 *pos, const float4 *vel,

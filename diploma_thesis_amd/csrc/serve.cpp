@@ -139,6 +139,7 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
     };
 
     if (cfg->oneAtATime) {
+        arsim::Engine eng;   // rebuilt for every request, keeping its allocations
         for (auto &r : reqs) {
             while (now() < r.arrival) std::this_thread::sleep_for(std::chrono::microseconds(200));
             ArSimSceneOwned *sc = nullptr;
@@ -147,7 +148,6 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
             ArSimSceneHost v;
             arsim_scene_view(sc, &v);
             r.admitted = now();
-            arsim::Engine eng;
             std::vector<arsim::SceneSource> src(1);
             src[0].host = &v;
             ArSimTuning tg = tune;
@@ -173,7 +173,12 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
         std::deque<int> pending;
         for (int i = 0; i < cfg->numRequests; ++i) pending.push_back(i);
         std::vector<int> resident;            // request ids, in engine scene order
-        std::unique_ptr<arsim::Engine> eng;
+        // two engines alternate: a change of the resident set rebuilds the idle one from the
+        // active one (surviving scenes move on the device in one launch per array group) and
+        // keeps its allocations from earlier rebuilds
+        std::unique_ptr<arsim::Engine> engs[2];
+        int cur = 0;
+        arsim::Engine *eng = nullptr;
         size_t residentBytes = 0;
         std::vector<char> countedReject(cfg->numRequests, 0);
         while (completed < cfg->numRequests) {
@@ -211,7 +216,7 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
                 changed = true;
             }
             if (keep.empty() && admitted.empty()) {
-                eng.reset();
+                eng = nullptr;
                 resident.clear();
                 if (pending.empty()) break;
                 double wait = reqs[pending.front()].arrival - now();
@@ -223,7 +228,7 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
                 std::vector<ArSimSceneHost> views(admitted.size());
                 for (size_t k = 0; k < keep.size(); ++k) {
                     arsim::SceneSource s;
-                    s.from = eng.get();
+                    s.from = eng;
                     s.fromScene = (int)(std::find(resident.begin(), resident.end(), keep[k]) - resident.begin());
                     src.push_back(s);
                 }
@@ -233,16 +238,19 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
                     s.host = &views[k];
                     src.push_back(s);
                 }
-                std::unique_ptr<arsim::Engine> ne(new arsim::Engine());
-                int rc = ne->build(src, &tune, 0, nullptr);
+                const int nxt = eng ? 1 - cur : cur;
+                if (!engs[nxt]) engs[nxt].reset(new arsim::Engine());
+                if (eng) hipStreamSynchronize(eng->st);
+                int rc = engs[nxt]->build(src, &tune, 0, nullptr);
                 for (int id : admitted) { arsim_scene_free(reqs[id].scene); reqs[id].scene = nullptr; }
                 if (rc) return rc;
-                if (eng) hipStreamSynchronize(eng->st);
-                eng = std::move(ne);
+                cur = nxt;
+                eng = engs[cur].get();
                 resident = keep;
                 resident.insert(resident.end(), admitted.begin(), admitted.end());
                 maxResident = std::max(maxResident, (int)resident.size());
-                peakBytes = std::max(peakBytes, (double)eng->devBytes);
+                peakBytes = std::max(peakBytes, (double)(engs[0] ? engs[0]->devBytes : 0) +
+                                                    (double)(engs[1] ? engs[1]->devBytes : 0));
             }
             progress(completed);
             double s0 = now();

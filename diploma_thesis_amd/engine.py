@@ -83,6 +83,18 @@ class Engine:
         capi.check(capi.lib.arsim_engine_snapshot(self.h, int(scene), C.byref(owned)), "engine_snapshot")
         return _from_owned(owned)
 
+    def rebuild(self, source, keep=(), added=()):
+        """Holds scenes `keep` of engine `source` (state carried over on the device) followed by the
+        host scenes `added`; this engine's allocations are reused where they fit."""
+        keep = np.ascontiguousarray(list(keep), np.int32)
+        added = list(added)
+        views = (capi.ArSimSceneHost * max(1, len(added)))(*[s.view() for s in added])
+        capi.check(capi.lib.arsim_engine_rebuild(self.h, source.h if source is not None else None,
+                                                 capi.ptr(keep) if keep.size else None, int(keep.size),
+                                                 views if added else None, len(added)), "engine_rebuild")
+        self._views = views
+        self.scenes = [source.scenes[int(k)] for k in keep] + added
+
     def set_static(self, pos4, norm4):
         """Static collider points and normals (float4 each) for a one-scene engine; None clears them."""
         if pos4 is None:

@@ -428,6 +428,13 @@ int arsim_scene_add_grains(ArSimSceneOwned *s, int n, const float *pos4, const f
  * fixtures): particle pos / vel, body position, velocity, angular velocity and orientation, plus the
  * scene's fixed data and current dt.  arsim_engine_create on it continues the run bit for bit. */
 int arsim_engine_snapshot(arsim_engine *e, int scene, ArSimSceneOwned **out);
+/* Rebuilds engine e for a new scene set: scenes keepScenes[0..numKeep) of engine `from` (moved on
+ * the device, state and all, in one launch per array group) followed by numAdded new host scenes.
+ * e keeps its allocations where they are large enough, so a serving loop that alternates two engines
+ * changes its resident set without new allocations; e != from.  Every kept scene continues bit for
+ * bit as if it had never moved.  No counterpart in the reference (one scene per process there). */
+int arsim_engine_rebuild(arsim_engine *e, arsim_engine *from, const int *keepScenes, int numKeep,
+                         const ArSimSceneHost *added, int numAdded);
 /* Headless render state after the last step (the per-body modelMatrixGPU / cumulativeQuaternionGPU
  * outputs of GPUintegratorKernel, Kernels/rigidBodyKernels.cu:322-332): 16 floats (column-major mat4)
  * and 4 floats (x,y,z,w) per body of the scene.  Either pointer may be NULL. */

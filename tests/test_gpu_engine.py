@@ -151,3 +151,34 @@ def test_serve_small_stream():
         rep = capi.ArSimServeReport()
         capi.check(capi.lib.arsim_serve_run(C.byref(cfg), C.byref(rep)), "serve_run")
         assert rep.completed == 6 and rep.particleUpdates > 0 and rep.stepLatencyP99Ms >= rep.stepLatencyP50Ms
+
+
+@pytest.mark.parametrize("method", [capi.METHOD_GRID, capi.METHOD_LBVH30])
+def test_rebuild_carry_over_matches_alone(method):
+    """The serving loop's resident-set change: two engines alternate, each rebuilt from the other
+    with some scenes dropped and new ones admitted.  Every scene ends bit-identical to stepping it
+    alone for the same number of steps."""
+    print(f"engine: rebuild carry-over method={method}", flush=True)
+    sc = [scenes.mix_scene(n, 90 + n, method) for n in (3000, 1200, 5000, 2500, 4000)]
+    a = Engine(sc[:3])
+    b = Engine([sc[4]])
+    a.step(4)                                   # s0, s1, s2: 4 steps
+    b.rebuild(a, keep=[0, 2], added=[sc[3]])    # s0, s2 carried, s3 admitted
+    b.step(3)                                   # s0 7, s2 7, s3 3
+    a.rebuild(b, keep=[1, 2], added=[sc[4]])    # s2, s3 carried, s4 admitted (a's buffers reused)
+    a.step(5)                                   # s2 12, s3 8, s4 5
+    a.sync()
+    got = {2: a.download(0), 3: a.download(1), 4: a.download(2)}
+    b.rebuild(a, keep=[0])
+    b.step(1)                                   # s2 13
+    b.sync()
+    got[20] = b.download(0)
+    a.close()
+    b.close()
+    for k, steps in ((2, 12), (3, 8), (4, 5), (20, 13)):
+        with Engine(sc[k % 10]) as e:
+            e.step(steps)
+            e.sync()
+            want = e.download()
+        for key in ("pos", "vel", "rb_pos", "rb_vel", "rb_ang", "rb_quat", "rb_iw"):
+            assert np.array_equal(got[k][key], want[key]), (k, key)
