@@ -332,18 +332,13 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     A(rbPos, nb); A(rbVel, nb); A(rbAng, nb); A(qinc, nb); A(qcum, nb);
     A(Ib, nb * 9); A(Iw, nb * 9); A(mass, nb); A(pstartD, nb); A(pcountD, nb); A(model, nb * 16);
     A(geomD, (size_t)S); A(sceneStartD, (size_t)S); A(sceneCountD, (size_t)S); A(carryTab, (size_t)S);
-    {
-        unsigned char *ka = nullptr, *kb = nullptr;
-        A(ka, (size_t)N * keyBytes); A(kb, (size_t)N * keyBytes);
-        keysA = ka; keysB = kb;
-    }
+    // (alloc keys its records by the member's address: only members are allocated through it)
+    A(keyBufA, (size_t)N * keyBytes); A(keyBufB, (size_t)N * keyBytes);
+    keysA = keyBufA; keysB = keyBufB;
     A(valsA, N); A(valsB, N);
     sortWsBytes = sort_workspace_bytes(N, keyBytes);
-    {
-        unsigned char *w = nullptr;
-        A(w, sortWsBytes);
-        sortWs = w;
-    }
+    A(sortWsBuf, sortWsBytes);
+    sortWs = sortWsBuf;
     A(overflowD, 4); A(flagD, 4);
     if (method == ARSIM_METHOD_GRID) {
         A(segStart, (size_t)numSegs + 1);

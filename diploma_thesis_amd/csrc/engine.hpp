@@ -49,6 +49,7 @@ struct Engine {
     SceneGeom *geomD = nullptr;
     int *sceneStartD = nullptr, *sceneCountD = nullptr;
     void *keysA = nullptr, *keysB = nullptr;
+    unsigned char *keyBufA = nullptr, *keyBufB = nullptr, *sortWsBuf = nullptr;
     unsigned *valsA = nullptr, *valsB = nullptr;
     unsigned *segStart = nullptr;           // grid: CSR offsets over segments of 2^segBits cells
     int segBits = 3;
