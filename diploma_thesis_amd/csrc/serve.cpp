@@ -11,6 +11,7 @@
 // order-preserving, which tests/ check by running a scene alone and in a mix.
 // The one-at-a-time mode serves the same stream FIFO on the plain path (one engine per scene).
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <chrono>
 #include <cmath>
@@ -121,6 +122,48 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
         reqs[i].bytes = scene_bytes(reqs[i].n, (int)(0.75 * reqs[i].n / 64), method, 18);
     }
 
+    // ---- clients: request inputs are prepared ahead of their arrival by a few producer threads
+    // (a request arrives with its scene), at most kLookahead requests ahead of the server
+    const int kLookahead = 256;
+    std::vector<std::atomic<int>> ready(cfg->numRequests);
+    for (auto &x : ready) x.store(0);
+    std::atomic<int> nextGen{0}, served{0}, genError{0};
+    auto producer = [&]() {
+        for (;;) {
+            if (genError.load()) return;
+            const int i = nextGen.fetch_add(1);
+            if (i >= cfg->numRequests) return;
+            while (i - served.load() > kLookahead && !genError.load())
+                std::this_thread::sleep_for(std::chrono::microseconds(200));
+            ArSimSceneOwned *sc = nullptr;
+            if (arsim_generate_mix_scene(reqs[i].n, reqs[i].seed, method, &sc) != ARSIM_OK) sc = nullptr;
+            reqs[i].scene = sc;
+            ready[i].store(sc ? 1 : -1);
+            if (!sc) return;
+        }
+    };
+    std::vector<std::thread> producers;
+    for (int k = 0; k < 4; ++k) producers.emplace_back(producer);
+    struct Joiner {   // on every exit: stop the producers and free scenes nobody took
+        std::vector<std::thread> &t;
+        std::atomic<int> &stop;
+        std::vector<Req> &reqs;
+        ~Joiner() {
+            stop.store(1);
+            for (auto &x : t) x.join();
+            for (auto &r : reqs)
+                if (r.scene) { arsim_scene_free(r.scene); r.scene = nullptr; }
+        }
+    } joiner{producers, genError, reqs};
+    // the request's scene, waiting for its producer if it is not ready yet (NULL on failure)
+    auto take = [&](int i) -> ArSimSceneOwned * {
+        while (ready[i].load() == 0) std::this_thread::sleep_for(std::chrono::microseconds(50));
+        ArSimSceneOwned *sc = reqs[i].scene;
+        reqs[i].scene = nullptr;
+        served.store(i + 1);
+        return ready[i].load() > 0 ? sc : nullptr;
+    };
+
     std::vector<double> stepLat, complLat, waitLat;
     double updates = 0;
     int completed = 0, maxResident = 0, rejected = 0;
@@ -142,9 +185,9 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
         arsim::Engine eng;   // rebuilt for every request, keeping its allocations
         for (auto &r : reqs) {
             while (now() < r.arrival) std::this_thread::sleep_for(std::chrono::microseconds(200));
-            ArSimSceneOwned *sc = nullptr;
-            int rc = arsim_generate_mix_scene(r.n, r.seed, method, &sc);
-            if (rc) return rc;
+            ArSimSceneOwned *sc = take((int)(&r - &reqs[0]));
+            if (!sc) return fail(ARSIM_ERR_INVALID_ARG, "serve: request generation failed");
+            int rc = 0;
             ArSimSceneHost v;
             arsim_scene_view(sc, &v);
             r.admitted = now();
@@ -206,8 +249,8 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
                     if (!countedReject[pending.front()]) { ++rejected; countedReject[pending.front()] = 1; }
                     break;
                 }
-                int rc = arsim_generate_mix_scene(r.n, r.seed, method, &r.scene);
-                if (rc) return rc;
+                r.scene = take(pending.front());
+                if (!r.scene) return fail(ARSIM_ERR_INVALID_ARG, "serve: request generation failed");
                 r.admitted = now();
                 waitLat.push_back((r.admitted - r.arrival) * 1e3);
                 residentBytes += r.bytes;
