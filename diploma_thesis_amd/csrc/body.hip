@@ -125,7 +125,8 @@ __device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int 
     const int s = p.scene ? (int)p.scene[i] : 0;
     SceneGeom g;
     load_geom(sv, s, g);
-    float4 P;
+    float4 P, Vfin;
+    bool haveV = false;
     if (rbi >= 0) {
         if (doBodies) {
             float4 Qc = b.qcum[rbi];
@@ -137,7 +138,9 @@ __device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int 
             P = f4(add3(xyz(CM), r), CM.w);
             p.pos[i] = P;
             float4 V = b.vel[rbi], W = b.ang[rbi];
-            p.vel[i] = f4(add3(xyz(V), cross3(xyz(W), r)), V.w);
+            Vfin = f4(add3(xyz(V), cross3(xyz(W), r)), V.w);
+            haveV = true;
+            p.vel[i] = Vfin;
         } else {
             P = p.pos[i];
         }
@@ -164,10 +167,17 @@ __device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int 
             }
             P = f4(x, P.w);
             p.pos[i] = P;
-            p.vel[i] = f4(v, Vv.w);
+            Vfin = f4(v, Vv.w);
+            haveV = true;
+            p.vel[i] = Vfin;
         }
     }
     if (MODE == kKeyNone) return;
+    if (p.rec) {
+        if (!haveV) Vfin = p.vel[i];
+        p.rec[2 * (size_t)i] = make_float4(P.x, P.y, P.z, __int_as_float(rbi));
+        p.rec[2 * (size_t)i + 1] = make_float4(Vfin.x, Vfin.y, Vfin.z, p.scene ? __int_as_float(s) : Vfin.w);
+    }
     uint64_t key;
     if (MODE == kKeyGrid) {
         int gx, gy, gz;

@@ -333,7 +333,8 @@ int arsim_lbvh_refit(const int *sortedIndices, const float *pos, const float *ve
     t.left = (int *)left; t.right = (int *)right; t.parent = (int *)parent; t.minR = minRange; t.maxR = maxRange;
     t.isLeaf = isLeaf; t.bounds = (float4 *)bounds; t.cms = (float4 *)CMs; t.sortedVel = (float4 *)sortedVel;
     t.counter = nodeCounter;
-    launch_lbvh_refit(sortedIndices, (const float4 *)pos, (const float4 *)vel, rbIndices, nullptr, radius, n, t, S(st));
+    launch_lbvh_refit(sortedIndices, (const float4 *)pos, (const float4 *)vel, rbIndices, nullptr, nullptr, radius, n, t,
+                      S(st));
     return check_launch("lbvh_refit");
 }
 

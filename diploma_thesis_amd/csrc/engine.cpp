@@ -196,6 +196,7 @@ PartBufs Engine::part_bufs() const {
     PartBufs p{};
     p.pos = pos; p.vel = vel; p.relPos = relPos; p.bodyFrame = bodyFrame; p.rb = rb;
     p.scene = S > 1 ? scene : nullptr;
+    p.rec = gatherRec;
     return p;
 }
 
@@ -327,7 +328,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     int rc = 0;
 #define A(ptr, cnt) if ((rc = alloc(&(ptr), (cnt))) != 0) return rc
     A(pos, N); A(vel, N); A(relPos, N); A(bodyFrame, N); A(pForce, N);
-    A(rb, N); A(scene, N);
+    A(rb, N); A(scene, N); A(gatherRec, 2 * (size_t)N);
     size_t nb = std::max(B, 1);
     A(rbPos, nb); A(rbVel, nb); A(rbAng, nb); A(qinc, nb); A(qcum, nb);
     A(Ib, nb * 9); A(Iw, nb * 9); A(mass, nb); A(pstartD, nb); A(pcountD, nb); A(model, nb * 16);
@@ -487,8 +488,8 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st, true))
             return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
         rec();
-        launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, sortedPos, sortedVel, segStart, segBits,
-                           numSegs, N, st);
+        launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart,
+                           segBits, numSegs, N, st);
         rec();
         DemOut o{};
         o.pForce = pForce; o.vel = vel; o.relPos = relPos;
@@ -538,7 +539,7 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         rec();
         launch_lbvh_build(sortedKeys, keyBytes, N, S > 1 ? 30 : 0, sceneStartD, sceneCountD, S, tree, st);
         rec();
-        launch_lbvh_refit((const int *)vA, pos, vel, rb, S > 1 ? scene : nullptr, ph.r, N, tree, st);
+        launch_lbvh_refit((const int *)vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, ph.r, N, tree, st);
         rec();
         DemOut o{};
         o.pForce = pForce; o.vel = vel; o.relPos = relPos;
@@ -599,8 +600,8 @@ int Engine::enqueue_broadphase(int *pairsD, int *countsD, int maxPer) {
         unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
         if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
             return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
-        launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, sortedPos, sortedVel, segStart, segBits,
-                           numSegs, N, st);
+        launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart,
+                           segBits, numSegs, N, st);
         launch_dem_grid_seg(sortedPos, sortedVel, kA, vA, segStart, segBits, N, ph, sv, S > 1, o, tu.demBlock, st);
         prevSorted = kA;
         lastKeys = kA;
@@ -625,7 +626,7 @@ int Engine::enqueue_broadphase(int *pairsD, int *countsD, int maxPer) {
             sortedKeys = kA;
         }
         launch_lbvh_build(sortedKeys, keyBytes, N, S > 1 ? 30 : 0, sceneStartD, sceneCountD, S, tree, st);
-        launch_lbvh_refit((const int *)vA, pos, vel, rb, S > 1 ? scene : nullptr, ph.r, N, tree, st);
+        launch_lbvh_refit((const int *)vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, ph.r, N, tree, st);
         launch_dem_lbvh((const int *)vA, tree, N, ph, sv, S > 1, o, tu.demBlock, st, overflowD);
         lastKeys = sortedKeys;
         lastIdx = vA;

@@ -42,6 +42,7 @@ struct Engine {
     float4 *pos = nullptr, *vel = nullptr, *relPos = nullptr, *bodyFrame = nullptr, *pForce = nullptr;
     int *rb = nullptr;
     uint16_t *scene = nullptr;
+    float4 *gatherRec = nullptr;   // the key kernel's 32-byte gather record per particle
     float4 *rbPos = nullptr, *rbVel = nullptr, *rbAng = nullptr, *qinc = nullptr, *qcum = nullptr;
     float *Ib = nullptr, *Iw = nullptr, *mass = nullptr;
     float *model = nullptr;       // per-body column-major mat4 (render state), written each step

@@ -109,8 +109,8 @@ __global__ void k_dem_grid(const float4 *__restrict__ sPos, const float4 *__rest
 __global__ void k_reorder_seg(const unsigned *__restrict__ keys, const unsigned *__restrict__ idx,
                               const float4 *__restrict__ pos, const float4 *__restrict__ vel,
                               const int *__restrict__ rb, const uint16_t *__restrict__ scene,
-                              float4 *__restrict__ sPos, float4 *__restrict__ sVel, unsigned *__restrict__ segStart,
-                              int segBits, long long numSegs, int n) {
+                              const float4 *__restrict__ rec, float4 *__restrict__ sPos, float4 *__restrict__ sVel,
+                              unsigned *__restrict__ segStart, int segBits, long long numSegs, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
     const unsigned k = valid ? keys[i] : 0u;
@@ -120,6 +120,11 @@ __global__ void k_reorder_seg(const unsigned *__restrict__ keys, const unsigned 
     wave_fill(segStart, i == n - 1, s + 1u, (unsigned)numSegs + 1u, (unsigned)n);
     if (!valid) return;
     const unsigned o = idx[i];
+    if (rec) {
+        sPos[i] = rec[2 * (size_t)o];
+        sVel[i] = rec[2 * (size_t)o + 1];
+        return;
+    }
     float4 p = pos[o];
     p.w = __int_as_float(rb[o]);
     sPos[i] = p;
@@ -292,12 +297,12 @@ __global__ void k_dem_grid_seg(const float4 *__restrict__ sPos, const float4 *__
 }  // namespace
 
 void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, const float4 *pos, const float4 *vel,
-                        const int *rb, const uint16_t *scene, float4 *sortedPos, float4 *sortedVel,
-                        unsigned *segStart, int segBits, long long numSegs, int n, hipStream_t st) {
+                        const int *rb, const uint16_t *scene, const float4 *rec, float4 *sortedPos,
+                        float4 *sortedVel, unsigned *segStart, int segBits, long long numSegs, int n, hipStream_t st) {
     if (n <= 0) return;
     const int block = 256;
     hipLaunchKernelGGL(k_reorder_seg, dim3((n + block - 1) / block), dim3(block), 0, st, sortedKeys, sortedIdx, pos,
-                       vel, rb, scene, sortedPos, sortedVel, segStart, segBits, numSegs, n);
+                       vel, rb, scene, rec, sortedPos, sortedVel, segStart, segBits, numSegs, n);
 }
 
 void launch_dem_grid_seg(const float4 *sortedPos, const float4 *sortedVel, const unsigned *sortedKeys,

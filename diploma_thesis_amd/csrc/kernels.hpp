@@ -49,6 +49,10 @@ struct PartBufs {
     const float4 *bodyFrame;
     const int *rb;
     const uint16_t *scene;        // per-particle scene id (NULL when there is one scene)
+    // optional gather record, written by the key kernel in the original order: rec[2i] = (pos.xyz,
+    // rbIndex bits), rec[2i+1] = (vel.xyz, scene id bits or vel.w); the reorder / refit then gather
+    // one 32-byte record per particle instead of pos, vel, rbIndex and scene separately
+    float4 *rec;
 };
 
 enum KeyMode { kKeyNone = 0, kKeyGrid = 1, kKeyMorton30 = 2, kKeyMorton63 = 3, kKeyForest30 = 4 };
@@ -122,9 +126,10 @@ void launch_dem_grid(const float4 *sortedPos, const float4 *sortedVel, const uns
 
 // Engine form of the grid broadphase: a dense CSR offset array over segments of 2^segBits cells
 // (segStart has numSegs + 1 entries) replaces cellStart / cellEnd; the DEM scans the sorted keys.
+// rec: the key kernel's gather record (pos / vel / rb / scene are then not read), or NULL
 void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, const float4 *pos, const float4 *vel,
-                        const int *rb, const uint16_t *scene, float4 *sortedPos, float4 *sortedVel,
-                        unsigned *segStart, int segBits, long long numSegs, int n, hipStream_t st);
+                        const int *rb, const uint16_t *scene, const float4 *rec, float4 *sortedPos,
+                        float4 *sortedVel, unsigned *segStart, int segBits, long long numSegs, int n, hipStream_t st);
 void launch_dem_grid_seg(const float4 *sortedPos, const float4 *sortedVel, const unsigned *sortedKeys,
                          const unsigned *sortedIdx, const unsigned *segStart, int segBits, int n, const DevPhys &ph,
                          const SceneView &sv, bool sceneInVelW, const DemOut &o, int block, hipStream_t st);
@@ -147,7 +152,7 @@ struct Lbvh {
 void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShift, const int *sceneStart,
                        const int *sceneCount, int numScenes, const Lbvh &t, hipStream_t st);
 void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *vel, const int *rb,
-                       const uint16_t *scene, float radius, int n, const Lbvh &t, hipStream_t st);
+                       const uint16_t *scene, const float4 *rec, float radius, int n, const Lbvh &t, hipStream_t st);
 // overflow: device int set to 1 when a traversal stack would exceed its LDS depth.
 void launch_dem_lbvh(const int *sortedIdx, const Lbvh &t, int n, const DevPhys &ph, const SceneView &sv,
                      bool sceneInVelW, const DemOut &o, int block, hipStream_t st, int *overflow);

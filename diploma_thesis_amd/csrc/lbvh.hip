@@ -114,16 +114,24 @@ __device__ __forceinline__ void stores_complete() {
 // AABB = c ± r, then walks up; the first arrival at a node stops, the second merges the children.
 __global__ void k_leaf_refit(const int *__restrict__ sIdx, const float4 *__restrict__ pos,
                              const float4 *__restrict__ vel, const int *__restrict__ rb,
-                             const uint16_t *__restrict__ scene, float r, int n, Lbvh t) {
+                             const uint16_t *__restrict__ scene, const float4 *__restrict__ rec, float r, int n,
+                             Lbvh t) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int o = sIdx[i];
-    const float4 p = pos[o];
-    t.cms[i] = make_float4(p.x, p.y, p.z, rb ? __int_as_float(rb[o]) : p.w);
-    if (t.sortedVel) {
-        float4 v = vel[o];
-        if (scene) v.w = __int_as_float((int)scene[o]);
-        t.sortedVel[i] = v;
+    float4 p;
+    if (rec) {   // the key kernel's gather record: (pos, rb bits), (vel, scene bits)
+        p = rec[2 * (size_t)o];
+        t.cms[i] = p;
+        if (t.sortedVel) t.sortedVel[i] = rec[2 * (size_t)o + 1];
+    } else {
+        p = pos[o];
+        t.cms[i] = make_float4(p.x, p.y, p.z, rb ? __int_as_float(rb[o]) : p.w);
+        if (t.sortedVel) {
+            float4 v = vel[o];
+            if (scene) v.w = __int_as_float((int)scene[o]);
+            t.sortedVel[i] = v;
+        }
     }
     store_bounds(t.bounds, (size_t)i, make_float4(p.x - r, p.y - r, p.z - r, 0.0f),
                  make_float4(p.x + r, p.y + r, p.z + r, 0.0f));
@@ -387,11 +395,11 @@ void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShi
 }
 
 void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *vel, const int *rb,
-                       const uint16_t *scene, float radius, int n, const Lbvh &t, hipStream_t st) {
+                       const uint16_t *scene, const float4 *rec, float radius, int n, const Lbvh &t, hipStream_t st) {
     if (n <= 0) return;
     const int block = 256;
     hipLaunchKernelGGL(k_leaf_refit, dim3((n + block - 1) / block), dim3(block), 0, st, sortedIdx, pos, vel, rb,
-                       scene, radius, n, t);
+                       scene, rec, radius, n, t);
 }
 
 void launch_dem_lbvh(const int *sortedIdx, const Lbvh &t, int n, const DevPhys &ph, const SceneView &sv,
