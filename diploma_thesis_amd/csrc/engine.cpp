@@ -347,6 +347,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     } else {
         size_t nodes = (size_t)2 * N;
         A(tree.left, nodes); A(tree.right, nodes); A(tree.parent, nodes);
+        A(tree.minR, nodes); A(tree.maxR, nodes);   // node ranges: the refit merges workgroup-local nodes in LDS
         A(tree.packed, 4 * (size_t)N);
         A(tree.bounds, 2 * nodes); A(tree.cms, N); A(tree.sortedVel, N); A(tree.counter, N);
         if (S > 1) A(sceneRootBuf, S);

@@ -112,33 +112,93 @@ __device__ __forceinline__ void stores_complete() {
 // kernelConstructLeafNodesSoA (BVHSoAKernels.cu:108-138) + kernelConstructInternalNodesSoA
 // (:148-176) in one launch: each leaf writes its centre (w = rbIndex bits), velocity copy and
 // AABB = c ± r, then walks up; the first arrival at a node stops, the second merges the children.
-__global__ void k_leaf_refit(const int *__restrict__ sIdx, const float4 *__restrict__ pos,
-                             const float4 *__restrict__ vel, const int *__restrict__ rb,
-                             const uint16_t *__restrict__ scene, const float4 *__restrict__ rec, float r, int n,
-                             Lbvh t) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int o = sIdx[i];
-    float4 p;
-    if (rec) {   // the key kernel's gather record: (pos, rb bits), (vel, scene bits)
-        p = rec[2 * (size_t)o];
-        t.cms[i] = p;
-        if (t.sortedVel) t.sortedVel[i] = rec[2 * (size_t)o + 1];
-    } else {
-        p = pos[o];
-        t.cms[i] = make_float4(p.x, p.y, p.z, rb ? __int_as_float(rb[o]) : p.w);
-        if (t.sortedVel) {
-            float4 v = vel[o];
-            if (scene) v.w = __int_as_float((int)scene[o]);
-            t.sortedVel[i] = v;
+// With the node ranges (minR / maxR), nodes whose leaves all belong to this workgroup's 256 leaves
+// are merged through LDS (counters and bounds in LDS, workgroup-scope ordering); only nodes whose
+// range leaves the workgroup use the device-wide counters and coherent bound loads.  A node's bounds
+// are the exact min / max of its leaves' either way.
+__device__ __forceinline__ float4 fmin4(float4 a, float4 b) {
+    return make_float4(a.x < b.x ? a.x : b.x, a.y < b.y ? a.y : b.y, a.z < b.z ? a.z : b.z, 0.0f);
+}
+__device__ __forceinline__ float4 fmax4(float4 a, float4 b) {
+    return make_float4(a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y, a.z > b.z ? a.z : b.z, 0.0f);
+}
+__device__ __forceinline__ void write_packed(const Lbvh &t, int n, int node, int L, int R, float4 aMin, float4 aMax,
+                                             float4 bMin, float4 bMax) {
+    if (!t.packed) return;
+    float4 *q = t.packed + 4 * (size_t)(node - n);
+    q[0] = make_float4(aMin.x, aMin.y, aMin.z, __int_as_float(L));
+    q[1] = make_float4(aMax.x, aMax.y, aMax.z, __int_as_float(R));
+    q[2] = make_float4(bMin.x, bMin.y, bMin.z, 0.0f);
+    q[3] = make_float4(bMax.x, bMax.y, bMax.z, 0.0f);
+}
+
+constexpr int kRefitBlock = 256;
+
+__global__ __launch_bounds__(kRefitBlock) void k_leaf_refit(const int *__restrict__ sIdx,
+                                                            const float4 *__restrict__ pos,
+                                                            const float4 *__restrict__ vel,
+                                                            const int *__restrict__ rb,
+                                                            const uint16_t *__restrict__ scene,
+                                                            const float4 *__restrict__ rec, float r, int n, Lbvh t) {
+    __shared__ float4 sLeaf[2 * kRefitBlock];   // leaf bounds of this workgroup
+    __shared__ float4 sNode[2 * kRefitBlock];   // bounds of internal nodes local to it
+    __shared__ int sCnt[kRefitBlock];
+    const int tid = (int)threadIdx.x;
+    const int b0 = (int)blockIdx.x * kRefitBlock;
+    const int b1 = b0 + kRefitBlock < n ? b0 + kRefitBlock : n;
+    const int i = b0 + tid;
+    const bool local = t.minR != nullptr;
+    if (local) sCnt[tid] = 0;
+    if (i < n) {
+        const int o = sIdx[i];
+        float4 p;
+        if (rec) {   // the key kernel's gather record: (pos, rb bits), (vel, scene bits)
+            p = rec[2 * (size_t)o];
+            t.cms[i] = p;
+            if (t.sortedVel) t.sortedVel[i] = rec[2 * (size_t)o + 1];
+        } else {
+            p = pos[o];
+            t.cms[i] = make_float4(p.x, p.y, p.z, rb ? __int_as_float(rb[o]) : p.w);
+            if (t.sortedVel) {
+                float4 v = vel[o];
+                if (scene) v.w = __int_as_float((int)scene[o]);
+                t.sortedVel[i] = v;
+            }
+        }
+        const float4 lmin = make_float4(p.x - r, p.y - r, p.z - r, 0.0f);
+        const float4 lmax = make_float4(p.x + r, p.y + r, p.z + r, 0.0f);
+        store_bounds(t.bounds, (size_t)i, lmin, lmax);
+        if (local) {
+            sLeaf[2 * tid] = lmin;
+            sLeaf[2 * tid + 1] = lmax;
+        }
+        if (t.isLeaf) t.isLeaf[i] = 1;
+        if (t.minR) { t.minR[i] = i; t.maxR[i] = i; }
+    }
+    if (local) __syncthreads();
+    if (i >= n || n < 2) return;
+    int node = t.parent[i];
+    if (local) {
+        for (;;) {
+            if (t.minR[node] < b0 || t.maxR[node] >= b1) break;   // leaves the workgroup
+            const int slot = node - n - b0;   // an internal node's index is an end of its range
+            const int prevL = __hip_atomic_fetch_add(&sCnt[slot], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prevL == 0) return;
+            const int L = t.left[node], R = t.right[node];
+            const float4 aMin = L < n ? sLeaf[2 * (L - b0)] : sNode[2 * (L - n - b0)];
+            const float4 aMax = L < n ? sLeaf[2 * (L - b0) + 1] : sNode[2 * (L - n - b0) + 1];
+            const float4 bMin = R < n ? sLeaf[2 * (R - b0)] : sNode[2 * (R - n - b0)];
+            const float4 bMax = R < n ? sLeaf[2 * (R - b0) + 1] : sNode[2 * (R - n - b0) + 1];
+            const float4 mn = fmin4(aMin, bMin), mx = fmax4(aMax, bMax);
+            sNode[2 * slot] = mn;
+            sNode[2 * slot + 1] = mx;
+            store_bounds(t.bounds, (size_t)node, mn, mx);
+            write_packed(t, n, node, L, R, aMin, aMax, bMin, bMax);
+            const int up = t.parent[node];
+            if (up < 0) return;
+            node = up;
         }
     }
-    store_bounds(t.bounds, (size_t)i, make_float4(p.x - r, p.y - r, p.z - r, 0.0f),
-                 make_float4(p.x + r, p.y + r, p.z + r, 0.0f));
-    if (t.isLeaf) t.isLeaf[i] = 1;
-    if (t.minR) { t.minR[i] = i; t.maxR[i] = i; }
-    if (n < 2) return;
-    int node = t.parent[i];
     stores_complete();
     int prev = __hip_atomic_fetch_add(&t.counter[node - n], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     while (prev == 1) {
@@ -146,18 +206,9 @@ __global__ void k_leaf_refit(const int *__restrict__ sIdx, const float4 *__restr
         float4 aMin, aMax, bMin, bMax;
         load_bounds(t.bounds, (size_t)L, aMin, aMax);
         load_bounds(t.bounds, (size_t)R, bMin, bMax);
-        float4 mn = make_float4(aMin.x < bMin.x ? aMin.x : bMin.x, aMin.y < bMin.y ? aMin.y : bMin.y,
-                                aMin.z < bMin.z ? aMin.z : bMin.z, 0.0f);
-        float4 mx = make_float4(aMax.x > bMax.x ? aMax.x : bMax.x, aMax.y > bMax.y ? aMax.y : bMax.y,
-                                aMax.z > bMax.z ? aMax.z : bMax.z, 0.0f);
+        const float4 mn = fmin4(aMin, bMin), mx = fmax4(aMax, bMax);
         store_bounds(t.bounds, (size_t)node, mn, mx);
-        if (t.packed) {
-            float4 *q = t.packed + 4 * (size_t)(node - n);
-            q[0] = make_float4(aMin.x, aMin.y, aMin.z, __int_as_float(L));
-            q[1] = make_float4(aMax.x, aMax.y, aMax.z, __int_as_float(R));
-            q[2] = make_float4(bMin.x, bMin.y, bMin.z, 0.0f);
-            q[3] = make_float4(bMax.x, bMax.y, bMax.z, 0.0f);
-        }
+        write_packed(t, n, node, L, R, aMin, aMax, bMin, bMax);
         const int up = t.parent[node];
         if (up < 0) break;
         node = up;
@@ -397,7 +448,7 @@ void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShi
 void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *vel, const int *rb,
                        const uint16_t *scene, const float4 *rec, float radius, int n, const Lbvh &t, hipStream_t st) {
     if (n <= 0) return;
-    const int block = 256;
+    const int block = kRefitBlock;
     hipLaunchKernelGGL(k_leaf_refit, dim3((n + block - 1) / block), dim3(block), 0, st, sortedIdx, pos, vel, rb,
                        scene, rec, radius, n, t);
 }
