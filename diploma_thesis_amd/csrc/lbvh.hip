@@ -132,73 +132,44 @@ __device__ __forceinline__ void write_packed(const Lbvh &t, int n, int node, int
     q[3] = make_float4(bMax.x, bMax.y, bMax.z, 0.0f);
 }
 
-constexpr int kRefitBlock = 256;
+// Leaf initialisation shared by both refit forms: centre (w = rbIndex bits), velocity copy, AABB.
+__device__ __forceinline__ void leaf_init(int i, const int *__restrict__ sIdx, const float4 *__restrict__ pos,
+                                          const float4 *__restrict__ vel, const int *__restrict__ rb,
+                                          const uint16_t *__restrict__ scene, const float4 *__restrict__ rec, float r,
+                                          const Lbvh &t, float4 &lmin, float4 &lmax) {
+    const int o = sIdx[i];
+    float4 p;
+    if (rec) {   // the key kernel's gather record: (pos, rb bits), (vel, scene bits)
+        p = rec[2 * (size_t)o];
+        t.cms[i] = p;
+        if (t.sortedVel) t.sortedVel[i] = rec[2 * (size_t)o + 1];
+    } else {
+        p = pos[o];
+        t.cms[i] = make_float4(p.x, p.y, p.z, rb ? __int_as_float(rb[o]) : p.w);
+        if (t.sortedVel) {
+            float4 v = vel[o];
+            if (scene) v.w = __int_as_float((int)scene[o]);
+            t.sortedVel[i] = v;
+        }
+    }
+    lmin = make_float4(p.x - r, p.y - r, p.z - r, 0.0f);
+    lmax = make_float4(p.x + r, p.y + r, p.z + r, 0.0f);
+    if (t.isLeaf) t.isLeaf[i] = 1;
+    if (t.minR) { t.minR[i] = i; t.maxR[i] = i; }
+}
 
-__global__ __launch_bounds__(kRefitBlock) void k_leaf_refit(const int *__restrict__ sIdx,
-                                                            const float4 *__restrict__ pos,
-                                                            const float4 *__restrict__ vel,
-                                                            const int *__restrict__ rb,
-                                                            const uint16_t *__restrict__ scene,
-                                                            const float4 *__restrict__ rec, float r, int n, Lbvh t) {
-    __shared__ float4 sLeaf[2 * kRefitBlock];   // leaf bounds of this workgroup
-    __shared__ float4 sNode[2 * kRefitBlock];   // bounds of internal nodes local to it
-    __shared__ int sCnt[kRefitBlock];
-    const int tid = (int)threadIdx.x;
-    const int b0 = (int)blockIdx.x * kRefitBlock;
-    const int b1 = b0 + kRefitBlock < n ? b0 + kRefitBlock : n;
-    const int i = b0 + tid;
-    const bool local = t.minR != nullptr;
-    if (local) sCnt[tid] = 0;
-    if (i < n) {
-        const int o = sIdx[i];
-        float4 p;
-        if (rec) {   // the key kernel's gather record: (pos, rb bits), (vel, scene bits)
-            p = rec[2 * (size_t)o];
-            t.cms[i] = p;
-            if (t.sortedVel) t.sortedVel[i] = rec[2 * (size_t)o + 1];
-        } else {
-            p = pos[o];
-            t.cms[i] = make_float4(p.x, p.y, p.z, rb ? __int_as_float(rb[o]) : p.w);
-            if (t.sortedVel) {
-                float4 v = vel[o];
-                if (scene) v.w = __int_as_float((int)scene[o]);
-                t.sortedVel[i] = v;
-            }
-        }
-        const float4 lmin = make_float4(p.x - r, p.y - r, p.z - r, 0.0f);
-        const float4 lmax = make_float4(p.x + r, p.y + r, p.z + r, 0.0f);
-        store_bounds(t.bounds, (size_t)i, lmin, lmax);
-        if (local) {
-            sLeaf[2 * tid] = lmin;
-            sLeaf[2 * tid + 1] = lmax;
-        }
-        if (t.isLeaf) t.isLeaf[i] = 1;
-        if (t.minR) { t.minR[i] = i; t.maxR[i] = i; }
-    }
-    if (local) __syncthreads();
-    if (i >= n || n < 2) return;
+// Without node ranges: every node by the device-wide protocol (coherent bounds, counters).
+__global__ __launch_bounds__(256) void k_leaf_refit(const int *__restrict__ sIdx, const float4 *__restrict__ pos,
+                                                    const float4 *__restrict__ vel, const int *__restrict__ rb,
+                                                    const uint16_t *__restrict__ scene,
+                                                    const float4 *__restrict__ rec, float r, int n, Lbvh t) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    float4 lmin, lmax;
+    leaf_init(i, sIdx, pos, vel, rb, scene, rec, r, t, lmin, lmax);
+    store_bounds(t.bounds, (size_t)i, lmin, lmax);
+    if (n < 2) return;
     int node = t.parent[i];
-    if (local) {
-        for (;;) {
-            if (t.minR[node] < b0 || t.maxR[node] >= b1) break;   // leaves the workgroup
-            const int slot = node - n - b0;   // an internal node's index is an end of its range
-            const int prevL = __hip_atomic_fetch_add(&sCnt[slot], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (prevL == 0) return;
-            const int L = t.left[node], R = t.right[node];
-            const float4 aMin = L < n ? sLeaf[2 * (L - b0)] : sNode[2 * (L - n - b0)];
-            const float4 aMax = L < n ? sLeaf[2 * (L - b0) + 1] : sNode[2 * (L - n - b0) + 1];
-            const float4 bMin = R < n ? sLeaf[2 * (R - b0)] : sNode[2 * (R - n - b0)];
-            const float4 bMax = R < n ? sLeaf[2 * (R - b0) + 1] : sNode[2 * (R - n - b0) + 1];
-            const float4 mn = fmin4(aMin, bMin), mx = fmax4(aMax, bMax);
-            sNode[2 * slot] = mn;
-            sNode[2 * slot + 1] = mx;
-            store_bounds(t.bounds, (size_t)node, mn, mx);
-            write_packed(t, n, node, L, R, aMin, aMax, bMin, bMax);
-            const int up = t.parent[node];
-            if (up < 0) return;
-            node = up;
-        }
-    }
     stores_complete();
     int prev = __hip_atomic_fetch_add(&t.counter[node - n], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     while (prev == 1) {
@@ -214,6 +185,101 @@ __global__ __launch_bounds__(kRefitBlock) void k_leaf_refit(const int *__restric
         node = up;
         stores_complete();
         prev = __hip_atomic_fetch_add(&t.counter[node - n], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// With node ranges, in two launches.
+//  1. k_refit_local: workgroups of kLocalLeaves consecutive leaves; a node whose leaves all lie in
+//     one workgroup ("local") is merged there through LDS (counters and bounds in LDS, workgroup-scope
+//     ordering); a chain stops at the first node whose range crosses a workgroup boundary.
+//  2. k_refit_top: one workgroup merges the remaining ("crossing") nodes.  A crossing node's split
+//     either falls on a workgroup boundary (one arrival from that boundary's thread) or not, and it
+//     waits for one arrival per crossing child; the last arrival merges it.  Its local children were
+//     completed by launch 1 (visible across the launch boundary), its crossing children by other
+//     threads of this workgroup (workgroup-scope release / acquire on the counter), so plain loads
+//     and stores suffice throughout.
+constexpr int kLocalLeaves = 1024;
+
+__device__ __forceinline__ bool crosses(const Lbvh &t, int node, int n) {
+    return node >= n && (t.minR[node] / kLocalLeaves) != (t.maxR[node] / kLocalLeaves);
+}
+__device__ __forceinline__ void store_plain(float4 *bounds, int node, float4 mn, float4 mx) {
+    bounds[2 * (size_t)node] = mn;
+    bounds[2 * (size_t)node + 1] = mx;
+}
+
+__global__ __launch_bounds__(kLocalLeaves) void k_refit_local(const int *__restrict__ sIdx,
+                                                              const float4 *__restrict__ pos,
+                                                              const float4 *__restrict__ vel,
+                                                              const int *__restrict__ rb,
+                                                              const uint16_t *__restrict__ scene,
+                                                              const float4 *__restrict__ rec, float r, int n,
+                                                              Lbvh t) {
+    __shared__ float4 sLeaf[2 * kLocalLeaves];   // leaf bounds of this workgroup
+    __shared__ float4 sNode[2 * kLocalLeaves];   // bounds of its local internal nodes
+    __shared__ int sCnt[kLocalLeaves];
+    const int tid = (int)threadIdx.x;
+    const int b0 = (int)blockIdx.x * kLocalLeaves;
+    const int i = b0 + tid;
+    sCnt[tid] = 0;
+    if (i < n) {
+        float4 lmin, lmax;
+        leaf_init(i, sIdx, pos, vel, rb, scene, rec, r, t, lmin, lmax);
+        store_plain(t.bounds, i, lmin, lmax);
+        sLeaf[2 * tid] = lmin;
+        sLeaf[2 * tid + 1] = lmax;
+    }
+    __syncthreads();
+    if (i >= n || n < 2) return;
+    int node = t.parent[i];
+    while (node >= 0 && !crosses(t, node, n)) {
+        const int slot = node - n - b0;   // an internal node's index is an end of its range
+        const int prev = __hip_atomic_fetch_add(&sCnt[slot], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (prev == 0) return;
+        const int L = t.left[node], R = t.right[node];
+        const float4 aMin = L < n ? sLeaf[2 * (L - b0)] : sNode[2 * (L - n - b0)];
+        const float4 aMax = L < n ? sLeaf[2 * (L - b0) + 1] : sNode[2 * (L - n - b0) + 1];
+        const float4 bMin = R < n ? sLeaf[2 * (R - b0)] : sNode[2 * (R - n - b0)];
+        const float4 bMax = R < n ? sLeaf[2 * (R - b0) + 1] : sNode[2 * (R - n - b0) + 1];
+        const float4 mn = fmin4(aMin, bMin), mx = fmax4(aMax, bMax);
+        sNode[2 * slot] = mn;
+        sNode[2 * slot + 1] = mx;
+        store_plain(t.bounds, node, mn, mx);
+        write_packed(t, n, node, L, R, aMin, aMax, bMin, bMax);
+        node = t.parent[node];
+    }
+}
+
+// Arrivals a crossing node waits for: one per crossing child, plus one when its split falls on a
+// workgroup boundary (the split is after leaf maxR[left child]).
+__device__ __forceinline__ int arrivals_needed(const Lbvh &t, int node, int n) {
+    const int L = t.left[node], R = t.right[node];
+    const int split = L < n ? L : t.maxR[L];
+    return (crosses(t, L, n) ? 1 : 0) + (crosses(t, R, n) ? 1 : 0) + ((split + 1) % kLocalLeaves == 0 ? 1 : 0);
+}
+
+__global__ __launch_bounds__(1024) void k_refit_top(int n, Lbvh t) {
+    const int boundaries = (n - 1) / kLocalLeaves;   // splits after leaf b * kLocalLeaves - 1, b = 1..
+    for (int b = 1 + (int)threadIdx.x; b <= boundaries; b += (int)blockDim.x) {
+        // the node that splits after leaf g has leaf g or internal node g as its left child
+        const int g = b * kLocalLeaves - 1;
+        int node = t.maxR[g + n] == g ? t.parent[g + n] : t.parent[g];
+        for (;;) {
+            const int need = arrivals_needed(t, node, n);
+            if (need > 1) {
+                const int prev =
+                    __hip_atomic_fetch_add(&t.counter[node - n], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (prev + 1 < need) break;
+            }
+            const int L = t.left[node], R = t.right[node];
+            const float4 aMin = t.bounds[2 * (size_t)L], aMax = t.bounds[2 * (size_t)L + 1];
+            const float4 bMin = t.bounds[2 * (size_t)R], bMax = t.bounds[2 * (size_t)R + 1];
+            store_plain(t.bounds, node, fmin4(aMin, bMin), fmax4(aMax, bMax));
+            write_packed(t, n, node, L, R, aMin, aMax, bMin, bMax);
+            const int up = t.parent[node];
+            if (up < 0) break;
+            node = up;
+        }
     }
 }
 
@@ -448,7 +514,13 @@ void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShi
 void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *vel, const int *rb,
                        const uint16_t *scene, const float4 *rec, float radius, int n, const Lbvh &t, hipStream_t st) {
     if (n <= 0) return;
-    const int block = kRefitBlock;
+    if (t.minR && t.maxR) {
+        hipLaunchKernelGGL(k_refit_local, dim3((n + kLocalLeaves - 1) / kLocalLeaves), dim3(kLocalLeaves), 0, st,
+                           sortedIdx, pos, vel, rb, scene, rec, radius, n, t);
+        if (n > kLocalLeaves) hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, n, t);
+        return;
+    }
+    const int block = 256;
     hipLaunchKernelGGL(k_leaf_refit, dim3((n + block - 1) / block), dim3(block), 0, st, sortedIdx, pos, vel, rb,
                        scene, rec, radius, n, t);
 }
