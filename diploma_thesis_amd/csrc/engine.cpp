@@ -347,7 +347,10 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     } else {
         size_t nodes = (size_t)2 * N;
         A(tree.left, nodes); A(tree.right, nodes); A(tree.parent, nodes);
-        A(tree.minR, nodes); A(tree.maxR, nodes);   // node ranges: the refit merges workgroup-local nodes in LDS
+        // range refit: node records, per-leaf workgroup prefix / suffix boxes, table of workgroup boxes
+        A(tree.meta, (size_t)N); A(tree.pre, 2 * (size_t)N); A(tree.suf, 2 * (size_t)N);
+        tree.blockTabLevels = lbvh_refit_levels(N);
+        A(tree.blockTab, 2 * (size_t)tree.blockTabLevels * (size_t)((N + lbvh_refit_block() - 1) / lbvh_refit_block()));
         A(tree.packed, 4 * (size_t)N);
         A(tree.bounds, 2 * nodes); A(tree.cms, N); A(tree.sortedVel, N); A(tree.counter, N);
         if (S > 1) A(sceneRootBuf, S);

@@ -146,7 +146,16 @@ struct Lbvh {
     // optional traversal layout: internal node k = node - N as 4 float4
     // {Lmin.xyz, L}, {Lmax.xyz, R}, {Rmin.xyz, 0}, {Rmax.xyz, 0} (child ids as int bits)
     float4 *packed;
+    // optional (engine): internal node k as (left, right, minR, maxR), and the buffers of the range
+    // refit: per leaf the AABB of the leaves from its refit workgroup's start (pre) and to its end
+    // (suf), 2 float4 each, and a sparse table of workgroup AABBs (levels x numBlocks x 2 float4)
+    int4 *meta;
+    float4 *pre, *suf, *blockTab;
+    int blockTabLevels;
 };
+// Workgroup (leaf block) size of the range refit and the sizes of its buffers for n leaves.
+int lbvh_refit_block();
+int lbvh_refit_levels(int n);
 // keyBytes 4 or 8; sceneShift > 0: the scene id is key >> sceneShift and equal-key ties use the
 // scene-local leaf index (sceneStart[scene] is subtracted).
 void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShift, const int *sceneStart,

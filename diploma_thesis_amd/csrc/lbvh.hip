@@ -70,6 +70,7 @@ __global__ void k_karras(const K *__restrict__ codes, int n, int sceneShift, con
     t.parent[L] = node;
     t.parent[R] = node;
     if (t.minR) { t.minR[node] = lo; t.maxR[node] = hi; }
+    if (t.meta) t.meta[i] = make_int4(L, R, lo, hi);
     if (t.isLeaf) t.isLeaf[node] = 0;
     if (t.sceneRoot && sceneShift > 0) {
         int sc = (int)(codes[lo] >> sceneShift);
@@ -188,55 +189,91 @@ __global__ __launch_bounds__(256) void k_leaf_refit(const int *__restrict__ sIdx
     }
 }
 
-// With node ranges, in two launches.
-//  1. k_refit_local: workgroups of kLocalLeaves consecutive leaves; a node whose leaves all lie in
-//     one workgroup ("local") is merged there through LDS (counters and bounds in LDS, workgroup-scope
-//     ordering); a chain stops at the first node whose range crosses a workgroup boundary.
-//  2. k_refit_top: one workgroup merges the remaining ("crossing") nodes.  A crossing node's split
-//     either falls on a workgroup boundary (one arrival from that boundary's thread) or not, and it
-//     waits for one arrival per crossing child; the last arrival merges it.  Its local children were
-//     completed by launch 1 (visible across the launch boundary), its crossing children by other
-//     threads of this workgroup (workgroup-scope release / acquire on the counter), so plain loads
-//     and stores suffice throughout.
-constexpr int kLocalLeaves = 1024;
+// Range refit (engine), three launches and no device-wide synchronisation.  A node's AABB is the
+// min / max over the leaves of its range [minR, maxR], whatever the order of the merges, so:
+//  1. k_refit_local, one workgroup per kBlk consecutive leaves: leaf initialisation; inclusive
+//     prefix (pre) and suffix (suf) min / max of the leaf boxes within the workgroup; the
+//     workgroup's box (level 0 of the table); and the bottom-up merge, through LDS counters and
+//     bounds, of every node whose range lies inside the workgroup ("local").
+//  2. k_refit_table, one workgroup: a sparse table of the workgroup boxes (level l holds the box of
+//     2^l consecutive workgroups).
+//  3. k_refit_cross, one thread per internal node: a node whose range crosses a workgroup boundary
+//     takes suf[lo] ∪ pre[hi] ∪ (the full workgroups between, two table lookups), and the same for
+//     its crossing children, for its packed traversal record.
+constexpr int kBlk = 1024;
 
-__device__ __forceinline__ bool crosses(const Lbvh &t, int node, int n) {
-    return node >= n && (t.minR[node] / kLocalLeaves) != (t.maxR[node] / kLocalLeaves);
-}
 __device__ __forceinline__ void store_plain(float4 *bounds, int node, float4 mn, float4 mx) {
     bounds[2 * (size_t)node] = mn;
     bounds[2 * (size_t)node + 1] = mx;
 }
 
-__global__ __launch_bounds__(kLocalLeaves) void k_refit_local(const int *__restrict__ sIdx,
-                                                              const float4 *__restrict__ pos,
-                                                              const float4 *__restrict__ vel,
-                                                              const int *__restrict__ rb,
-                                                              const uint16_t *__restrict__ scene,
-                                                              const float4 *__restrict__ rec, float r, int n,
-                                                              Lbvh t) {
-    __shared__ float4 sLeaf[2 * kLocalLeaves];   // leaf bounds of this workgroup
-    __shared__ float4 sNode[2 * kLocalLeaves];   // bounds of its local internal nodes
-    __shared__ int sCnt[kLocalLeaves];
-    const int tid = (int)threadIdx.x;
-    const int b0 = (int)blockIdx.x * kLocalLeaves;
+__device__ __forceinline__ bool range_crosses(int lo, int hi) { return lo / kBlk != hi / kBlk; }
+
+__device__ __forceinline__ void shfl_minmax(float4 &mn, float4 &mx, int d, bool up) {
+    float v[6] = {mn.x, mn.y, mn.z, mx.x, mx.y, mx.z};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) v[c] = up ? __shfl_up(v[c], d) : __shfl_down(v[c], d);
+    mn = make_float4(v[0], v[1], v[2], 0.0f);
+    mx = make_float4(v[3], v[4], v[5], 0.0f);
+}
+
+__global__ __launch_bounds__(kBlk) void k_refit_local(const int *__restrict__ sIdx, const float4 *__restrict__ pos,
+                                                      const float4 *__restrict__ vel, const int *__restrict__ rb,
+                                                      const uint16_t *__restrict__ scene,
+                                                      const float4 *__restrict__ rec, float r, int n, Lbvh t) {
+    __shared__ float4 sLeaf[2 * kBlk];   // leaf boxes of this workgroup
+    __shared__ float4 sNode[2 * kBlk];   // boxes of its local internal nodes
+    __shared__ int sCnt[kBlk];
+    __shared__ float4 sWave[4 * (kBlk / 64)];   // per wave: prefix total min / max, suffix total min / max
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b0 = (int)blockIdx.x * kBlk;
     const int i = b0 + tid;
+    const bool valid = i < n;
     sCnt[tid] = 0;
-    if (i < n) {
-        float4 lmin, lmax;
+    const float inf = __int_as_float(0x7f800000);
+    float4 lmin = make_float4(inf, inf, inf, 0.0f), lmax = make_float4(-inf, -inf, -inf, 0.0f);
+    if (valid) {
         leaf_init(i, sIdx, pos, vel, rb, scene, rec, r, t, lmin, lmax);
         store_plain(t.bounds, i, lmin, lmax);
         sLeaf[2 * tid] = lmin;
         sLeaf[2 * tid + 1] = lmax;
     }
+    // inclusive prefix and suffix min / max within the wave
+    float4 pmn = lmin, pmx = lmax, smn = lmin, smx = lmax;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        float4 a = pmn, b = pmx;
+        shfl_minmax(a, b, d, true);
+        if (lane >= d) { pmn = fmin4(pmn, a); pmx = fmax4(pmx, b); }
+        float4 c = smn, e = smx;
+        shfl_minmax(c, e, d, false);
+        if (lane + d < 64) { smn = fmin4(smn, c); smx = fmax4(smx, e); }
+    }
+    if (lane == 63) { sWave[4 * wave] = pmn; sWave[4 * wave + 1] = pmx; }
+    if (lane == 0) { sWave[4 * wave + 2] = smn; sWave[4 * wave + 3] = smx; }
     __syncthreads();
-    if (i >= n || n < 2) return;
+    const int nw = kBlk / 64;
+    for (int w = 0; w < nw; ++w) {   // the waves before this one for the prefix, after it for the suffix
+        if (w < wave) { pmn = fmin4(pmn, sWave[4 * w]); pmx = fmax4(pmx, sWave[4 * w + 1]); }
+        if (w > wave) { smn = fmin4(smn, sWave[4 * w + 2]); smx = fmax4(smx, sWave[4 * w + 3]); }
+    }
+    if (valid) {
+        t.pre[2 * (size_t)i] = pmn; t.pre[2 * (size_t)i + 1] = pmx;
+        t.suf[2 * (size_t)i] = smn; t.suf[2 * (size_t)i + 1] = smx;
+    }
+    if (tid == kBlk - 1) {   // the whole workgroup's box: the last thread's prefix
+        t.blockTab[2 * (size_t)blockIdx.x] = pmn;
+        t.blockTab[2 * (size_t)blockIdx.x + 1] = pmx;
+    }
+    if (!valid || n < 2) return;
     int node = t.parent[i];
-    while (node >= 0 && !crosses(t, node, n)) {
+    while (node >= 0) {
+        const int4 m = t.meta[node - n];   // left, right, lo, hi
+        if (range_crosses(m.z, m.w)) break;
         const int slot = node - n - b0;   // an internal node's index is an end of its range
         const int prev = __hip_atomic_fetch_add(&sCnt[slot], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (prev == 0) return;
-        const int L = t.left[node], R = t.right[node];
+        const int L = m.x, R = m.y;
         const float4 aMin = L < n ? sLeaf[2 * (L - b0)] : sNode[2 * (L - n - b0)];
         const float4 aMax = L < n ? sLeaf[2 * (L - b0) + 1] : sNode[2 * (L - n - b0) + 1];
         const float4 bMin = R < n ? sLeaf[2 * (R - b0)] : sNode[2 * (R - n - b0)];
@@ -250,37 +287,58 @@ __global__ __launch_bounds__(kLocalLeaves) void k_refit_local(const int *__restr
     }
 }
 
-// Arrivals a crossing node waits for: one per crossing child, plus one when its split falls on a
-// workgroup boundary (the split is after leaf maxR[left child]).
-__device__ __forceinline__ int arrivals_needed(const Lbvh &t, int node, int n) {
-    const int L = t.left[node], R = t.right[node];
-    const int split = L < n ? L : t.maxR[L];
-    return (crosses(t, L, n) ? 1 : 0) + (crosses(t, R, n) ? 1 : 0) + ((split + 1) % kLocalLeaves == 0 ? 1 : 0);
+// Level l, entry b: box of workgroups [b, b + 2^l).  One workgroup builds the levels in order.
+__global__ __launch_bounds__(1024) void k_refit_table(int numBlocks, int levels, float4 *__restrict__ tab) {
+    for (int l = 1; l < levels; ++l) {
+        const float4 *src = tab + 2 * (size_t)(l - 1) * numBlocks;
+        float4 *dst = tab + 2 * (size_t)l * numBlocks;
+        const int half = 1 << (l - 1), cnt = numBlocks - (1 << l) + 1;
+        for (int b = (int)threadIdx.x; b < cnt; b += (int)blockDim.x) {
+            dst[2 * b] = fmin4(src[2 * b], src[2 * (b + half)]);
+            dst[2 * b + 1] = fmax4(src[2 * b + 1], src[2 * (b + half) + 1]);
+        }
+        __syncthreads();
+    }
 }
 
-__global__ __launch_bounds__(1024) void k_refit_top(int n, Lbvh t) {
-    const int boundaries = (n - 1) / kLocalLeaves;   // splits after leaf b * kLocalLeaves - 1, b = 1..
-    for (int b = 1 + (int)threadIdx.x; b <= boundaries; b += (int)blockDim.x) {
-        // the node that splits after leaf g has leaf g or internal node g as its left child
-        const int g = b * kLocalLeaves - 1;
-        int node = t.maxR[g + n] == g ? t.parent[g + n] : t.parent[g];
-        for (;;) {
-            const int need = arrivals_needed(t, node, n);
-            if (need > 1) {
-                const int prev =
-                    __hip_atomic_fetch_add(&t.counter[node - n], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (prev + 1 < need) break;
-            }
-            const int L = t.left[node], R = t.right[node];
-            const float4 aMin = t.bounds[2 * (size_t)L], aMax = t.bounds[2 * (size_t)L + 1];
-            const float4 bMin = t.bounds[2 * (size_t)R], bMax = t.bounds[2 * (size_t)R + 1];
-            store_plain(t.bounds, node, fmin4(aMin, bMin), fmax4(aMax, bMax));
-            write_packed(t, n, node, L, R, aMin, aMax, bMin, bMax);
-            const int up = t.parent[node];
-            if (up < 0) break;
-            node = up;
+// Box of the leaves [lo, hi] that span at least two workgroups.
+__device__ __forceinline__ void range_box(const Lbvh &t, int numBlocks, int lo, int hi, float4 &mn, float4 &mx) {
+    mn = fmin4(t.suf[2 * (size_t)lo], t.pre[2 * (size_t)hi]);
+    mx = fmax4(t.suf[2 * (size_t)lo + 1], t.pre[2 * (size_t)hi + 1]);
+    const int bl = lo / kBlk + 1, bh = hi / kBlk - 1;
+    if (bl <= bh) {
+        const int len = bh - bl + 1;
+        const int l = 31 - __clz(len);
+        const float4 *lev = t.blockTab + 2 * (size_t)l * numBlocks;
+        const int b2 = bh - (1 << l) + 1;
+        mn = fmin4(mn, fmin4(lev[2 * bl], lev[2 * b2]));
+        mx = fmax4(mx, fmax4(lev[2 * bl + 1], lev[2 * b2 + 1]));
+    }
+}
+
+__device__ __forceinline__ void child_box(const Lbvh &t, int n, int numBlocks, int c, float4 &mn, float4 &mx) {
+    if (c >= n) {
+        const int4 m = t.meta[c - n];
+        if (range_crosses(m.z, m.w)) {
+            range_box(t, numBlocks, m.z, m.w, mn, mx);
+            return;
         }
     }
+    mn = t.bounds[2 * (size_t)c];
+    mx = t.bounds[2 * (size_t)c + 1];
+}
+
+__global__ void k_refit_cross(int n, int numBlocks, Lbvh t) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= n - 1) return;
+    const int4 m = t.meta[k];
+    if (!range_crosses(m.z, m.w)) return;
+    float4 mn, mx, aMin, aMax, bMin, bMax;
+    range_box(t, numBlocks, m.z, m.w, mn, mx);
+    child_box(t, n, numBlocks, m.x, aMin, aMax);
+    child_box(t, n, numBlocks, m.y, bMin, bMax);
+    store_plain(t.bounds, k + n, mn, mx);
+    write_packed(t, n, k + n, m.x, m.y, aMin, aMax, bMin, bMax);
 }
 
 __device__ __forceinline__ bool overlap(float4 aMin, float4 aMax, float4 bMin, float4 bMax) {
@@ -496,6 +554,14 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
 
 }  // namespace
 
+int lbvh_refit_block() { return kBlk; }
+int lbvh_refit_levels(int n) {
+    const int numBlocks = (n + kBlk - 1) / kBlk;
+    int l = 1;
+    while ((1 << l) <= numBlocks) ++l;
+    return l;
+}
+
 void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShift, const int *sceneStart,
                        const int *sceneCount, int numScenes, const Lbvh &t, hipStream_t st) {
     (void)numScenes;
@@ -514,10 +580,14 @@ void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShi
 void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *vel, const int *rb,
                        const uint16_t *scene, const float4 *rec, float radius, int n, const Lbvh &t, hipStream_t st) {
     if (n <= 0) return;
-    if (t.minR && t.maxR) {
-        hipLaunchKernelGGL(k_refit_local, dim3((n + kLocalLeaves - 1) / kLocalLeaves), dim3(kLocalLeaves), 0, st,
-                           sortedIdx, pos, vel, rb, scene, rec, radius, n, t);
-        if (n > kLocalLeaves) hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, n, t);
+    if (t.meta && t.pre && t.suf && t.blockTab) {
+        const int numBlocks = (n + kBlk - 1) / kBlk;
+        hipLaunchKernelGGL(k_refit_local, dim3(numBlocks), dim3(kBlk), 0, st, sortedIdx, pos, vel, rb, scene, rec,
+                           radius, n, t);
+        if (numBlocks > 1) {
+            hipLaunchKernelGGL(k_refit_table, dim3(1), dim3(1024), 0, st, numBlocks, lbvh_refit_levels(n), t.blockTab);
+            hipLaunchKernelGGL(k_refit_cross, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, numBlocks, t);
+        }
         return;
     }
     const int block = 256;
