@@ -225,17 +225,11 @@ __global__ __launch_bounds__(kBlk) void k_refit_local(const int *__restrict__ sI
     __shared__ float4 sNode[2 * kBlk];   // boxes of its local internal nodes
     __shared__ int sCnt[kBlk];
     __shared__ float4 sWave[4 * (kBlk / 64)];   // per wave: prefix total min / max, suffix total min / max
-    __shared__ int4 sMeta[kBlk];                 // records of the internal nodes b0 .. b0 + kBlk - 1
-    __shared__ int sPar[kBlk];                   // and their parents
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b0 = (int)blockIdx.x * kBlk;
     const int i = b0 + tid;
     const bool valid = i < n;
     sCnt[tid] = 0;
-    if (i < n - 1) {   // a local node's index lies in its range, so in this workgroup's index range
-        sMeta[tid] = t.meta[i];
-        sPar[tid] = t.parent[n + i];
-    }
     const float inf = __int_as_float(0x7f800000);
     float4 lmin = make_float4(inf, inf, inf, 0.0f), lmax = make_float4(-inf, -inf, -inf, 0.0f);
     if (valid) {
@@ -274,10 +268,9 @@ __global__ __launch_bounds__(kBlk) void k_refit_local(const int *__restrict__ sI
     if (!valid || n < 2) return;
     int node = t.parent[i];
     while (node >= 0) {
-        const int slot = node - n - b0;   // an internal node's index is an end of its range
-        if (slot < 0 || slot >= kBlk) break;   // outside the workgroup's indices: not local
-        const int4 m = sMeta[slot];   // left, right, lo, hi
+        const int4 m = t.meta[node - n];   // left, right, lo, hi
         if (range_crosses(m.z, m.w)) break;
+        const int slot = node - n - b0;   // an internal node's index is an end of its range
         const int prev = __hip_atomic_fetch_add(&sCnt[slot], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (prev == 0) return;
         const int L = m.x, R = m.y;
@@ -290,7 +283,7 @@ __global__ __launch_bounds__(kBlk) void k_refit_local(const int *__restrict__ sI
         sNode[2 * slot + 1] = mx;
         store_plain(t.bounds, node, mn, mx);
         write_packed(t, n, node, L, R, aMin, aMax, bMin, bMax);
-        node = sPar[slot];
+        node = t.parent[node];
     }
 }
 
