@@ -342,9 +342,10 @@ __global__ void k_refit_cross(int n, int numBlocks, Lbvh t) {
 }
 
 __device__ __forceinline__ bool overlap(float4 aMin, float4 aMax, float4 bMin, float4 bMax) {
-    // checkOverlap(AABB, AABB) (BVHAuxiliary.cuh:139-152)
-    return (aMin.x <= bMax.x && aMax.x >= bMin.x) && (aMin.y <= bMax.y && aMax.y >= bMin.y) &&
-           (aMin.z <= bMax.z && aMax.z >= bMin.z);
+    // checkOverlap(AABB, AABB) (BVHAuxiliary.cuh:139-152); evaluated without short-circuiting, so
+    // that a node's boxes are needed (and loaded) together
+    return (aMin.x <= bMax.x) & (aMax.x >= bMin.x) & (aMin.y <= bMax.y) & (aMax.y >= bMin.y) &
+           (aMin.z <= bMax.z) & (aMax.z >= bMin.z);
 }
 
 // collideBVHSoARigidBody (Kernels/RB_BVHKernels.cu:100-244): per sorted leaf, traversal from the
@@ -486,8 +487,8 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
             const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3];
             const int L = __builtin_amdgcn_readfirstlane(__float_as_int(a0.w));
             const int R = __builtin_amdgcn_readfirstlane(__float_as_int(a1.w));
-            const bool oL = act && L != i && overlap(qMin, qMax, a0, a1);
-            const bool oR = act && R != i && overlap(qMin, qMax, b0, b1);
+            const bool oL = act & (L != i) & overlap(qMin, qMax, a0, a1);
+            const bool oR = act & (R != i) & overlap(qMin, qMax, b0, b1);
             if (L < n && oL) {
                 const float4 CL = cms[L];
                 if (candidate(CL, pa, ra, false, ph)) {
