@@ -351,7 +351,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         A(tree.meta, (size_t)N); A(tree.pre, 2 * (size_t)N); A(tree.suf, 2 * (size_t)N);
         tree.blockTabLevels = lbvh_refit_levels(N);
         A(tree.blockTab, 2 * (size_t)tree.blockTabLevels * (size_t)((N + lbvh_refit_block() - 1) / lbvh_refit_block()));
-        A(tree.packed, 6 * (size_t)N);   // 96-byte traversal record per internal node
+        A(tree.packed, 4 * (size_t)N);
         A(tree.bounds, 2 * nodes); A(tree.cms, N); A(tree.sortedVel, N); A(tree.counter, N);
         if (S > 1) A(sceneRootBuf, S);
         tree.sceneRoot = S > 1 ? sceneRootBuf : nullptr;

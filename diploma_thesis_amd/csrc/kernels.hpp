@@ -143,9 +143,8 @@ struct Lbvh {
     float4 *sortedVel;            // leaf velocities, w = scene id bits when batched
     int *counter;                 // N - 1 refit counters
     int *sceneRoot;               // per scene (batched) or NULL
-    // optional traversal layout: internal node k = node - N as 6 float4
-    // {Lmin.xyz, L}, {Lmax.xyz, R}, {Rmin.xyz, 0}, {Rmax.xyz, 0} (child ids as int bits), then the
-    // CMs entry of a leaf left / right child
+    // optional traversal layout: internal node k = node - N as 4 float4
+    // {Lmin.xyz, L}, {Lmax.xyz, R}, {Rmin.xyz, 0}, {Rmax.xyz, 0} (child ids as int bits)
     float4 *packed;
     // optional (engine): internal node k as (left, right, minR, maxR), and the buffers of the range
     // refit: per leaf the AABB of the leaves from its refit workgroup's start (pre) and to its end

@@ -123,21 +123,14 @@ __device__ __forceinline__ float4 fmin4(float4 a, float4 b) {
 __device__ __forceinline__ float4 fmax4(float4 a, float4 b) {
     return make_float4(a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y, a.z > b.z ? a.z : b.z, 0.0f);
 }
-// Traversal record of internal node `node` (kPackedF4 float4): {Lmin.xyz, L}, {Lmax.xyz, R},
-// {Rmin.xyz, 0}, {Rmax.xyz, 0}, then for a leaf child its centre and rbIndex bits (the leaf's CMs
-// entry) in slot 4 (left) / 5 (right), so that the traversal tests a leaf without another load.
-// The leaves' CMs entries must be visible to the writer (written before a barrier or a launch).
-constexpr int kPackedF4 = 6;
 __device__ __forceinline__ void write_packed(const Lbvh &t, int n, int node, int L, int R, float4 aMin, float4 aMax,
                                              float4 bMin, float4 bMax) {
     if (!t.packed) return;
-    float4 *q = t.packed + kPackedF4 * (size_t)(node - n);
+    float4 *q = t.packed + 4 * (size_t)(node - n);
     q[0] = make_float4(aMin.x, aMin.y, aMin.z, __int_as_float(L));
     q[1] = make_float4(aMax.x, aMax.y, aMax.z, __int_as_float(R));
     q[2] = make_float4(bMin.x, bMin.y, bMin.z, 0.0f);
     q[3] = make_float4(bMax.x, bMax.y, bMax.z, 0.0f);
-    if (L < n) q[4] = t.cms[L];
-    if (R < n) q[5] = t.cms[R];
 }
 
 // Leaf initialisation shared by both refit forms: centre (w = rbIndex bits), velocity copy, AABB.
@@ -379,10 +372,20 @@ __global__ void k_dem_lbvh(const int *__restrict__ sIdx, Lbvh t, int n, DevPhys 
     while (node >= n) {
         int L, R;
         bool oL, oR;
-        L = t.left[node];
-        R = t.right[node];
-        oL = L != i && overlap(qMin, qMax, t.bounds[2 * (size_t)L], t.bounds[2 * (size_t)L + 1]);
-        oR = R != i && overlap(qMin, qMax, t.bounds[2 * (size_t)R], t.bounds[2 * (size_t)R + 1]);
+        if (t.packed) {
+            // one 64-byte node: both children's boxes and ids
+            const float4 *q = t.packed + 4 * (size_t)(node - n);
+            const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3];
+            L = __float_as_int(a0.w);
+            R = __float_as_int(a1.w);
+            oL = L != i && overlap(qMin, qMax, a0, a1);
+            oR = R != i && overlap(qMin, qMax, b0, b1);
+        } else {
+            L = t.left[node];
+            R = t.right[node];
+            oL = L != i && overlap(qMin, qMax, t.bounds[2 * (size_t)L], t.bounds[2 * (size_t)L + 1]);
+            oR = R != i && overlap(qMin, qMax, t.bounds[2 * (size_t)R], t.bounds[2 * (size_t)R + 1]);
+        }
         const bool lL = L < n, lR = R < n;
         if (oL && lL) {
             float4 CL = t.cms[L];
@@ -480,17 +483,14 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
         pending &= ~__ballot(act);
         int node = root, sp = 0;
         while (true) {
-            const float4 *q = packed + kPackedF4 * (size_t)(node - n);
-            const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3], lc = q[4], rc = q[5];
-            // keeps the whole record in one round of loads (the leaf slots would otherwise be sunk
-            // into the leaf branches as separate loads)
-            asm volatile("" ::"s"(lc.x), "s"(lc.y), "s"(lc.z), "s"(lc.w), "s"(rc.x), "s"(rc.y), "s"(rc.z), "s"(rc.w));
+            const float4 *q = packed + 4 * (size_t)(node - n);
+            const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3];
             const int L = __builtin_amdgcn_readfirstlane(__float_as_int(a0.w));
             const int R = __builtin_amdgcn_readfirstlane(__float_as_int(a1.w));
             const bool oL = act & (L != i) & overlap(qMin, qMax, a0, a1);
             const bool oR = act & (R != i) & overlap(qMin, qMax, b0, b1);
             if (L < n && oL) {
-                const float4 CL = lc;   // the leaf's CMs entry, from the node record
+                const float4 CL = cms[L];
                 if (candidate(CL, pa, ra, false, ph)) {
                     if (EMIT) {
                         if (in_contact(pa, xyz(CL), ph)) {
@@ -504,7 +504,7 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
                 }
             }
             if (R < n && oR) {
-                const float4 CR = rc;
+                const float4 CR = cms[R];
                 if (candidate(CR, pa, ra, false, ph)) {
                     if (EMIT) {
                         if (in_contact(pa, xyz(CR), ph)) {
