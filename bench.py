@@ -50,7 +50,32 @@ def parse():
                     help="run a plain 256 MiB device copy kernel (3 launches) to calibrate HBM byte counters, then exit")
     ap.add_argument("--batched-only", action="store_true",
                     help="only step the 512-scene batch (default tuning, no autotune): for profilers")
+    ap.add_argument("--scene", choices=["small", "pile"], default=None,
+                    help="time another plain scene instead of the headline one: 'small' = 256 cubes (16,384 "
+                         "particles, 64^3 grid), 'pile' = the dense ellipsoid pile (~4.2 M particles)")
     return ap.parse_args()
+
+
+def other_scene(args):
+    """Plain-path throughput of the small scene or the dense pile (stage times included)."""
+    from diploma_thesis_amd import Engine, capi, scenes
+    method = {"grid": capi.METHOD_GRID, "lbvh30": capi.METHOD_LBVH30, "lbvh63": capi.METHOD_LBVH63}[args.method]
+    sc = scenes.config0(method) if args.scene == "small" else scenes.config2(method)
+    with Engine(sc, use_graph=True) as e:
+        e.step(args.warmup)
+        e.sync()
+        t0 = time.perf_counter()
+        e.step(args.steps)
+        e.sync()
+        sec = time.perf_counter() - t0
+        stage_ms = {}
+        for _ in range(3):
+            for name, ms in e.profile():
+                stage_ms[name] = stage_ms.get(name, 0.0) + ms / 3
+        finite = e.check_finite()
+    print(json.dumps({"scene": args.scene, "method": args.method, "particles": sc.n, "bodies": sc.b,
+                      "steps": args.steps, "updates_per_s": sc.n * args.steps / sec,
+                      "ms_per_step": sec * 1e3 / args.steps, "stage_ms": stage_ms, "finite": finite}))
 
 
 def batched_only(args):
@@ -105,6 +130,9 @@ def main():
         return
     if args.batched_only:
         batched_only(args)
+        return
+    if args.scene:
+        other_scene(args)
         return
     from diploma_thesis_amd import Engine, capi, scenes
 
