@@ -23,12 +23,17 @@ sys.path.insert(0, ROOT)
 
 # algorithmic bytes per particle (or per body) of each stage, compulsory traffic in the reference
 # layout with neighbour re-reads assumed cached
+# Algorithmic HBM bytes per launch = per_particle * N + per_cell * C (compulsory traffic of the reference
+# layout, neighbour re-reads assumed cached; C = grid cells)
 ALGO_BYTES = {
-    "dem_grid": 120.0,          # sorted pos+vel 32, index 4, rbIndex 4, relativePos 16, pForce/pTorque RMW 64
-    "dem_lbvh": 120.0,
-    "reorder_cellstart": 72.0,  # hash 4, index 4, gathered pos+vel 32, sorted pos+vel 32
-    "pose_key": 68.0 + 24.0,    # body integrate + pose 68, hash 24
-    "reduce_apply": 32.0,
+    "dem_grid": (120.0, 8.0),           # sorted pos+vel 32, index 4, rbIndex 4, relativePos 16, pForce/pTorque RMW 64;
+                                        # cellStart + cellEnd 8 per cell
+    "dem_lbvh": (120.0, 0.0),
+    "reorder_cellstart": (72.0, 4.0),   # hash 4, index 4, gathered pos+vel 32, sorted pos+vel 32; cellStart clear 4 per
+                                        # cell (the 8 B per occupied cell are left out)
+    "pose_key": (68.0 + 24.0, 0.0),     # body integrate + pose 68, hash 24
+    "sort": (4.0 + 16.0 * 3, 0.0),      # onesweep, u32 key + u32 value, 3 digit passes over the 24-bit grid key
+    "reduce_apply": (32.0, 0.0),
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -182,8 +187,13 @@ def main():
         for name, ms in eng.profile():
             stage_ms[name] = stage_ms.get(name, 0.0) + ms / reps
     dom = max((k for k in stage_ms if k != "sort"), key=lambda k: stage_ms[k])
-    per_unit = ALGO_BYTES.get(dom, 0.0)
-    achieved = per_unit * n / (stage_ms[dom] * 1e-3) / 1e9
+    cells = int(sc.params.numCells)
+
+    def algo_bytes(stage):
+        per_p, per_c = ALGO_BYTES.get(stage, (0.0, 0.0))
+        return per_p * n + per_c * cells
+
+    achieved = algo_bytes(dom) / (stage_ms[dom] * 1e-3) / 1e9
     traffic = None
     pmc_path = os.path.join(ROOT, "profiles", f"{dom}_c1_pmc.json")
     if os.path.exists(pmc_path):
@@ -191,7 +201,9 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_particle": per_unit, "kernel_ms": round(stage_ms[dom], 5)}
+                "algorithmic_bytes": int(algo_bytes(dom)), "kernel_ms": round(stage_ms[dom], 5),
+                "stage_frac": {k: round(algo_bytes(k) / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                               for k, v in stage_ms.items() if k in ALGO_BYTES and args.method == "grid"}}
     stage_summary = {k: round(v, 5) for k, v in stage_ms.items()}
     eng.close()
 
