@@ -27,6 +27,7 @@ ARSIM_ERR_UNSUPPORTED = 10
 METHOD_GRID, METHOD_LBVH30, METHOD_LBVH63 = 0, 1, 2
 WALLS_OPEN_TOP = 0x37
 COMPAT_REF_UNITS, COMPAT_DROP_FIRST, COMPAT_GRAIN_CLAMP, DYNAMIC_AABB = 0x1, 0x2, 0x4, 0x8
+CONTACT_SIS = 0x10
 
 
 class Float3(C.Structure):

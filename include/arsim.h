@@ -104,6 +104,8 @@ typedef struct ArSimParams {
 #define ARSIM_COMPAT_DROP_FIRST   0x02u /* per-body sum omits the body's first particle (bodies > 0), UniformGridDEM.cu:435-442 */
 #define ARSIM_COMPAT_GRAIN_CLAMP  0x04u /* grains: position clamp + boundaryDamping reflect instead of wall DEM, particleKernels.cu:76-122 */
 #define ARSIM_DYNAMIC_AABB        0x08u /* Morton box = per-step particle AABB instead of the container box */
+#define ARSIM_CONTACT_SIS         0x10u /* contact model of updateUniformGridSIS (RigidBodies/UniformGridUpdate.cpp:734-816):
+                                          deepest contact per particle + sequential impulses, bodies only, grid method */
 
 typedef struct ArSimStepConfig {
     float        dt;
@@ -111,7 +113,7 @@ typedef struct ArSimStepConfig {
     float        boxMax[3];
     unsigned int wallMask;           /* ARSIM_WALL_* bits */
     int          method;             /* ARSIM_METHOD_* */
-    unsigned int flags;              /* ARSIM_COMPAT_* | ARSIM_DYNAMIC_AABB */
+    unsigned int flags;              /* ARSIM_COMPAT_* | ARSIM_DYNAMIC_AABB | ARSIM_CONTACT_SIS */
 } ArSimStepConfig;
 
 /* ------------------------------------------------------------------ device-memory helpers

@@ -68,6 +68,11 @@ for name, res, args in [
     ("orc_range_image", None, [VP, C.c_int, C.c_int, VP, C.c_float, VP, VP, VP, VP]),
     ("orc_dem_static_f32", C.c_int, [VP, VP, VP, VP, VP, VP, VP, C.c_int]),
     ("orc_dem_static_f64", C.c_int, [VP, VP, VP, VP, VP, VP, VP, C.c_int]),
+    ("orc_sis_find_deepest_f32", None, [C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+    ("orc_sis_find_static_f32", None, [C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+    ("orc_sis_gather_f32", C.c_long, [VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, C.c_long]),
+    ("orc_sis_solve_f32", C.c_int, [VP, VP, VP, VP, VP, VP, VP, C.c_long, C.c_int]),
+    ("orc_sis_solve_f64", C.c_int, [VP, VP, VP, VP, VP, VP, VP, C.c_long, C.c_int]),
 ]:
     f = getattr(lib, name)
     f.restype = res
@@ -218,6 +223,35 @@ def wall_force_f64(p, v, params, cfg):
     return out
 
 
+def sis_find_deepest(sorted_idx, cell_start, cell_end, sorted_pos, rb, params):
+    """Deepest contact per particle against other bodies' particles (reference layout cell tables);
+    returns (dist, partner body, partner particle) per original index (dist 0 = no contact)."""
+    n = int(sorted_idx.size)
+    dist = np.zeros(n, np.float32)
+    body = np.full(n, -1, np.int32)
+    part = np.full(n, -1, np.int32)
+    lib.orc_sis_find_deepest_f32(n, C.byref(params), _p(np.ascontiguousarray(sorted_idx, np.uint32)),
+                                 _p(np.ascontiguousarray(cell_start, np.uint32)),
+                                 _p(np.ascontiguousarray(cell_end, np.uint32)),
+                                 _p(np.ascontiguousarray(sorted_pos, np.float32)),
+                                 _p(np.ascontiguousarray(rb, np.int32)), _p(dist), _p(body), _p(part))
+    return dist, body, part
+
+
+def sis_find_static(sorted_idx, sorted_pos, static_sorted_idx, cell_start, cell_end, static_sorted_pos, params):
+    """Deepest contact per particle against the static points; returns (dist, static point index)."""
+    n = int(sorted_idx.size)
+    dist = np.zeros(n, np.float32)
+    part = np.full(n, -1, np.int32)
+    lib.orc_sis_find_static_f32(n, C.byref(params), _p(np.ascontiguousarray(sorted_idx, np.uint32)),
+                                _p(np.ascontiguousarray(sorted_pos, np.float32)),
+                                _p(np.ascontiguousarray(static_sorted_idx, np.uint32)),
+                                _p(np.ascontiguousarray(cell_start, np.uint32)),
+                                _p(np.ascontiguousarray(cell_end, np.uint32)),
+                                _p(np.ascontiguousarray(static_sorted_pos, np.float32)), _p(dist), _p(part))
+    return dist, part
+
+
 # ----------------------------------------------------------------------------- full state
 class _StateF(C.Structure):
     _fields_ = [("N", C.c_int), ("B", C.c_int)] + [(k, VP) for k in (
@@ -292,10 +326,40 @@ class State:
         s = self._struct()
         f = lib.orc_step_f32 if self.dtype == np.float32 else lib.orc_step_f64
         rc = f(C.byref(s), C.byref(self.params), C.byref(self.cfg), n)
-        assert rc == 0
+        assert rc == 0, "oracle step refused the scene (the sequential-impulse model takes bodies only, grid method)"
 
     def integrate_bodies(self):
         self._call("orc_integrate_bodies")
+
+    def sis_gather(self, dist_static, part_static, static_norm, dist, body, part):
+        """Contact list of the sequential-impulse model (static contacts first); float32 state only."""
+        assert self.dtype == np.float32
+        cap = 2 * self.N + 1
+        n4 = np.zeros((cap, 4), np.float32)
+        pt4 = np.zeros((cap, 4), np.float32)
+        b12 = np.zeros((cap, 2), np.int32)
+        bias = np.zeros(cap, np.float32)
+        s = self._struct()
+        arr = lambda a, t: None if a is None else np.ascontiguousarray(a, t)
+        cnt = lib.orc_sis_gather_f32(C.byref(s), C.byref(self.params), _p(arr(dist_static, np.float32)),
+                                     _p(arr(part_static, np.int32)), _p(arr(static_norm, np.float32)),
+                                     _p(arr(dist, np.float32)), _p(arr(body, np.int32)), _p(arr(part, np.int32)),
+                                     _p(n4), _p(pt4), _p(b12), _p(bias), cap)
+        assert cnt >= 0
+        return {"normal": n4[:cnt].copy(), "point": pt4[:cnt].copy(), "bodies": b12[:cnt].copy(),
+                "bias": bias[:cnt].copy(), "impulse": np.zeros((cnt, 4), np.float32)}
+
+    def sis_solve(self, contacts, iterations=18):
+        """`iterations` sequential sweeps over the contacts; updates body velocities and
+        contacts["impulse"] (normal, friction, friction 2 accumulators)."""
+        s = self._struct()
+        f = lib.orc_sis_solve_f32 if self.dtype == np.float32 else lib.orc_sis_solve_f64
+        acc = contacts["impulse"]
+        rc = f(C.byref(s), C.byref(self.params), _p(np.ascontiguousarray(contacts["normal"], np.float32)),
+               _p(np.ascontiguousarray(contacts["point"], np.float32)),
+               _p(np.ascontiguousarray(contacts["bodies"], np.int32)),
+               _p(np.ascontiguousarray(contacts["bias"], np.float32)), _p(acc), int(acc.shape[0]), int(iterations))
+        assert rc == 0
 
     def pose_grains(self):
         self._call("orc_pose_grains")

@@ -740,6 +740,275 @@ template <class T> void reduce_apply(State<T> &s, const Phys<T> &f) {
     }
 }
 
+/* ===================================================================== sequential impulses (SIS)
+ * Contact model of updateUniformGridSIS (RigidBodies/UniformGridUpdate.cpp:734-816): after the body
+ * integration, the deepest contact of every particle against the static scene points (if any) and
+ * against other bodies' particles, one contact per particle, gathered in body / particle order
+ * (static contacts first), then 18 Gauss-Seidel sweeps of sequential impulses over the list. */
+
+// One contact of the list (ContactNormal / ContactPoint / ContactRigidBody_1/2 / ContactBias /
+// ContactAccumulated* of Particles/particleSystem.h:606-613).
+constexpr int kSisIterations = 18;   // iterations, updateFunctions.cpp:1598
+
+template <class T> struct SisContact {
+    T n[4], pt[4];
+    int b1, b2;                 // b2 = -1: static contact
+    T bias;
+    T acc, accF1, accF2;
+};
+
+// FindRigidBodyCollisionsUniformGridKernel + ...Cell (Kernels/RB_UniformGridKernels.cu:730-845):
+// per sorted particle, the 27 cells in z, y, x order, each cell's particles in sorted order; a
+// candidate of another body (grains all share -1) whose penetration 2r - |p - q| exceeds the best
+// so far (starting at dist[orig], cleared to 0 by the caller) becomes the contact.
+template <class T>
+void sis_find_deepest(int N, const ArSimParams *p, const uint32_t *sIdx, const uint32_t *cellStart,
+                      const uint32_t *cellEnd, const T *sPos, const int *rb, T *dist, int *partnerBody,
+                      int *partnerParticle) {
+    const T thr = T(2 * p->particleRadius);
+    for (int i = 0; i < N; ++i) {
+        V3<T> pa = ld3(sPos, i);
+        const uint32_t orig = sIdx[i];
+        const int ra = rb[orig];
+        int gx = (int)std::floor(((float)pa.x - p->worldOrigin.x) / p->cellSize.x);
+        int gy = (int)std::floor(((float)pa.y - p->worldOrigin.y) / p->cellSize.y);
+        int gz = (int)std::floor(((float)pa.z - p->worldOrigin.z) / p->cellSize.z);
+        for (int z = -1; z <= 1; ++z)
+            for (int y = -1; y <= 1; ++y)
+                for (int x = -1; x <= 1; ++x) {
+                    uint32_t h = grid_hash_cell(gx + x, gy + y, gz + z, p);
+                    uint32_t st = cellStart[h];
+                    if (st == 0xffffffffu) continue;
+                    T best = dist[orig];
+                    for (uint32_t j = st; j < cellEnd[h]; ++j) {
+                        const int o2 = (int)sIdx[j];
+                        const int r2 = rb[o2];
+                        if ((int)j == i || ra == r2) continue;
+                        const V3<T> d = sub(pa, ld3(sPos, j));
+                        const T len = vsqrt(dot(d, d));
+                        if (thr - len > best) {
+                            best = thr - len;
+                            dist[orig] = best;
+                            partnerParticle[orig] = o2;
+                            partnerBody[orig] = r2;
+                        }
+                    }
+                }
+    }
+}
+
+// FindAugmentedRealityCollisionsUniformGridKernel + ...Cell (RB_UniformGridKernels.cu:878-960):
+// the same search over the static points (sorted by cell, cellStart / cellEnd of the static grid);
+// the contact records the static point's original index.
+template <class T>
+void sis_find_static(int N, const ArSimParams *p, const uint32_t *sIdx, const T *sPos, const uint32_t *stIdx,
+                     const uint32_t *cellStart, const uint32_t *cellEnd, const float *ssPos, T *dist,
+                     int *partner) {
+    const T thr = T(2 * p->particleRadius);
+    for (int i = 0; i < N; ++i) {
+        V3<T> pa = ld3(sPos, i);
+        const uint32_t orig = sIdx[i];
+        int gx = (int)std::floor(((float)pa.x - p->worldOrigin.x) / p->cellSize.x);
+        int gy = (int)std::floor(((float)pa.y - p->worldOrigin.y) / p->cellSize.y);
+        int gz = (int)std::floor(((float)pa.z - p->worldOrigin.z) / p->cellSize.z);
+        for (int z = -1; z <= 1; ++z)
+            for (int y = -1; y <= 1; ++y)
+                for (int x = -1; x <= 1; ++x) {
+                    uint32_t h = grid_hash_cell(gx + x, gy + y, gz + z, p);
+                    uint32_t st = cellStart[h];
+                    if (st == 0xffffffffu) continue;
+                    T best = dist[orig];
+                    for (uint32_t j = st; j < cellEnd[h]; ++j) {
+                        const V3<T> q = mk<T>((T)ssPos[4 * j], (T)ssPos[4 * j + 1], (T)ssPos[4 * j + 2]);
+                        const V3<T> d = sub(pa, q);
+                        const T len = vsqrt(dot(d, d));
+                        if (thr - len > best) {
+                            best = thr - len;
+                            dist[orig] = best;
+                            partner[orig] = (int)stIdx[j];
+                        }
+                    }
+                }
+    }
+}
+
+template <class T> inline T dot4(const T *a, const T *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; }
+
+// GatherAugmentedRealityCollisions (Particles/updateFunctions.cpp:1521-1578): for every body
+// particle (bodies in order, their particles in order) with a static contact: normal = the static
+// point's normal, point = relativePos + CM, bias = ARrestitution * (v + w x r) . n.
+template <class T>
+void sis_gather_static(const State<T> &s, const ArSimParams *p, const T *dist, const int *partner,
+                       const float *staticNorm, std::vector<SisContact<T>> &out) {
+    const T eps = (T)p->ARrestitution;
+    for (int b = 0; b < s.B; ++b)
+        for (int k = 0; k < s.pcount[b]; ++k) {
+            const long i = (long)s.pstart[b] + k;
+            if (!(dist[i] > T(0))) continue;
+            SisContact<T> c{};
+            const long q = partner[i];
+            for (int a = 0; a < 4; ++a) {
+                c.n[a] = (T)staticNorm[4 * q + a];
+                c.pt[a] = s.relPos[4 * i + a] + s.rbPos[4 * b + a];
+            }
+            c.b1 = b; c.b2 = -1;
+            const V3<T> r = ld3(s.relPos, i), n = mk<T>(c.n[0], c.n[1], c.n[2]);
+            const T vrel = dot(add(ld3(s.rbVel, b), cross(ld3(s.rbAng, b), r)), n);
+            c.bias = eps * vrel;
+            out.push_back(c);
+        }
+}
+
+// GatherRigidBodyCollisions (updateFunctions.cpp:1431-1519) with findExactContactPoint
+// (Particles/CPU_Engine.cpp:461-473): for every body particle with a contact, p1 = CM1 + r1,
+// p2 = CM2 + r2 (float4, w included), point = p1 + (p2 - p1) * (r / (r + r)),
+// normal = normalize(p2 - p1) * -1 (normalize = v * (1 / sqrt(v . v)) over four components),
+// bias = RBrestitution * ((v1 + w1 x r1) . n - (v2 + w2 x r2) . n).  A partner that is a grain
+// (body -1) has no body state; such contacts are left out.
+template <class T>
+void sis_gather_bodies(const State<T> &s, const ArSimParams *p, const T *dist, const int *pBody, const int *pPart,
+                       std::vector<SisContact<T>> &out) {
+    const T eps = (T)p->RBrestitution;
+    const T r = (T)p->particleRadius;
+    const T t = r / (r + r);
+    for (int b = 0; b < s.B; ++b)
+        for (int k = 0; k < s.pcount[b]; ++k) {
+            const long i = (long)s.pstart[b] + k;
+            if (!(dist[i] > T(0))) continue;
+            const int b2 = pBody[i];
+            if (b2 < 0) continue;
+            const long j = pPart[i];
+            SisContact<T> c{};
+            T p1[4], p2[4], d[4];
+            for (int a = 0; a < 4; ++a) {
+                p1[a] = s.rbPos[4 * b + a] + s.relPos[4 * i + a];
+                p2[a] = s.rbPos[4 * (long)b2 + a] + s.relPos[4 * j + a];
+                d[a] = p2[a] - p1[a];
+            }
+            const T inv = T(1) / vsqrt(dot4(d, d));
+            for (int a = 0; a < 4; ++a) {
+                c.pt[a] = p1[a] + d[a] * t;
+                c.n[a] = (d[a] * inv) * T(-1);
+            }
+            c.b1 = b; c.b2 = b2;
+            const V3<T> n = mk<T>(c.n[0], c.n[1], c.n[2]);
+            const T v1 = dot(add(ld3(s.rbVel, b), cross(ld3(s.rbAng, b), ld3(s.relPos, i))), n);
+            const T v2 = dot(add(ld3(s.rbVel, b2), cross(ld3(s.rbAng, b2), ld3(s.relPos, j))), n);
+            c.bias = eps * (v1 - v2);
+            out.push_back(c);
+        }
+}
+
+// glm::normalize(vec3) of GLM 0.9.7: v * inversesqrt(v . v), inversesqrt(x) = 1 / sqrt(x).
+template <class T> inline V3<T> gnormalize(V3<T> v) {
+    const T inv = T(1) / vsqrt(dot(v, v));
+    return mk<T>(v.x * inv, v.y * inv, v.z * inv);
+}
+// Effective mass term (I^-1 (p x d)) x p . d of SequentialImpulseSolver.
+template <class T> inline T sis_k(const T *Iinv, V3<T> p, V3<T> d) { return dot(cross(mv(Iinv, cross(p, d)), p), d); }
+
+// One contact of one sweep: the two branches of SequentialImpulseSolver
+// (Particles/updateFunctions.cpp:1580-1874).
+//  static contact (b2 == -1, :1603-1713): normal impulse clamped to [0, 100] accumulated, two
+//    friction impulses along t = normalize((1,0,0) - ((1,0,0).n) n) (or (0,1,0) when n.x is within
+//    1e-4 of +-1) and t2 = normalize(t x n), each clamped to +-ARfriction * mc * |g.y| accumulated;
+//    v += J / m, w += I^-1 (p x J) with J = jn n + jt t + jt2 t2;
+//  body contact (:1714-1860): the same normal impulse with both bodies' terms, applied with
+//    opposite signs; its friction impulses are computed but not applied (impulse_vector =
+//    normal_impulse, :1784), so they have no effect and are not formed here.
+// Velocities are stored with w = 0, as make_float4(v.x, v.y, v.z, 0) does.
+template <class T>
+void sis_contact(SisContact<T> &c, const T *rbPos, T *rbVel, T *rbAng, const T *Iw, const T *mass,
+                 const ArSimParams *p) {
+    const V3<T> n = mk<T>(c.n[0], c.n[1], c.n[2]);
+    const T upper = T(100);
+    if (c.b2 < 0) {
+        const long b = c.b1;
+        const V3<T> pp = mk<T>(c.pt[0] - rbPos[4 * b], c.pt[1] - rbPos[4 * b + 1], c.pt[2] - rbPos[4 * b + 2]);
+        const T *I = Iw + 9 * b;
+        const T m = mass[b];
+        V3<T> v = ld3(rbVel, b), w = ld3(rbAng, b);
+        T mc = T(1) / m + sis_k(I, pp, n);
+        if (std::fabs((double)mc) < 0.00001) mc = T(1);
+        const T vrel = dot(add(v, cross(w, pp)), n);
+        T ci = -(vrel + c.bias) / mc;
+        T tmp = c.acc + ci;
+        if (tmp < T(0)) tmp = T(0);
+        else if (tmp > upper) tmp = upper;
+        ci = tmp - c.acc;
+        c.acc = tmp;
+        const T fb = (T)p->ARfriction * mc * std::fabs((T)p->gravity.y);
+        const V3<T> vel = add(v, cross(w, pp));
+        V3<T> t = mk<T>(T(1), T(0), T(0));
+        if (std::fabs((double)(t.x - n.x)) < 0.0001) t = mk<T>(T(0), T(1), T(0));
+        else if (std::fabs((double)(t.x + n.x)) < 0.0001) t = mk<T>(T(0), T(1), T(0));
+        t = sub(t, scl(dot(t, n), n));
+        t = gnormalize(t);
+        const T kt = T(1) / m + sis_k(I, pp, t);
+        T cf = -dot(vel, t) / kt;
+        T tf = c.accF1 + cf;
+        if (tf < -fb) tf = -fb;
+        else if (tf > fb) tf = fb;
+        cf = tf - c.accF1;
+        c.accF1 = tf;
+        const V3<T> t2 = gnormalize(cross(t, n));
+        const T kt2 = T(1) / m + sis_k(I, pp, t2);
+        T cf2 = -dot(vel, t2) / kt2;
+        T tf2 = c.accF2 + cf2;
+        if (tf2 < -fb) tf2 = -fb;
+        else if (tf2 > fb) tf2 = fb;
+        cf2 = tf2 - c.accF2;
+        c.accF2 = tf2;
+        const V3<T> J = add(add(scl(ci, n), scl(cf, t)), scl(cf2, t2));
+        v = add(v, divs(J, m));
+        w = add(w, mv(I, cross(pp, J)));
+        st4(rbVel, b, v, T(0));
+        st4(rbAng, b, w, T(0));
+    } else {
+        const long b1 = c.b1, b2 = c.b2;
+        const V3<T> p1 = mk<T>(c.pt[0] - rbPos[4 * b1], c.pt[1] - rbPos[4 * b1 + 1], c.pt[2] - rbPos[4 * b1 + 2]);
+        const V3<T> p2 = mk<T>(c.pt[0] - rbPos[4 * b2], c.pt[1] - rbPos[4 * b2 + 1], c.pt[2] - rbPos[4 * b2 + 2]);
+        const T *I1 = Iw + 9 * b1, *I2 = Iw + 9 * b2;
+        const T m1 = mass[b1], m2 = mass[b2];
+        V3<T> v1 = ld3(rbVel, b1), w1 = ld3(rbAng, b1), v2 = ld3(rbVel, b2), w2 = ld3(rbAng, b2);
+        T mc = T(1) / m1 + T(1) / m2 + sis_k(I1, p1, n) + sis_k(I2, p2, n);
+        if (std::fabs((double)mc) < 0.00001) mc = T(1);
+        const T vrel = dot(add(v1, cross(w1, p1)), n) - dot(add(v2, cross(w2, p2)), n);
+        T ci = -(vrel + c.bias) / mc;
+        T tmp = c.acc + ci;
+        if (tmp < T(0)) tmp = T(0);
+        else if (tmp > upper) tmp = upper;
+        ci = tmp - c.acc;
+        c.acc = tmp;
+        const V3<T> J = scl(ci, n);
+        v1 = add(v1, divs(J, m1));
+        w1 = add(w1, mv(I1, cross(p1, J)));
+        st4(rbVel, b1, v1, T(0));
+        st4(rbAng, b1, w1, T(0));
+        v2 = sub(v2, divs(J, m2));
+        w2 = sub(w2, mv(I2, cross(p2, J)));
+        st4(rbVel, b2, v2, T(0));
+        st4(rbAng, b2, w2, T(0));
+    }
+}
+
+template <class T>
+void sis_solve(std::vector<SisContact<T>> &cs, const T *rbPos, T *rbVel, T *rbAng, const T *Iw, const T *mass,
+               const ArSimParams *p, int iterations) {
+    for (int k = 0; k < iterations; ++k)
+        for (auto &c : cs) sis_contact(c, rbPos, rbVel, rbAng, Iw, mass, p);
+}
+
+// One SIS step (updateUniformGridSIS, RigidBodies/UniformGridUpdate.cpp:734-797): integrate the
+// bodies and pose their particles (Integrate_Rigid_Body_System_GPU), static contacts then body
+// contacts, the solver, and the particles' pose from the solved velocities (the
+// computeGlobalAttributesWrapper call at the end of SequentialImpulseSolver).
+template <class T> struct SisScratch {
+    std::vector<T> distA, distB;
+    std::vector<int> partA, bodyB, partB;
+    std::vector<SisContact<T>> contacts;
+};
+
 template <class T> struct Scratch {
     std::vector<uint32_t> hash, idx, sh, si, cs, ce;
     std::vector<uint64_t> k64, sk64;
@@ -757,7 +1026,15 @@ template <class T>
 int step_impl(State<T> &s, const ArSimParams *p, const ArSimStepConfig *cfg, int nsteps) {
     Phys<T> f = mkphys<T>(p, cfg);
     Scratch<T> w;
+    SisScratch<T> sw;
     const int N = s.N;
+    const bool sis = (cfg->flags & ARSIM_CONTACT_SIS) != 0;
+    if (sis) {
+        // the SIS model is defined for bodies on the grid broadphase
+        if (cfg->method != ARSIM_METHOD_GRID) return -1;
+        for (int i = 0; i < N; ++i)
+            if (s.rb[i] < 0) return -1;
+    }
     for (int it = 0; it < nsteps; ++it) {
         // stage order of updateUniformGridDEM (RigidBodies/UniformGridUpdate.cpp:939-999)
         integrate_bodies(s, f);
@@ -778,7 +1055,8 @@ int step_impl(State<T> &s, const ArSimParams *p, const ArSimStepConfig *cfg, int
                 for (int c = 0; c < 4; ++c) { w.sPos[4 * i + c] = s.pos[4 * o + c]; w.sVel[4 * i + c] = s.vel[4 * o + c]; }
                 w.sRb[i] = s.rb[o];
             }
-            dem_grid(s, f, p, w.si.data(), w.cs.data(), w.ce.data(), w.sPos.data(), w.sVel.data(), w.sRb.data());
+            if (!sis)
+                dem_grid(s, f, p, w.si.data(), w.cs.data(), w.ce.data(), w.sPos.data(), w.sVel.data(), w.sRb.data());
         } else {
             float bmin[3] = {cfg->boxMin[0], cfg->boxMin[1], cfg->boxMin[2]};
             float bmax[3] = {cfg->boxMax[0], cfg->boxMax[1], cfg->boxMax[2]};
@@ -836,9 +1114,28 @@ int step_impl(State<T> &s, const ArSimParams *p, const ArSimStepConfig *cfg, int
                     w.stPos[4 * j + c] = s.staticPos[4 * (size_t)w.stSI[j] + c];
                     w.stNorm[4 * j + c] = s.staticNorm[4 * (size_t)w.stSI[j] + c];
                 }
-            dem_static(s, f, p, w.stCs.data(), w.stCe.data(), w.stPos.data(), w.stNorm.data());
+            if (!sis) dem_static(s, f, p, w.stCs.data(), w.stCe.data(), w.stPos.data(), w.stNorm.data());
         }
-        reduce_apply(s, f);
+        if (!sis) {
+            reduce_apply(s, f);
+            continue;
+        }
+        sw.contacts.clear();
+        if (s.numStatic > 0) {
+            sw.distA.assign(N, T(0));
+            sw.partA.assign(N, -1);
+            sis_find_static(N, p, w.si.data(), w.sPos.data(), w.stSI.data(), w.stCs.data(), w.stCe.data(),
+                            w.stPos.data(), sw.distA.data(), sw.partA.data());
+            sis_gather_static(s, p, sw.distA.data(), sw.partA.data(), s.staticNorm, sw.contacts);
+        }
+        sw.distB.assign(N, T(0));
+        sw.bodyB.assign(N, -1);
+        sw.partB.assign(N, -1);
+        sis_find_deepest(N, p, w.si.data(), w.cs.data(), w.ce.data(), w.sPos.data(), s.rb, sw.distB.data(),
+                         sw.bodyB.data(), sw.partB.data());
+        sis_gather_bodies(s, p, sw.distB.data(), sw.bodyB.data(), sw.partB.data(), sw.contacts);
+        sis_solve(sw.contacts, s.rbPos, s.rbVel, s.rbAng, s.Iw, s.mass, p, kSisIterations);
+        pose_and_grains(s, f);
     }
     return 0;
 }
@@ -886,6 +1183,27 @@ template <class T, class S> static State<T> to_state(S *o) {
     s.model = o->model;
     s.staticPos = o->staticPos; s.staticNorm = o->staticNorm; s.numStatic = o->numStatic;
     return s;
+}
+
+/* `iterations` sweeps over the given contacts; acc4 holds (normal, friction, friction 2, 0)
+ * accumulators in and out. */
+template <class T, class S>
+static int sis_solve_export(S *o, const ArSimParams *p, const float *n4, const float *pt4, const int *b12,
+                            const float *bias, float *acc4, long count, int iterations) {
+    State<T> s = to_state<T>(o);
+    std::vector<SisContact<T>> cs((size_t)count);
+    for (long c = 0; c < count; ++c) {
+        for (int a = 0; a < 4; ++a) { cs[c].n[a] = (T)n4[4 * c + a]; cs[c].pt[a] = (T)pt4[4 * c + a]; }
+        cs[c].b1 = b12[2 * c]; cs[c].b2 = b12[2 * c + 1];
+        cs[c].bias = (T)bias[c];
+        cs[c].acc = (T)acc4[4 * c]; cs[c].accF1 = (T)acc4[4 * c + 1]; cs[c].accF2 = (T)acc4[4 * c + 2];
+    }
+    sis_solve(cs, s.rbPos, s.rbVel, s.rbAng, s.Iw, s.mass, p, iterations);
+    for (long c = 0; c < count; ++c) {
+        acc4[4 * c] = (float)cs[c].acc; acc4[4 * c + 1] = (float)cs[c].accF1; acc4[4 * c + 2] = (float)cs[c].accF2;
+        acc4[4 * c + 3] = 0.f;
+    }
+    return 0;
 }
 
 extern "C" {
@@ -1026,6 +1344,43 @@ void orc_range_image(const uint16_t *img, int W, int H, const float *cam, float 
             const float inv = 1.0f / std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
             norm[4 * idx] = n0 * inv; norm[4 * idx + 1] = n1 * inv; norm[4 * idx + 2] = n2 * inv; norm[4 * idx + 3] = 0.f;
         }
+}
+
+/* Sequential-impulse stages (deepest contacts, gather, solve) for stage-level checks. */
+void orc_sis_find_deepest_f32(int N, const ArSimParams *p, const uint32_t *sIdx, const uint32_t *cellStart,
+                              const uint32_t *cellEnd, const float *sPos, const int *rb, float *dist, int *pBody,
+                              int *pPart) {
+    sis_find_deepest<float>(N, p, sIdx, cellStart, cellEnd, sPos, rb, dist, pBody, pPart);
+}
+void orc_sis_find_static_f32(int N, const ArSimParams *p, const uint32_t *sIdx, const float *sPos,
+                             const uint32_t *stIdx, const uint32_t *cellStart, const uint32_t *cellEnd,
+                             const float *ssPos, float *dist, int *partner) {
+    sis_find_static<float>(N, p, sIdx, sPos, stIdx, cellStart, cellEnd, ssPos, dist, partner);
+}
+/* Contact list (static contacts first, then body contacts) into n4 / pt4 (float4 each), b12 (two ints),
+ * bias; returns the count, or -1 past cap.  distA / partA / staticNorm may be NULL (no static set). */
+long orc_sis_gather_f32(OrcStateF *o, const ArSimParams *p, const float *distA, const int *partA,
+                        const float *staticNorm, const float *distB, const int *bodyB, const int *partB, float *n4,
+                        float *pt4, int *b12, float *bias, long cap) {
+    State<float> s = to_state<float>(o);
+    std::vector<SisContact<float>> cs;
+    if (distA) sis_gather_static(s, p, distA, partA, staticNorm, cs);
+    sis_gather_bodies(s, p, distB, bodyB, partB, cs);
+    if ((long)cs.size() > cap) return -1;
+    for (size_t c = 0; c < cs.size(); ++c) {
+        for (int a = 0; a < 4; ++a) { n4[4 * c + a] = cs[c].n[a]; pt4[4 * c + a] = cs[c].pt[a]; }
+        b12[2 * c] = cs[c].b1; b12[2 * c + 1] = cs[c].b2;
+        bias[c] = cs[c].bias;
+    }
+    return (long)cs.size();
+}
+int orc_sis_solve_f32(OrcStateF *o, const ArSimParams *p, const float *n4, const float *pt4, const int *b12,
+                      const float *bias, float *acc4, long count, int iterations) {
+    return sis_solve_export<float>(o, p, n4, pt4, b12, bias, acc4, count, iterations);
+}
+int orc_sis_solve_f64(OrcStateD *o, const ArSimParams *p, const float *n4, const float *pt4, const int *b12,
+                      const float *bias, float *acc4, long count, int iterations) {
+    return sis_solve_export<double>(o, p, n4, pt4, b12, bias, acc4, count, iterations);
 }
 
 /* Static scene DEM as a stage on a state (sorted static points, their normals, static cell bounds). */
