@@ -175,6 +175,12 @@ SIGNATURES = {
     "arsim_engine_set_static": (I, [VP, VP, VP, I]),
     "arsim_range_image_load": (I, [VP, I, I, VP, F, VP, VP, VP, VP, VP]),
     "arsim_dem_static_grid": (I, [VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, I, VP, VP, VP]),
+    "arsim_find_deepest_contacts_grid": (I, [VP, VP, VP, VP, VP, VP, VP, VP, VP, I, VP, VP]),
+    "arsim_find_static_contacts_grid": (I, [VP, VP, VP, VP, VP, VP, VP, VP, VP, I, VP, VP]),
+    "arsim_sis_workspace_bytes": (SZ, [I, I]),
+    "arsim_sis_gather": (I, [VP, VP, VP, VP, VP, I, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, I, VP, VP, SZ,
+                             VP]),
+    "arsim_sis_solve": (I, [VP, VP, VP, VP, VP, VP, I, VP, VP, VP, VP, VP, I, I, VP, VP, SZ, VP]),
 }
 
 

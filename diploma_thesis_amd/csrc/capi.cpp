@@ -448,4 +448,107 @@ int arsim_dem_static_grid(float *pForce, float *pTorque, const float *sortedPos,
     return check_launch("dem_static_grid");
 }
 
+int arsim_find_deepest_contacts_grid(const int *rbIndices, int *collidingRigidBodyIndex, int *collidingParticleIndex,
+                                     float *contactDistance, float *color, const float *sortedPos,
+                                     const unsigned int *gridParticleIndex, const unsigned int *cellStart,
+                                     const unsigned int *cellEnd, int numParticles, const ArSimParams *params,
+                                     arsim_stream_t st) {
+    if (numParticles < 0 || !params) return fail(ARSIM_ERR_INVALID_ARG, "find_deepest_contacts_grid: bad arguments");
+    if (numParticles == 0) return ARSIM_OK;
+    if (!rbIndices || !collidingRigidBodyIndex || !collidingParticleIndex || !contactDistance || !sortedPos ||
+        !gridParticleIndex || !cellStart || !cellEnd)
+        return fail(ARSIM_ERR_INVALID_ARG, "find_deepest_contacts_grid: null buffer");
+    ArSimStepConfig cfg{};
+    SceneGeom g{};
+    int rc = make_geom(*params, cfg, 0, numParticles, g);
+    if (rc) return rc;
+    launch_sis_find_ref((const float4 *)sortedPos, gridParticleIndex, rbIndices, cellStart, cellEnd, g, numParticles,
+                        params->particleRadius, contactDistance, collidingRigidBodyIndex, collidingParticleIndex,
+                        (float4 *)color, S(st));
+    return check_launch("find_deepest_contacts_grid");
+}
+
+int arsim_find_static_contacts_grid(int *collidingParticleIndex, float *contactDistance, float *color,
+                                    const float *sortedPos, const float *staticSortedPos,
+                                    const unsigned int *gridParticleIndex, const unsigned int *staticSortedIndex,
+                                    const unsigned int *staticCellStart, const unsigned int *staticCellEnd,
+                                    int numParticles, const ArSimParams *params, arsim_stream_t st) {
+    if (numParticles < 0 || !params) return fail(ARSIM_ERR_INVALID_ARG, "find_static_contacts_grid: bad arguments");
+    if (numParticles == 0) return ARSIM_OK;
+    if (!collidingParticleIndex || !contactDistance || !sortedPos || !staticSortedPos || !gridParticleIndex ||
+        !staticSortedIndex || !staticCellStart || !staticCellEnd)
+        return fail(ARSIM_ERR_INVALID_ARG, "find_static_contacts_grid: null buffer");
+    ArSimStepConfig cfg{};
+    SceneGeom g{};
+    int rc = make_geom(*params, cfg, 0, numParticles, g);
+    if (rc) return rc;
+    launch_sis_find_static((const float4 *)sortedPos, gridParticleIndex, numParticles, (const float4 *)staticSortedPos,
+                           staticSortedIndex, staticCellStart, staticCellEnd, g, params->particleRadius, true,
+                           contactDistance, collidingParticleIndex, (float4 *)color, S(st));
+    return check_launch("find_static_contacts_grid");
+}
+
+size_t arsim_sis_workspace_bytes(int maxContacts, int numBodyParticles) {
+    if (maxContacts < 1 || numBodyParticles < 0) return 0;
+    return sis_workspace_bytes(maxContacts, 1, numBodyParticles);
+}
+
+int arsim_sis_gather(const float *rbPositions, const float *rbVelocities, const float *rbAngularVelocity,
+                     const float *relativePos, const int *rbIndices, int numBodyParticles,
+                     const float *staticContactDistance, const int *staticContactIndex, const float *staticNorm,
+                     const float *contactDistance, const int *collidingRigidBodyIndex,
+                     const int *collidingParticleIndex, float *contactNormal, float *contactPoint,
+                     int *contactBodies, float *contactBias, float *contactImpulse, int *numContacts,
+                     int maxContacts, const ArSimParams *params, void *workspace, size_t workspaceBytes,
+                     arsim_stream_t st) {
+    if (numBodyParticles < 0 || maxContacts < 1 || !params)
+        return fail(ARSIM_ERR_INVALID_ARG, "sis_gather: bad arguments");
+    if (!rbPositions || !rbVelocities || !rbAngularVelocity || !relativePos || !rbIndices || !contactDistance ||
+        !collidingRigidBodyIndex || !collidingParticleIndex || !contactNormal || !contactPoint || !contactBodies ||
+        !contactBias || !contactImpulse || !numContacts || !workspace)
+        return fail(ARSIM_ERR_INVALID_ARG, "sis_gather: null buffer");
+    if (staticContactDistance && (!staticContactIndex || !staticNorm))
+        return fail(ARSIM_ERR_INVALID_ARG, "sis_gather: static contacts need their indices and normals");
+    const long long need = (long long)numBodyParticles * (staticContactDistance ? 2 : 1);
+    if (need > maxContacts) return fail(ARSIM_ERR_OVERFLOW, "sis_gather: maxContacts below the worst case");
+    if (workspaceBytes < sis_workspace_bytes(maxContacts, 1, numBodyParticles))
+        return fail(ARSIM_ERR_WORKSPACE, "sis_gather: workspace too small");
+    const SisWork w = sis_work_layout(workspace, maxContacts, 1, numBodyParticles);
+    const SisGatherIn gi{staticContactDistance, staticContactIndex, (const float4 *)staticNorm, contactDistance,
+                         collidingRigidBodyIndex, collidingParticleIndex, rbIndices, nullptr,
+                         (const float4 *)rbPositions, (const float4 *)rbVelocities, (const float4 *)rbAngularVelocity,
+                         (const float4 *)relativePos, params->particleRadius, params->ARrestitution,
+                         params->RBrestitution};
+    const SisContacts ct{(float4 *)contactNormal, (float4 *)contactPoint, (int2 *)contactBodies, contactBias,
+                         (float4 *)contactImpulse};
+    launch_sis_gather(gi, numBodyParticles, ct, w, false, S(st));
+    ARSIM_HIP(hipMemcpyAsync(numContacts, w.numContacts, sizeof(int), hipMemcpyDeviceToDevice, S(st)));
+    return check_launch("sis_gather");
+}
+
+int arsim_sis_solve(float *contactNormal, float *contactPoint, int *contactBodies, float *contactBias,
+                    float *contactImpulse, const int *numContacts, int maxContacts, const float *rbPositions,
+                    float *rbVelocities, float *rbAngularVelocity, const float *rbCurrentInertia, const float *rbMass,
+                    int numBodies, int iterations, const ArSimParams *params, void *workspace,
+                    size_t workspaceBytes, arsim_stream_t st) {
+    if (maxContacts < 1 || numBodies < 0 || iterations < 0 || !params)
+        return fail(ARSIM_ERR_INVALID_ARG, "sis_solve: bad arguments");
+    if (!contactNormal || !contactPoint || !contactBodies || !contactBias || !contactImpulse || !numContacts ||
+        !rbPositions || !rbVelocities || !rbAngularVelocity || !rbCurrentInertia || !rbMass || !workspace)
+        return fail(ARSIM_ERR_INVALID_ARG, "sis_solve: null buffer");
+    if (workspaceBytes < sis_workspace_bytes(maxContacts, 1, 0))
+        return fail(ARSIM_ERR_WORKSPACE, "sis_solve: workspace too small");
+    const SisWork w = sis_work_layout(workspace, maxContacts, 1, 0);
+    ARSIM_HIP(hipMemcpyAsync(w.numContacts, numContacts, sizeof(int), hipMemcpyDeviceToDevice, S(st)));
+    const SisContacts ct{(float4 *)contactNormal, (float4 *)contactPoint, (int2 *)contactBodies, contactBias,
+                         (float4 *)contactImpulse};
+    const SisBodies bd{(const float4 *)rbPositions, (float4 *)rbVelocities, (float4 *)rbAngularVelocity,
+                       rbCurrentInertia, rbMass};
+    const SisSolvePhys sp{params->ARfriction, params->gravity.y};
+    SortTuning stt{};
+    if (launch_sis_solve(ct, bd, sp, maxContacts, 1, numBodies, iterations, w, stt, S(st)))
+        return fail(ARSIM_ERR_INVALID_ARG, "sis_solve: contact sort rejected");
+    return check_launch("sis_solve");
+}
+
 }  // extern "C"

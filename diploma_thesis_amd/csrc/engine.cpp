@@ -120,6 +120,10 @@ int Engine::set_static(const float *pos4, const float *norm4, int n) {
     invalidate_graph();
     free_static();
     if (n == 0) return ARSIM_OK;
+    if ((flags & ARSIM_CONTACT_SIS) && sisCap < 2 * N) {
+        const int rc0 = alloc_sis(2 * N);   // room for one static and one body contact per particle
+        if (rc0) return rc0;
+    }
     const SceneGeom &g = geoms[0];
     const unsigned cells = (g.gridMask[0] + 1u) * (g.gridMask[1] + 1u) * (g.gridMask[2] + 1u);
     const size_t wsBytes = sort_workspace_bytes(n, 4);
@@ -200,7 +204,10 @@ PartBufs Engine::part_bufs() const {
     return p;
 }
 
-const char *const *Engine::stage_names(int method, int *count) {
+const char *const *Engine::stage_names(int method, unsigned flags, int *count) {
+    static const char *const sis[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "sis_find",
+                                      "sis_gather", "sis_solve", "pose"};
+    if (flags & ARSIM_CONTACT_SIS) { *count = 8; return sis; }
     static const char *const grid[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "dem_grid",
                                        "reduce_apply"};
     static const char *const lbvh[] = {"body_integrate", "pose_key", "sort", "lbvh_build", "lbvh_refit", "dem_lbvh",
@@ -248,6 +255,12 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
                                      !h->rbInertia || !h->rbMass || !h->particlesPerObject))
                 return fail(ARSIM_ERR_INVALID_ARG, "engine: null body array");
             d[s] = Desc{h->numParticles, h->numBodies, h->params, h->cfg};
+            if (h->cfg.flags & ARSIM_CONTACT_SIS) {
+                long long inBodies = 0;
+                for (int k = 0; k < h->numBodies; ++k) inBodies += h->particlesPerObject[k];
+                if (inBodies != h->numParticles)
+                    return fail(ARSIM_ERR_UNSUPPORTED, "engine: the sequential-impulse model takes bodies only (no grains)");
+            }
         } else if (src[s].from) {
             const Engine *o = src[s].from;
             if (o == this) return fail(ARSIM_ERR_INVALID_ARG, "engine: a rebuild cannot read its own buffers");
@@ -273,6 +286,8 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         return fail(ARSIM_ERR_UNSUPPORTED, "engine: 63-bit Morton codes leave no key bits for scene ids");
     if ((flags & ARSIM_DYNAMIC_AABB) && S > 1)
         return fail(ARSIM_ERR_UNSUPPORTED, "engine: a dynamic Morton box is per scene only with one scene");
+    if ((flags & ARSIM_CONTACT_SIS) && method != ARSIM_METHOD_GRID)
+        return fail(ARSIM_ERR_UNSUPPORTED, "engine: the sequential-impulse model runs on the grid method");
     geoms.resize(S);
     pStart.resize(S); pCount.resize(S); bStart.resize(S); bCount.resize(S);
     long long nTot = 0, bTot = 0;
@@ -311,6 +326,8 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         // beyond that, segments of 2^segBits cells keep the table at that size
         segBits = keyBits > 27 ? keyBits - 27 : 0;
         numSegs = (long long)((numCells - 1) >> segBits) + 1;
+        if ((flags & ARSIM_CONTACT_SIS) && segBits != 0)
+            return fail(ARSIM_ERR_UNSUPPORTED, "engine: the sequential-impulse model needs at most 2^27 cells");
         long long base = 0;
         for (int s = 0; s < S; ++s) {
             geoms[s].cellBase = (unsigned)base;
@@ -356,6 +373,10 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         if (S > 1) A(sceneRootBuf, S);
         tree.sceneRoot = S > 1 ? sceneRootBuf : nullptr;
         A(aabbPart, (size_t)2 * aabb_max_partials()); A(aabbBox, 2);
+    }
+    if (flags & ARSIM_CONTACT_SIS) {
+        A(sisDistA, N); A(sisPartA, N); A(sisDistB, N); A(sisBodyB, N); A(sisPartB, N);
+        if ((rc = alloc_sis(N)) != 0) return rc;
     }
 #undef A
 
@@ -456,6 +477,20 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     return ARSIM_OK;
 }
 
+// Contact list of `cap` entries and the sweeps' scratch (one body contact per particle, plus one
+// static contact per particle once a static set is attached).
+int Engine::alloc_sis(int cap) {
+    int rc = 0;
+    if ((rc = alloc(&cNormal, (size_t)cap)) || (rc = alloc(&cPoint, (size_t)cap)) || (rc = alloc(&cImpulse, (size_t)cap)) ||
+        (rc = alloc(&cBodies, (size_t)cap)) || (rc = alloc(&cBias, (size_t)cap)))
+        return rc;
+    const size_t ws = sis_workspace_bytes(cap, S, N);
+    if ((rc = alloc(&sisWsBuf, ws))) return rc;
+    sisW = sis_work_layout(sisWsBuf, cap, S, N);
+    sisCap = cap;
+    return ARSIM_OK;
+}
+
 int Engine::reset_grid_tables() {
     invalidate_graph();
     first = true;
@@ -477,7 +512,46 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         ++k;
     };
     rec();
-    if (method == ARSIM_METHOD_GRID) {
+    if (flags & ARSIM_CONTACT_SIS) {
+        // updateUniformGridSIS (RigidBodies/UniformGridUpdate.cpp:734-797): integrate and pose, the grid
+        // broadphase, deepest contacts (static set first), the contact list, the sweeps, the pose again
+        const SortPlan pl = sort_plan(sortWs, N, keyBytes, 0, keyBits, stt);
+        const ZeroJobs z = sort_zero_jobs(pl);
+        launch_body_integrate(bb, B, ph, &z, st);
+        rec();
+        KeyOut ko{keysA, nullptr, nullptr};
+        attach_hist(ko, pl, stt, keyBits);
+        launch_pose_key(pb, bb, N, ph, sv, kKeyGrid, ko, true, true, st);
+        rec();
+        unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
+        if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st, true))
+            return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
+        rec();
+        launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart,
+                           segBits, numSegs, N, st);
+        rec();
+        if (numStatic)
+            launch_sis_find_static(sortedPos, vA, N, stPos, nullptr, stTable, nullptr, geoms[0], ph.r, false, sisDistA,
+                                   sisPartA, nullptr, st);
+        launch_sis_find(sortedPos, sortedVel, vA, segStart, sv, S > 1, N, ph.r, sisDistB, sisBodyB, sisPartB, st);
+        rec();
+        const SisGatherIn gi{numStatic ? sisDistA : nullptr, sisPartA, stNorm, sisDistB, sisBodyB, sisPartB, rb,
+                             S > 1 ? scene : nullptr, rbPos, rbVel, rbAng, relPos, ph.r, p0.ARrestitution,
+                             p0.RBrestitution};
+        const SisContacts ct{cNormal, cPoint, cBodies, cBias, cImpulse};
+        launch_sis_gather(gi, N, ct, sisW, S > 1, st);
+        rec();
+        const SisBodies bd{rbPos, rbVel, rbAng, Iw, mass};
+        const SisSolvePhys sp{p0.ARfriction, p0.gravity.y};
+        if (launch_sis_solve(ct, bd, sp, sisCap, S, B, kSisIterations, sisW, stt, st))
+            return fail(ARSIM_ERR_INVALID_ARG, "engine: contact sort rejected");
+        rec();
+        KeyOut none{};
+        launch_pose_key(pb, bb, N, ph, sv, kKeyNone, none, true, false, st);
+        rec();
+        lastKeys = kA;
+        lastIdx = vA;
+    } else if (method == ARSIM_METHOD_GRID) {
         // the sort's workspace is cleared by the body launch and its histograms are built while the
         // keys are generated, so the sort itself is only its digit passes
         const SortPlan pl = sort_plan(sortWs, N, keyBytes, 0, keyBits, stt);
@@ -855,7 +929,7 @@ int arsim_engine_profile_step(arsim_engine *e, float *ms, const char **names, in
     if (!e) return fail(ARSIM_ERR_INVALID_ARG, "null engine");
     Engine &E = e->e;
     int count = 0;
-    const char *const *nm = Engine::stage_names(E.method, &count);
+    const char *const *nm = Engine::stage_names(E.method, E.flags, &count);
     std::vector<hipEvent_t> ev(count + 1);
     for (auto &x : ev) ARSIM_HIP(hipEventCreate(&x));
     int ns = 0;

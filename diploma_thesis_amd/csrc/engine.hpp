@@ -65,6 +65,15 @@ struct Engine {
     int numStatic = 0;
     float4 *stPos = nullptr, *stNorm = nullptr;
     unsigned *stTable = nullptr;
+    // sequential-impulse contact model: deepest contacts per particle, the contact list, scratch
+    int sisCap = 0;
+    float *sisDistA = nullptr, *sisDistB = nullptr;
+    int *sisPartA = nullptr, *sisBodyB = nullptr, *sisPartB = nullptr;
+    float4 *cNormal = nullptr, *cPoint = nullptr, *cImpulse = nullptr;
+    int2 *cBodies = nullptr;
+    float *cBias = nullptr;
+    unsigned char *sisWsBuf = nullptr;
+    SisWork sisW{};
 
     // step state
     bool first = true;
@@ -83,7 +92,7 @@ struct Engine {
     int set_static(const float *pos4, const float *norm4, int n);
     void free_static();
     int reset_grid_tables();
-    static const char *const *stage_names(int method, int *count);
+    static const char *const *stage_names(int method, unsigned flags, int *count);
     SceneView scene_view() const;
     BodyBufs body_bufs() const;
     StateBufs state_bufs() const;
@@ -92,6 +101,7 @@ struct Engine {
 
   private:
     template <class T> int alloc(T **p, size_t count);
+    int alloc_sis(int cap);
 };
 
 }  // namespace arsim

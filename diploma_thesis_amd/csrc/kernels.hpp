@@ -185,4 +185,73 @@ void launch_dem_static(const float4 *sPos, const float4 *sVel, const unsigned *s
                        const DevPhys &ph, const SceneGeom &g, float4 *pForce, float4 *pTorque,
                        const float4 *relPos, bool refLayout, hipStream_t st);
 
+
+// ---------------------------------------------------------------- sequential impulses (sis.hip)
+constexpr int kSisIterations = 18;   // SequentialImpulseSolver sweeps (Particles/updateFunctions.cpp:1598)
+// Inputs of the contact gather, per body particle in the original order.
+struct SisGatherIn {
+    const float *distA;            // static contacts: penetration (NULL: no static set)
+    const int *partA;              //   the static point (index into staticNorm)
+    const float4 *staticNorm;
+    const float *distB;            // body contacts: penetration, partner body and partner particle
+    const int *bodyB, *partB;
+    const int *rb;
+    const uint16_t *scene;         // per-particle scene id (NULL: one scene)
+    const float4 *rbPos, *rbVel, *rbAng, *relPos;
+    float radius, arRest, rbRest;
+};
+// The contact list (ContactNormal / ContactPoint / ContactRigidBody_1/2 / ContactBias /
+// ContactAccumulated* of Particles/particleSystem.h:606-613); impulse = (normal, friction,
+// friction 2, 0) accumulators.
+struct SisContacts {
+    float4 *normal, *point;
+    int2 *bodies;                  // (b1, b2), b2 = -1 for a static contact
+    float *bias;
+    float4 *impulse;
+};
+struct SisBodies {
+    const float4 *pos;
+    float4 *vel, *ang;
+    const float *Iw;               // inverse world-frame inertia, 9 per body, column-major
+    const float *mass;
+};
+struct SisSolvePhys {
+    float arFric, gy;
+};
+// Scratch of the gather and the level-scheduled sweeps, carved from one device buffer.
+struct SisWork {
+    unsigned *keysA, *keysB, *valsA, *valsB;
+    void *sortWs;
+    size_t sortWsBytes;
+    int *pred, *lvl, *order, *off, *cur, *maxLvl;
+    int2 *sceneRange;
+    unsigned *blockCnt;
+    int *numContacts;
+    size_t end;
+};
+// Deepest contact per sorted particle against other bodies' particles (engine layout: exact cell
+// table, body index in sortedPos.w, scene id in sortedVel.w when sceneInVelW).
+void launch_sis_find(const float4 *sPos, const float4 *sVel, const unsigned *sIdx, const unsigned *segStart,
+                     const SceneView &sv, bool sceneInVelW, int n, float radius, float *dist, int *pBody, int *pPart,
+                     hipStream_t st);
+// The same in the reference layout (cellStart / cellEnd, rbIndices), writing only found contacts.
+void launch_sis_find_ref(const float4 *sPos, const unsigned *sIdx, const int *rbIdx, const unsigned *cellStart,
+                         const unsigned *cellEnd, const SceneGeom &g, int n, float radius, float *dist, int *pBody,
+                         int *pPart, float4 *color, hipStream_t st);
+// Deepest contact against the static points (refLayout: static cellStart / cellEnd and original
+// indices; otherwise the exact static table and sorted indices).
+void launch_sis_find_static(const float4 *sPos, const unsigned *sIdx, int n, const float4 *stPos,
+                            const unsigned *stIdx, const unsigned *cellStart, const unsigned *cellEnd,
+                            const SceneGeom &g, float radius, bool refLayout, float *dist, int *partner,
+                            float4 *color, hipStream_t st);
+size_t sis_workspace_bytes(int cap, int numScenes, int numBodyParticles);
+SisWork sis_work_layout(void *base, int cap, int numScenes, int numBodyParticles);
+// Contact list of nb body particles into out (static contacts first); count in w.numContacts.
+// perScene: the particles' scene ids give each scene's contact range (no static set then).
+void launch_sis_gather(const SisGatherIn &in, int nb, const SisContacts &out, const SisWork &w, bool perScene,
+                       hipStream_t st);
+// Level schedule and `iterations` sweeps per scene; cap = capacity of the contact list.
+int launch_sis_solve(const SisContacts &ct, const SisBodies &bd, const SisSolvePhys &ph, int cap, int numScenes,
+                     int numBodies, int iterations, const SisWork &w, const SortTuning &tu, hipStream_t st);
+
 }  // namespace arsim

@@ -1,0 +1,650 @@
+// sis.hip -- the sequential-impulse contact model on the device: the deepest contact of every
+// particle, the contact list in the reference's order, and the Gauss-Seidel impulse sweeps.
+//
+// The sweeps keep the reference's sequential semantics exactly: contact c depends on the last
+// contact before it that touches one of its bodies (its predecessors); level(c) = 1 + the larger
+// of its predecessors' levels.  Contacts of one level touch disjoint bodies and everything they
+// read was written by lower levels, so running a sweep level by level, with a barrier between
+// levels, reads and writes every body value exactly as the serial loop does.
+#include "kernels.hpp"
+
+namespace arsim {
+
+namespace {
+
+constexpr int kSisBlock = 1024;   // one workgroup per scene for levels / buckets / sweeps
+constexpr int kSisWaves = kSisBlock / 64;
+
+__device__ __forceinline__ float dot4(float4 a, float4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+
+// FindRigidBodyCollisionsUniformGridKernel (Kernels/RB_UniformGridKernels.cu:730-845): per sorted
+// particle, the 27 cells in z, y, x order, each cell's particles in sorted order; a candidate of
+// another body whose penetration 2r - |p - q| exceeds the best so far becomes the contact.
+//  REF: reference layout -- cellStart / cellEnd (0xffffffff = empty), bodies read through
+//       rbIndices[gridParticleIndex[j]], the best starts at contactDistance[orig] and the outputs are
+//       written only when a contact is found (color set to (1, 0, 0, 0) then, if given);
+//  engine: the exact per-cell offset table (cell c = [cs[c], cs[c + 1])), the body index in the bits
+//       of sortedPos.w, the scene id in sortedVel.w when there are several scenes; the best starts at
+//       0 and all three outputs are written (-1 partners when there is no contact).
+template <bool REF>
+__global__ void k_sis_find(const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
+                           const unsigned *__restrict__ sIdx, const int *__restrict__ rbIdx,
+                           const unsigned *__restrict__ cs, const unsigned *__restrict__ ce, SceneView sv,
+                           int sceneInVelW, int n, float thr, float *__restrict__ dist, int *__restrict__ pBody,
+                           int *__restrict__ pPart, float4 *__restrict__ color) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 P = sPos[i];
+    const unsigned orig = sIdx[i];
+    const int ra = REF ? rbIdx[orig] : __float_as_int(P.w);
+    const int s = (!REF && sceneInVelW) ? __float_as_int(sVel[i].w) : 0;
+    SceneGeom g;
+    load_geom(sv, s, g);
+    const f3 pa = xyz(P);
+    int gx, gy, gz;
+    grid_pos(pa, g, gx, gy, gz);
+    float best = REF ? dist[orig] : 0.0f;
+    int bj = -1;
+    for (int z = -1; z <= 1; ++z)
+        for (int y = -1; y <= 1; ++y)
+            for (int x = -1; x <= 1; ++x) {
+                const unsigned h = g.cellBase + grid_hash(gx + x, gy + y, gz + z, g);
+                unsigned j = cs[h], e;
+                if (REF) {
+                    if (j == kEmpty) continue;
+                    e = ce[h];
+                } else {
+                    e = cs[h + 1];
+                }
+                for (; j < e; ++j) {
+                    const float4 Q = sPos[j];
+                    const int rq = REF ? rbIdx[sIdx[j]] : __float_as_int(Q.w);
+                    if ((int)j == i || ra == rq) continue;
+                    const f3 d = sub3(pa, xyz(Q));
+                    const float len = sqrtf(dot3(d, d));
+                    if (thr - len > best) {
+                        best = thr - len;
+                        bj = (int)j;
+                    }
+                }
+            }
+    if (bj >= 0) {
+        const unsigned o2 = sIdx[bj];
+        dist[orig] = best;
+        pPart[orig] = (int)o2;
+        pBody[orig] = REF ? rbIdx[o2] : __float_as_int(sPos[bj].w);
+        if (REF && color) color[orig] = make_float4(1.f, 0.f, 0.f, 0.f);
+    } else if (!REF) {
+        dist[orig] = 0.0f;
+        pPart[orig] = -1;
+        pBody[orig] = -1;
+    }
+}
+
+// FindAugmentedRealityCollisionsUniformGridKernel (RB_UniformGridKernels.cu:879-1022): the same search
+// over the static points of one scene.
+//  REF: static cellStart / cellEnd, the partner is the static point's original index
+//       (staticSortedIndex[j]);
+//  engine: the static points' exact cell table; the partner is the sorted index j (the engine keeps
+//       the static normals in the same sorted order).
+template <bool REF>
+__global__ void k_sis_find_static(const float4 *__restrict__ sPos, const unsigned *__restrict__ sIdx, int n,
+                                  const float4 *__restrict__ stPos, const unsigned *__restrict__ stIdx,
+                                  const unsigned *__restrict__ cs, const unsigned *__restrict__ ce, SceneGeom g,
+                                  float thr, float *__restrict__ dist, int *__restrict__ partner,
+                                  float4 *__restrict__ color) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 pa = xyz(sPos[i]);
+    const unsigned orig = sIdx[i];
+    int gx, gy, gz;
+    grid_pos(pa, g, gx, gy, gz);
+    float best = REF ? dist[orig] : 0.0f;
+    int bj = -1;
+    for (int z = -1; z <= 1; ++z)
+        for (int y = -1; y <= 1; ++y)
+            for (int x = -1; x <= 1; ++x) {
+                const unsigned h = grid_hash(gx + x, gy + y, gz + z, g);
+                unsigned j = cs[h], e;
+                if (REF) {
+                    if (j == kEmpty) continue;
+                    e = ce[h];
+                } else {
+                    e = cs[h + 1];
+                }
+                for (; j < e; ++j) {
+                    const f3 d = sub3(pa, xyz(stPos[j]));
+                    const float len = sqrtf(dot3(d, d));
+                    if (thr - len > best) {
+                        best = thr - len;
+                        bj = (int)j;
+                    }
+                }
+            }
+    if (bj >= 0) {
+        dist[orig] = best;
+        partner[orig] = REF ? (int)stIdx[bj] : bj;
+        if (REF && color) color[orig] = make_float4(1.f, 0.f, 0.f, 0.f);
+    } else if (!REF) {
+        dist[orig] = 0.0f;
+        partner[orig] = -1;
+    }
+}
+
+__device__ __forceinline__ bool flag_static(const SisGatherIn &in, int i) {
+    return in.distA != nullptr && in.distA[i] > 0.0f;
+}
+__device__ __forceinline__ bool flag_body(const SisGatherIn &in, int i) {
+    return in.distB[i] > 0.0f && in.bodyB[i] >= 0;
+}
+
+// Block-wide exclusive rank of a 0/1 flag (kSisBlock threads) and the block's total.
+__device__ __forceinline__ unsigned block_rank(bool f, unsigned *sWave, unsigned &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(f);
+    const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+    const unsigned inWave = (unsigned)__popcll(below);
+    if (lane == 0) sWave[wave] = (unsigned)__popcll(m);
+    __syncthreads();
+    unsigned before = 0u, tot = 0u;
+    for (int w = 0; w < kSisWaves; ++w) {
+        const unsigned c = sWave[w];
+        before += w < wave ? c : 0u;
+        tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return before + inWave;
+}
+
+// Contact counts of each block of kSisBlock body particles: static contacts and body contacts.
+__global__ __launch_bounds__(kSisBlock) void k_sis_count(SisGatherIn in, int nb, unsigned *__restrict__ blockCnt) {
+    __shared__ unsigned sWave[kSisWaves];
+    const int i = blockIdx.x * kSisBlock + threadIdx.x;
+    const bool ok = i < nb;
+    unsigned ta, tb;
+    block_rank(ok && flag_static(in, i), sWave, ta);
+    block_rank(ok && flag_body(in, i), sWave, tb);
+    if (threadIdx.x == 0) {
+        blockCnt[2 * blockIdx.x] = ta;
+        blockCnt[2 * blockIdx.x + 1] = tb;
+    }
+}
+
+// Exclusive offsets of the blocks' contacts: static contacts first (all blocks), then body contacts;
+// the total; and, for one scene, its contact range.  One workgroup.
+__global__ __launch_bounds__(kSisBlock) void k_sis_scan(unsigned *__restrict__ blockCnt, int numBlocks,
+                                                        int *__restrict__ numContacts, int2 *__restrict__ sceneRange,
+                                                        int singleScene) {
+    __shared__ unsigned sWave[kSisWaves];
+    __shared__ unsigned sCarry[2];
+    if (threadIdx.x == 0) sCarry[0] = sCarry[1] = 0u;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int part = 0; part < 2; ++part) {
+        for (int base = 0; base < numBlocks; base += kSisBlock) {
+            const int b = base + (int)threadIdx.x;
+            const unsigned v = b < numBlocks ? blockCnt[2 * b + part] : 0u;
+            // inclusive wave scan
+            unsigned x = v;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const unsigned y = __shfl_up(x, d, 64);
+                if (lane >= d) x += y;
+            }
+            if (lane == 63) sWave[wave] = x;
+            __syncthreads();
+            unsigned before = 0u, tot = 0u;
+            for (int w = 0; w < kSisWaves; ++w) {
+                before += w < wave ? sWave[w] : 0u;
+                tot += sWave[w];
+            }
+            const unsigned carry = sCarry[part];
+            if (b < numBlocks) blockCnt[2 * b + part] = carry + before + x - v;
+            __syncthreads();
+            if (threadIdx.x == 0) sCarry[part] = carry + tot;
+            __syncthreads();
+        }
+    }
+    // body contacts follow every static contact
+    const unsigned ta = sCarry[0];
+    for (int b = threadIdx.x; b < numBlocks; b += kSisBlock) blockCnt[2 * b + 1] += ta;
+    if (threadIdx.x == 0) {
+        numContacts[0] = (int)(ta + sCarry[1]);
+        if (singleScene) sceneRange[0] = make_int2(0, (int)(ta + sCarry[1]));
+    }
+}
+
+// GatherAugmentedRealityCollisions (Particles/updateFunctions.cpp:1521-1578) and
+// GatherRigidBodyCollisions (:1431-1519) with findExactContactPoint (Particles/CPU_Engine.cpp:461-473).
+//  static contact: normal = the static point's normal, point = relativePos + CM,
+//    bias = ARrestitution * (v + w x r) . n;
+//  body contact: p1 = CM1 + r1, p2 = CM2 + r2 (float4), point = p1 + (p2 - p1) * (r / (r + r)),
+//    normal = normalize(p2 - p1) * -1 with normalize(v) = v * (1 / sqrt(v . v)) over four components,
+//    bias = RBrestitution * ((v1 + w1 x r1) . n - (v2 + w2 x r2) . n).
+// With several scenes (no static set), the first and last body particle of each scene record the
+// scene's contact range.
+__global__ __launch_bounds__(kSisBlock) void k_sis_emit(SisGatherIn in, int nb, const unsigned *__restrict__ blockCnt,
+                                                        SisContacts out, int2 *__restrict__ sceneRange) {
+    __shared__ unsigned sWave[kSisWaves];
+    const int i = blockIdx.x * kSisBlock + threadIdx.x;
+    const bool ok = i < nb;
+    const bool fa = ok && flag_static(in, i);
+    const bool fb = ok && flag_body(in, i);
+    unsigned ta, tb;
+    const unsigned ra = block_rank(fa, sWave, ta);
+    const unsigned rb = block_rank(fb, sWave, tb);
+    if (!ok) return;
+    const unsigned kb = blockCnt[2 * blockIdx.x + 1] + rb;
+    if (in.scene) {
+        const int s = in.scene[i];
+        if (i == 0 || in.scene[i - 1] != s) sceneRange[s].x = (int)kb;
+        if (i == nb - 1 || in.scene[i + 1] != s) sceneRange[s].y = (int)kb + (fb ? 1 : 0);
+    }
+    if (!fa && !fb) return;
+    const int b = in.rb[i];
+    const float4 cm = in.rbPos[b], r4 = in.relPos[i];
+    const f3 v1 = xyz(in.rbVel[b]), w1 = xyz(in.rbAng[b]), r1 = xyz(r4);
+    if (fa) {
+        const unsigned k = blockCnt[2 * blockIdx.x] + ra;
+        const float4 n = in.staticNorm[in.partA[i]];
+        const float4 pt = make_float4(r4.x + cm.x, r4.y + cm.y, r4.z + cm.z, r4.w + cm.w);
+        const float vrel = dot3(add3(v1, cross3(w1, r1)), xyz(n));
+        out.normal[k] = n;
+        out.point[k] = pt;
+        out.bodies[k] = make_int2(b, -1);
+        out.bias[k] = in.arRest * vrel;
+        out.impulse[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (fb) {
+        const int b2 = in.bodyB[i], j = in.partB[i];
+        const float4 cm2 = in.rbPos[b2], r24 = in.relPos[j];
+        const float4 p1 = make_float4(cm.x + r4.x, cm.y + r4.y, cm.z + r4.z, cm.w + r4.w);
+        const float4 p2 = make_float4(cm2.x + r24.x, cm2.y + r24.y, cm2.z + r24.z, cm2.w + r24.w);
+        const float4 d = make_float4(p2.x - p1.x, p2.y - p1.y, p2.z - p1.z, p2.w - p1.w);
+        const float inv = 1.0f / sqrtf(dot4(d, d));
+        const float t = in.radius / (in.radius + in.radius);
+        const float4 pt = make_float4(p1.x + d.x * t, p1.y + d.y * t, p1.z + d.z * t, p1.w + d.w * t);
+        const float4 n = make_float4((d.x * inv) * -1.0f, (d.y * inv) * -1.0f, (d.z * inv) * -1.0f,
+                                     (d.w * inv) * -1.0f);
+        const f3 n3 = xyz(n);
+        const float a = dot3(add3(v1, cross3(w1, r1)), n3);
+        const float c = dot3(add3(xyz(in.rbVel[b2]), cross3(xyz(in.rbAng[b2]), xyz(r24))), n3);
+        out.normal[kb] = n;
+        out.point[kb] = pt;
+        out.bodies[kb] = make_int2(b, b2);
+        out.bias[kb] = in.rbRest * (a - c);
+        out.impulse[kb] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+__device__ __forceinline__ int2 scene_range(const int2 *sceneRange, const int *numContacts, int s) {
+    return sceneRange ? sceneRange[s] : make_int2(0, numContacts[0]);
+}
+
+// Entries of the predecessor search: entry e = 2c + k names body k of contact c (k = 1 only for
+// body contacts); the key is the body, or numBodies for an unused entry.  Sorted stably by key,
+// the entries of one body come in contact order.
+__global__ void k_sis_entries(const int2 *__restrict__ bodies, const int *__restrict__ numContacts, int cap,
+                              unsigned invalid, unsigned *__restrict__ keys, int *__restrict__ pred) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2 * cap) return;
+    const int c = e >> 1, k = e & 1;
+    unsigned key = invalid;
+    if (c < numContacts[0]) {
+        const int2 b = bodies[c];
+        const int body = k ? b.y : b.x;
+        if (body >= 0) key = (unsigned)body;
+    }
+    keys[e] = key;
+    pred[e] = -1;
+}
+
+// pred[e] = the contact of the previous entry with the same body (-1 for the first).
+__global__ void k_sis_pred(const unsigned *__restrict__ keys, const unsigned *__restrict__ vals, int n,
+                           unsigned invalid, int *__restrict__ pred) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned key = keys[i];
+    if (key == invalid) return;
+    if (i > 0 && keys[i - 1] == key) pred[vals[i]] = (int)(vals[i - 1] >> 1);
+}
+
+__device__ __forceinline__ int block_max(int v, int *sRed) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+    if (lane == 0) sRed[wave] = v;
+    __syncthreads();
+    int m = sRed[0];
+    for (int w = 1; w < kSisWaves; ++w) m = max(m, sRed[w]);
+    __syncthreads();
+    return m;
+}
+
+// Levels of one scene's contacts (one workgroup per scene), chunk by chunk in contact order:
+// a predecessor before the chunk has its final level already; within the chunk the levels are
+// relaxed (level = 1 + max over predecessors) until nothing changes, which takes as many passes
+// as the longest dependency chain inside the chunk.
+__global__ __launch_bounds__(kSisBlock) void k_sis_levels(const int *__restrict__ pred,
+                                                          const int2 *__restrict__ sceneRange,
+                                                          const int *__restrict__ numContacts, int *__restrict__ lvl,
+                                                          int *__restrict__ maxLvl) {
+    __shared__ int sL[kSisBlock];
+    __shared__ int sRed[kSisWaves];
+    const int2 rg = scene_range(sceneRange, numContacts, blockIdx.x);
+    int mx = -1;
+    for (int c0 = rg.x; c0 < rg.y; c0 += kSisBlock) {
+        const int c = c0 + (int)threadIdx.x;
+        const bool ok = c < rg.y;
+        int p1 = -1, p2 = -1, base = 0;
+        if (ok) {
+            p1 = pred[2 * c];
+            p2 = pred[2 * c + 1];
+            if (p1 >= 0 && p1 < c0) base = max(base, lvl[p1] + 1);
+            if (p2 >= 0 && p2 < c0) base = max(base, lvl[p2] + 1);
+        }
+        const bool in1 = p1 >= c0, in2 = p2 >= c0;
+        sL[threadIdx.x] = base;
+        __syncthreads();
+        while (true) {
+            int v = base;
+            if (in1) v = max(v, sL[p1 - c0] + 1);
+            if (in2) v = max(v, sL[p2 - c0] + 1);
+            const bool changed = v != sL[threadIdx.x];
+            __syncthreads();
+            if (changed) sL[threadIdx.x] = v;
+            if (!__syncthreads_or(changed)) break;
+        }
+        if (ok) {
+            lvl[c] = sL[threadIdx.x];
+            mx = max(mx, sL[threadIdx.x]);
+        }
+        __syncthreads();
+    }
+    mx = block_max(mx, sRed);
+    if (threadIdx.x == 0) maxLvl[blockIdx.x] = mx;
+}
+
+// Contacts of one scene grouped by level: off[L] = first position of level L in order[], for
+// L = 0 .. maxLvl + 1; the scene's offsets start at off + range.x + scene.
+__global__ __launch_bounds__(kSisBlock) void k_sis_bucket(const int *__restrict__ lvl,
+                                                          const int2 *__restrict__ sceneRange,
+                                                          const int *__restrict__ numContacts,
+                                                          const int *__restrict__ maxLvl, int *__restrict__ offAll,
+                                                          int *__restrict__ curAll, int *__restrict__ order) {
+    __shared__ unsigned sWave[kSisWaves];
+    __shared__ int sCarry;
+    const int s = blockIdx.x;
+    const int2 rg = scene_range(sceneRange, numContacts, s);
+    const int levels = maxLvl[s] + 1;
+    int *off = offAll + rg.x + s;
+    int *cur = curAll + rg.x + s;
+    for (int L = threadIdx.x; L <= levels; L += kSisBlock) off[L] = 0;
+    __syncthreads();
+    for (int c = rg.x + (int)threadIdx.x; c < rg.y; c += kSisBlock) atomicAdd(&off[lvl[c] + 1], 1);
+    __syncthreads();
+    // inclusive scan of off[0 .. levels] (off[0] = 0): off[L] = contacts below level L
+    if (threadIdx.x == 0) sCarry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int base = 0; base <= levels; base += kSisBlock) {
+        const int L = base + (int)threadIdx.x;
+        const unsigned v = L <= levels ? (unsigned)off[L] : 0u;
+        unsigned x = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) sWave[wave] = x;
+        __syncthreads();
+        unsigned before = 0u, tot = 0u;
+        for (int w = 0; w < kSisWaves; ++w) {
+            before += w < wave ? sWave[w] : 0u;
+            tot += sWave[w];
+        }
+        const int carry = sCarry;
+        if (L <= levels) {
+            off[L] = carry + (int)(before + x);
+            cur[L] = carry + (int)(before + x);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) sCarry = carry + (int)tot;
+        __syncthreads();
+    }
+    for (int c = rg.x + (int)threadIdx.x; c < rg.y; c += kSisBlock) order[rg.x + atomicAdd(&cur[lvl[c]], 1)] = c;
+}
+
+__device__ __forceinline__ void ld_mat(const float *__restrict__ Iw, int b, float m[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = Iw[9 * b + k];
+}
+// (I^-1 (p x d)) x p . d
+__device__ __forceinline__ float sis_k(const float I[9], f3 p, f3 d) {
+    return dot3(cross3(mat_vec(I, cross3(p, d)), p), d);
+}
+// glm::normalize(vec3) of GLM 0.9.7: v * (1 / sqrt(v . v)).
+__device__ __forceinline__ f3 gnormalize(f3 v) {
+    const float inv = 1.0f / sqrtf(dot3(v, v));
+    return f3{v.x * inv, v.y * inv, v.z * inv};
+}
+__device__ __forceinline__ f3 divs3(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+
+// One contact of one sweep: SequentialImpulseSolver (Particles/updateFunctions.cpp:1580-1874).
+//  static contact (:1603-1713): normal impulse accumulated and clamped to [0, 100]; friction along
+//    t = normalize((1,0,0) - ((1,0,0) . n) n) (or (0,1,0) when n.x is within 1e-4 of +-1) and
+//    t2 = normalize(t x n), each accumulated and clamped to +-ARfriction * mc * |g.y|;
+//    v += J / m, w += I^-1 (p x J), J = jn n + jt t + jt2 t2;
+//  body contact (:1714-1860): the normal impulse with both bodies' terms, applied with opposite
+//    signs (its friction impulses are not applied there, so they are not formed here).
+// Velocities are stored with w = 0.
+__device__ void sis_contact(int c, const SisContacts &ct, const SisBodies &bd, const SisSolvePhys &ph) {
+    const float4 n4 = ct.normal[c], pt = ct.point[c];
+    const int2 bb = ct.bodies[c];
+    const float bias = ct.bias[c];
+    float4 acc = ct.impulse[c];
+    const f3 n = xyz(n4);
+    const float upper = 100.0f;
+    if (bb.y < 0) {
+        const int b = bb.x;
+        const float4 cm = bd.pos[b];
+        const f3 p = f3{pt.x - cm.x, pt.y - cm.y, pt.z - cm.z};
+        float I[9];
+        ld_mat(bd.Iw, b, I);
+        const float m = bd.mass[b];
+        f3 v = xyz(bd.vel[b]), w = xyz(bd.ang[b]);
+        float mc = 1.0f / m + sis_k(I, p, n);
+        if (fabs((double)mc) < 0.00001) mc = 1.0f;
+        const float vrel = dot3(add3(v, cross3(w, p)), n);
+        float ci = -(vrel + bias) / mc;
+        float tmp = acc.x + ci;
+        if (tmp < 0.0f) tmp = 0.0f;
+        else if (tmp > upper) tmp = upper;
+        ci = tmp - acc.x;
+        acc.x = tmp;
+        const float fb = ph.arFric * mc * fabsf(ph.gy);
+        const f3 vel = add3(v, cross3(w, p));
+        f3 t = f3{1.0f, 0.0f, 0.0f};
+        if (fabs((double)(t.x - n.x)) < 0.0001) t = f3{0.0f, 1.0f, 0.0f};
+        else if (fabs((double)(t.x + n.x)) < 0.0001) t = f3{0.0f, 1.0f, 0.0f};
+        t = sub3(t, scl3(dot3(t, n), n));
+        t = gnormalize(t);
+        const float kt = 1.0f / m + sis_k(I, p, t);
+        float cf = -dot3(vel, t) / kt;
+        float tf = acc.y + cf;
+        if (tf < -fb) tf = -fb;
+        else if (tf > fb) tf = fb;
+        cf = tf - acc.y;
+        acc.y = tf;
+        const f3 t2 = gnormalize(cross3(t, n));
+        const float kt2 = 1.0f / m + sis_k(I, p, t2);
+        float cf2 = -dot3(vel, t2) / kt2;
+        float tf2 = acc.z + cf2;
+        if (tf2 < -fb) tf2 = -fb;
+        else if (tf2 > fb) tf2 = fb;
+        cf2 = tf2 - acc.z;
+        acc.z = tf2;
+        const f3 J = add3(add3(scl3(ci, n), scl3(cf, t)), scl3(cf2, t2));
+        v = add3(v, divs3(J, m));
+        w = add3(w, mat_vec(I, cross3(p, J)));
+        bd.vel[b] = f4(v, 0.0f);
+        bd.ang[b] = f4(w, 0.0f);
+    } else {
+        const int b1 = bb.x, b2 = bb.y;
+        const float4 c1 = bd.pos[b1], c2 = bd.pos[b2];
+        const f3 p1 = f3{pt.x - c1.x, pt.y - c1.y, pt.z - c1.z};
+        const f3 p2 = f3{pt.x - c2.x, pt.y - c2.y, pt.z - c2.z};
+        float I1[9], I2[9];
+        ld_mat(bd.Iw, b1, I1);
+        ld_mat(bd.Iw, b2, I2);
+        const float m1 = bd.mass[b1], m2 = bd.mass[b2];
+        f3 v1 = xyz(bd.vel[b1]), w1 = xyz(bd.ang[b1]), v2 = xyz(bd.vel[b2]), w2 = xyz(bd.ang[b2]);
+        float mc = 1.0f / m1 + 1.0f / m2 + sis_k(I1, p1, n) + sis_k(I2, p2, n);
+        if (fabs((double)mc) < 0.00001) mc = 1.0f;
+        const float vrel = dot3(add3(v1, cross3(w1, p1)), n) - dot3(add3(v2, cross3(w2, p2)), n);
+        float ci = -(vrel + bias) / mc;
+        float tmp = acc.x + ci;
+        if (tmp < 0.0f) tmp = 0.0f;
+        else if (tmp > upper) tmp = upper;
+        ci = tmp - acc.x;
+        acc.x = tmp;
+        const f3 J = scl3(ci, n);
+        v1 = add3(v1, divs3(J, m1));
+        w1 = add3(w1, mat_vec(I1, cross3(p1, J)));
+        bd.vel[b1] = f4(v1, 0.0f);
+        bd.ang[b1] = f4(w1, 0.0f);
+        v2 = sub3(v2, divs3(J, m2));
+        w2 = sub3(w2, mat_vec(I2, cross3(p2, J)));
+        bd.vel[b2] = f4(v2, 0.0f);
+        bd.ang[b2] = f4(w2, 0.0f);
+    }
+    ct.impulse[c] = acc;
+}
+
+// The sweeps of one scene (one workgroup per scene): level by level, a barrier between levels.
+__global__ __launch_bounds__(kSisBlock) void k_sis_solve(SisContacts ct, SisBodies bd, SisSolvePhys ph,
+                                                         const int2 *__restrict__ sceneRange,
+                                                         const int *__restrict__ numContacts,
+                                                         const int *__restrict__ maxLvl,
+                                                         const int *__restrict__ offAll,
+                                                         const int *__restrict__ order, int iterations) {
+    const int s = blockIdx.x;
+    const int2 rg = scene_range(sceneRange, numContacts, s);
+    const int levels = maxLvl[s] + 1;
+    const int *off = offAll + rg.x + s;
+    for (int it = 0; it < iterations; ++it)
+        for (int L = 0; L < levels; ++L) {
+            const int a = off[L], e = off[L + 1];
+            for (int k = a + (int)threadIdx.x; k < e; k += kSisBlock) sis_contact(order[rg.x + k], ct, bd, ph);
+            __syncthreads();
+        }
+}
+
+}  // namespace
+
+void launch_sis_find(const float4 *sPos, const float4 *sVel, const unsigned *sIdx, const unsigned *segStart,
+                     const SceneView &sv, bool sceneInVelW, int n, float radius, float *dist, int *pBody, int *pPart,
+                     hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_sis_find<false>, dim3((n + 127) / 128), dim3(128), 0, st, sPos, sVel, sIdx, nullptr,
+                       segStart, nullptr, sv, sceneInVelW ? 1 : 0, n, 2.0f * radius, dist, pBody, pPart, nullptr);
+}
+
+void launch_sis_find_ref(const float4 *sPos, const unsigned *sIdx, const int *rbIdx, const unsigned *cellStart,
+                         const unsigned *cellEnd, const SceneGeom &g, int n, float radius, float *dist, int *pBody,
+                         int *pPart, float4 *color, hipStream_t st) {
+    if (n <= 0) return;
+    SceneView sv{};
+    sv.count = 1;
+    sv.single = g;
+    hipLaunchKernelGGL(k_sis_find<true>, dim3((n + 127) / 128), dim3(128), 0, st, sPos, nullptr, sIdx, rbIdx,
+                       cellStart, cellEnd, sv, 0, n, 2.0f * radius, dist, pBody, pPart, color);
+}
+
+void launch_sis_find_static(const float4 *sPos, const unsigned *sIdx, int n, const float4 *stPos,
+                            const unsigned *stIdx, const unsigned *cellStart, const unsigned *cellEnd,
+                            const SceneGeom &g, float radius, bool refLayout, float *dist, int *partner,
+                            float4 *color, hipStream_t st) {
+    if (n <= 0) return;
+    const dim3 grid((n + 127) / 128);
+    if (refLayout)
+        hipLaunchKernelGGL(k_sis_find_static<true>, grid, dim3(128), 0, st, sPos, sIdx, n, stPos, stIdx, cellStart,
+                           cellEnd, g, 2.0f * radius, dist, partner, color);
+    else
+        hipLaunchKernelGGL(k_sis_find_static<false>, grid, dim3(128), 0, st, sPos, sIdx, n, stPos, stIdx, cellStart,
+                           cellEnd, g, 2.0f * radius, dist, partner, color);
+}
+
+size_t sis_workspace_bytes(int cap, int numScenes, int numBodyParticles) {
+    const SisWork w = sis_work_layout(nullptr, cap, numScenes, numBodyParticles);
+    return (size_t)w.end;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+SisWork sis_work_layout(void *base, int cap, int numScenes, int numBodyParticles) {
+    SisWork w{};
+    char *p = (char *)base;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return p ? (void *)(p + at) : nullptr;
+    };
+    const size_t ne = 2 * (size_t)cap;
+    const int nblk = (numBodyParticles + kSisBlock - 1) / kSisBlock;
+    w.keysA = (unsigned *)take(ne * 4);
+    w.keysB = (unsigned *)take(ne * 4);
+    w.valsA = (unsigned *)take(ne * 4);
+    w.valsB = (unsigned *)take(ne * 4);
+    w.sortWsBytes = sort_workspace_bytes((int)ne, 4);
+    w.sortWs = take(w.sortWsBytes);
+    w.pred = (int *)take(ne * 4);
+    w.lvl = (int *)take((size_t)cap * 4);
+    w.order = (int *)take((size_t)cap * 4);
+    w.off = (int *)take(((size_t)cap + numScenes + 1) * 4);
+    w.cur = (int *)take(((size_t)cap + numScenes + 1) * 4);
+    w.maxLvl = (int *)take((size_t)numScenes * 4);
+    w.sceneRange = (int2 *)take((size_t)numScenes * 8);
+    w.blockCnt = (unsigned *)take(2 * (size_t)(nblk + 1) * 4);
+    w.numContacts = (int *)take(4);
+    w.end = o;
+    return w;
+}
+
+void launch_sis_gather(const SisGatherIn &in, int nb, const SisContacts &out, const SisWork &w, bool perScene,
+                       hipStream_t st) {
+    const int nblk = (nb + kSisBlock - 1) / kSisBlock;
+    if (nblk > 0) hipLaunchKernelGGL(k_sis_count, dim3(nblk), dim3(kSisBlock), 0, st, in, nb, w.blockCnt);
+    hipLaunchKernelGGL(k_sis_scan, dim3(1), dim3(kSisBlock), 0, st, w.blockCnt, nblk, w.numContacts, w.sceneRange,
+                       perScene ? 0 : 1);
+    if (nblk > 0)
+        hipLaunchKernelGGL(k_sis_emit, dim3(nblk), dim3(kSisBlock), 0, st, in, nb, w.blockCnt, out,
+                           perScene ? w.sceneRange : nullptr);
+}
+
+int launch_sis_solve(const SisContacts &ct, const SisBodies &bd, const SisSolvePhys &ph, int cap, int numScenes,
+                     int numBodies, int iterations, const SisWork &w, const SortTuning &tu, hipStream_t st) {
+    if (cap <= 0 || numScenes <= 0) return 0;
+    const int ne = 2 * cap;
+    const unsigned invalid = (unsigned)numBodies;
+    int bits = 1;
+    while (bits < 32 && (invalid >> bits) != 0u) ++bits;
+    hipLaunchKernelGGL(k_sis_entries, dim3((ne + 255) / 256), dim3(256), 0, st, ct.bodies, w.numContacts, cap,
+                       invalid, w.keysA, w.pred);
+    unsigned *kA = w.keysA, *kB = w.keysB, *vA = w.valsA, *vB = w.valsB;
+    if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, ne, 0, bits, true, w.sortWs, w.sortWsBytes, tu, st)) return 1;
+    hipLaunchKernelGGL(k_sis_pred, dim3((ne + 255) / 256), dim3(256), 0, st, kA, vA, ne, invalid, w.pred);
+    const int2 *rg = numScenes > 1 ? w.sceneRange : nullptr;
+    hipLaunchKernelGGL(k_sis_levels, dim3(numScenes), dim3(kSisBlock), 0, st, w.pred, rg, w.numContacts, w.lvl,
+                       w.maxLvl);
+    hipLaunchKernelGGL(k_sis_bucket, dim3(numScenes), dim3(kSisBlock), 0, st, w.lvl, rg, w.numContacts, w.maxLvl,
+                       w.off, w.cur, w.order);
+    if (iterations > 0)
+        hipLaunchKernelGGL(k_sis_solve, dim3(numScenes), dim3(kSisBlock), 0, st, ct, bd, ph, rg, w.numContacts,
+                           w.maxLvl, w.off, w.order, iterations);
+    return 0;
+}
+
+}  // namespace arsim

@@ -305,6 +305,56 @@ int arsim_dem_static_grid(float *pForce, float *pTorque, const float *sortedPos,
                           int numParticles, const ArSimParams *params, const ArSimStepConfig *cfg,
                           arsim_stream_t stream);
 
+/* Sequential-impulse contact model (the step of updateUniformGridSIS, RigidBodies/UniformGridUpdate.cpp:734-816).
+ * FindRigidBodyCollisionsUniformGridWrapper (RigidBodies/rigidBodyKernelWrappers.h:220-232,
+ * Kernels/RB_UniformGridKernels.cu:730-876): per sorted particle, over the 27 neighbour cells in z, y, x
+ * order and each cell's particles in sorted order, the candidate of another body with the largest
+ * penetration 2r - |p - q| above contactDistance[orig] (cleared to 0 by the caller, as
+ * UniformGridUpdate.cpp:412 does).  Writes contactDistance / collidingParticleIndex (original index) /
+ * collidingRigidBodyIndex only where a contact is found, and color[orig] = (1,0,0,0) then (color may
+ * be NULL). */
+int arsim_find_deepest_contacts_grid(const int *rbIndices, int *collidingRigidBodyIndex, int *collidingParticleIndex,
+                                     float *contactDistance, float *color, const float *sortedPos,
+                                     const unsigned int *gridParticleIndex, const unsigned int *cellStart,
+                                     const unsigned int *cellEnd, int numParticles, const ArSimParams *params,
+                                     arsim_stream_t stream);
+/* FindAugmentedRealityCollisionsUniformGridWrapper (RB_UniformGridKernels.cu:879-1108): the same search
+ * over the static points (sorted by cell, static cellStart / cellEnd); collidingParticleIndex receives
+ * the static point's original index (staticSortedIndex[j]). */
+int arsim_find_static_contacts_grid(int *collidingParticleIndex, float *contactDistance, float *color,
+                                    const float *sortedPos, const float *staticSortedPos,
+                                    const unsigned int *gridParticleIndex, const unsigned int *staticSortedIndex,
+                                    const unsigned int *staticCellStart, const unsigned int *staticCellEnd,
+                                    int numParticles, const ArSimParams *params, arsim_stream_t stream);
+/* Device scratch of arsim_sis_gather and arsim_sis_solve for maxContacts contacts. */
+size_t arsim_sis_workspace_bytes(int maxContacts, int numBodyParticles);
+/* GatherAugmentedRealityCollisions + GatherRigidBodyCollisions (Particles/updateFunctions.cpp:1431-1578)
+ * on the device: for every body particle i < numBodyParticles (bodies' particles come first, in body
+ * order) with a static contact (staticContactDistance[i] > 0; the three static arrays may be NULL) and
+ * then for every one with a body contact (contactDistance[i] > 0 and a partner body >= 0), one contact:
+ * contactNormal / contactPoint (float4), contactBodies (b1, b2; b2 = -1 static), contactBias, and
+ * contactImpulse (4 floats, cleared: normal, friction, friction 2 accumulators).  Static contacts come
+ * first.  The count goes to the device int *numContacts; maxContacts must cover numBodyParticles
+ * (twice that with static contacts).  A partner that is a grain has no body state and is skipped. */
+int arsim_sis_gather(const float *rbPositions, const float *rbVelocities, const float *rbAngularVelocity,
+                     const float *relativePos, const int *rbIndices, int numBodyParticles,
+                     const float *staticContactDistance, const int *staticContactIndex, const float *staticNorm,
+                     const float *contactDistance, const int *collidingRigidBodyIndex,
+                     const int *collidingParticleIndex, float *contactNormal, float *contactPoint,
+                     int *contactBodies, float *contactBias, float *contactImpulse, int *numContacts,
+                     int maxContacts, const ArSimParams *params, void *workspace, size_t workspaceBytes,
+                     arsim_stream_t stream);
+/* SequentialImpulseSolver (Particles/updateFunctions.cpp:1580-1874) on the device: `iterations` (the
+ * reference uses 18) Gauss-Seidel sweeps over the contacts in list order, updating rbVelocities /
+ * rbAngularVelocity (w = 0) and the accumulators, with the same results as the serial loop: contacts
+ * are run level by level, a level holding contacts whose bodies no earlier pending contact touches.
+ * Call arsim_particle_pose afterwards for the particles (computeGlobalAttributesWrapper, :1862-1871). */
+int arsim_sis_solve(float *contactNormal, float *contactPoint, int *contactBodies, float *contactBias,
+                    float *contactImpulse, const int *numContacts, int maxContacts, const float *rbPositions,
+                    float *rbVelocities, float *rbAngularVelocity, const float *rbCurrentInertia, const float *rbMass,
+                    int numBodies, int iterations, const ArSimParams *params, void *workspace,
+                    size_t workspaceBytes, arsim_stream_t stream);
+
 /* Body-index rebasing copy: dst[i] = src[i] + delta where src[i] >= 0, else src[i].  The reference
  * offsets a new body's rbIndices by the current body count when it appends it
  * (RigidBodies/RigidBodyAddition.cpp:111-139); the engine does the same when it moves scenes

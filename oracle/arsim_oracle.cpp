@@ -797,7 +797,7 @@ void sis_find_deepest(int N, const ArSimParams *p, const uint32_t *sIdx, const u
     }
 }
 
-// FindAugmentedRealityCollisionsUniformGridKernel + ...Cell (RB_UniformGridKernels.cu:878-960):
+// FindAugmentedRealityCollisionsUniformGridKernel + ...Cell (RB_UniformGridKernels.cu:879-1022):
 // the same search over the static points (sorted by cell, cellStart / cellEnd of the static grid);
 // the contact records the static point's original index.
 template <class T>
