@@ -55,9 +55,11 @@ def parse():
                     help="run a plain 256 MiB device copy kernel (3 launches) to calibrate HBM byte counters, then exit")
     ap.add_argument("--batched-only", action="store_true",
                     help="only step the 512-scene batch (default tuning, no autotune): for profilers")
-    ap.add_argument("--scene", choices=["small", "pile"], default=None,
+    ap.add_argument("--scene", choices=["small", "pile", "sis"], default=None,
                     help="time another plain scene instead of the headline one: 'small' = 256 cubes (16,384 "
-                         "particles, 64^3 grid), 'pile' = the dense ellipsoid pile (~4.2 M particles)")
+                         "particles, 64^3 grid), 'pile' = the dense ellipsoid pile (~4.2 M particles), 'sis' = the "
+                         "sequential-impulse model on two layers of 32 x 32 cubes resting on a static floor "
+                         "(131,072 particles)")
     return ap.parse_args()
 
 
@@ -65,8 +67,25 @@ def other_scene(args):
     """Plain-path throughput of the small scene or the dense pile (stage times included)."""
     from diploma_thesis_amd import Engine, capi, scenes
     method = {"grid": capi.METHOD_GRID, "lbvh30": capi.METHOD_LBVH30, "lbvh63": capi.METHOD_LBVH63}[args.method]
-    sc = scenes.config0(method) if args.scene == "small" else scenes.config2(method)
+    floor = None
+    if args.scene == "sis":
+        sc, fp, fn = scenes.stacked_cubes(32, 32, 2)
+        floor = (fp, fn)
+    else:
+        sc = scenes.config0(method) if args.scene == "small" else scenes.config2(method)
+    cpu = None
+    if floor is not None and not args.no_cpu_baseline:
+        import oracle
+        st = oracle.State(sc, np.float32)
+        st.set_static(*floor)
+        st.step(1)
+        t0 = time.perf_counter()
+        st.step(5)
+        cpu = {"value": sc.n * 5 / (time.perf_counter() - t0), "unit": "particle-updates/s", "cores": 1,
+               "kind": "port", "sample": "5 steps after 1 untimed step, oracle FP32 serial C++ (g++ -O2)"}
     with Engine(sc, use_graph=True) as e:
+        if floor is not None:
+            e.set_static(*floor)
         e.step(args.warmup)
         e.sync()
         t0 = time.perf_counter()
@@ -80,7 +99,8 @@ def other_scene(args):
         finite = e.check_finite()
     print(json.dumps({"scene": args.scene, "method": args.method, "particles": sc.n, "bodies": sc.b,
                       "steps": args.steps, "updates_per_s": sc.n * args.steps / sec,
-                      "ms_per_step": sec * 1e3 / args.steps, "stage_ms": stage_ms, "finite": finite}))
+                      "ms_per_step": sec * 1e3 / args.steps, "stage_ms": stage_ms, "finite": finite,
+                      "cpu_baseline": cpu}))
 
 
 def batched_only(args):

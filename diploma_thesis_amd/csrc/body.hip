@@ -11,13 +11,13 @@ constexpr int kMaxHistWords = 8 * 256;   // up to 8 digit passes of up to 8 bits
 // GPUintegratorKernel (Kernels/rigidBodyKernels.cu:262-348), with the orientation carried as a
 // cumulative quaternion (q_cum = normalize(q_inc * q_cum)) and I_w^-1 = R_cum I_b^-1 R_cum^T.
 // Blocks past bodyBlocks zero the given ranges (the next sort's histograms, tile counters and
-// first look-back array), which saves the sort a launch of its own.
+// first look-back array, the cell table's long-run count), which saves those a launch of their own.
 __global__ void k_body_integrate(BodyBufs b, int B, DevPhys ph, ZeroJobs z, int bodyBlocks) {
     if ((int)blockIdx.x >= bodyBlocks) {
         const size_t t = (size_t)((int)blockIdx.x - bodyBlocks) * blockDim.x + threadIdx.x;
         const size_t stride = (size_t)((int)gridDim.x - bodyBlocks) * blockDim.x;
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 4; ++k)
             for (size_t e = t; e < z.count[k]; e += stride) z.p[k][e] = 0u;
         return;
     }
@@ -416,7 +416,7 @@ void launch_body_integrate(const BodyBufs &b, int numBodies, const DevPhys &ph, 
     size_t zmax = 0;
     if (zero) {
         z = *zero;
-        for (int k = 0; k < 3; ++k) zmax = z.count[k] > zmax ? z.count[k] : zmax;
+        for (int k = 0; k < 4; ++k) zmax = z.count[k] > zmax ? z.count[k] : zmax;
     }
     int zeroBlocks = zmax ? (int)blocks_for((long long)zmax, block) : 0;
     if (zeroBlocks > 512) zeroBlocks = 512;

@@ -546,7 +546,7 @@ int arsim_sis_solve(float *contactNormal, float *contactPoint, int *contactBodie
                        rbCurrentInertia, rbMass};
     const SisSolvePhys sp{params->ARfriction, params->gravity.y};
     SortTuning stt{};
-    if (launch_sis_solve(ct, bd, sp, maxContacts, 1, numBodies, iterations, w, stt, S(st)))
+    if (launch_sis_solve(ct, bd, sp, maxContacts, 1, numBodies, nullptr, numBodies, iterations, w, stt, S(st)))
         return fail(ARSIM_ERR_INVALID_ARG, "sis_solve: contact sort rejected");
     return check_launch("sis_solve");
 }

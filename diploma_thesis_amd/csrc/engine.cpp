@@ -129,7 +129,8 @@ int Engine::set_static(const float *pos4, const float *norm4, int n) {
     const size_t wsBytes = sort_workspace_bytes(n, 4);
     void *tmp = nullptr;
     const size_t v4 = (size_t)n * 16, v1 = (size_t)n * 4;
-    const size_t tmpBytes = 2 * v4 + 4 * v1 + wsBytes + 256;
+    const unsigned longCapS = fill_long_capacity((unsigned long long)cells + 1ull);
+    const size_t tmpBytes = 2 * v4 + 4 * v1 + wsBytes + 256 + 16 * (size_t)longCapS + 256 + 16;
     ARSIM_HIP(hipMalloc(&tmp, tmpBytes));
     char *c = (char *)tmp;
     float4 *dPos = (float4 *)c; c += v4;
@@ -139,6 +140,9 @@ int Engine::set_static(const float *pos4, const float *norm4, int n) {
     unsigned *vA = (unsigned *)c; c += v1;
     unsigned *vB = (unsigned *)c; c += v1;
     void *ws = (void *)(((uintptr_t)c + 255) & ~(uintptr_t)255);
+    // long empty runs of the static table (its tail, gaps between occupied z slabs): list + count
+    uint4 *longListS = (uint4 *)(((uintptr_t)ws + wsBytes + 255) & ~(uintptr_t)255);
+    unsigned *longCntS = (unsigned *)(longListS + longCapS);
     int rc = ARSIM_OK;
     hipError_t e = hipMalloc(&stPos, v4);
     if (e == hipSuccess) e = hipMalloc(&stNorm, v4);
@@ -146,6 +150,7 @@ int Engine::set_static(const float *pos4, const float *norm4, int n) {
     if (e == hipSuccess) e = hipMemcpy(dPos, pos4, v4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dNorm, norm4, v4, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
+        launch_fill_u32(longCntS, 0u, 1, st);
         launch_static_hash(dPos, n, g, kA, st);
         int bits = 1;
         while ((1ull << bits) < (unsigned long long)cells) ++bits;
@@ -153,7 +158,8 @@ int Engine::set_static(const float *pos4, const float *norm4, int n) {
         if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, n, 0, bits, true, ws, wsBytes, stt, st))
             rc = fail(ARSIM_ERR_INVALID_ARG, "engine_set_static: sort rejected");
         else
-            launch_static_reorder(kA, vA, dPos, dNorm, n, cells, stPos, stNorm, stTable, st);
+            launch_static_reorder(kA, vA, dPos, dNorm, n, cells, stPos, stNorm, stTable,
+                                  LongFill{longCntS, longListS, longCapS}, st);
         e = hipStreamSynchronize(st);
         if (e == hipSuccess) e = hipGetLastError();
     }
@@ -361,6 +367,8 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     if (method == ARSIM_METHOD_GRID) {
         A(segStart, (size_t)numSegs + 1);
         A(sortedPos, N); A(sortedVel, N);
+        longCap = fill_long_capacity((unsigned long long)numSegs + 1ull);
+        A(longCnt, 2); A(longList, (size_t)longCap);
     } else {
         size_t nodes = (size_t)2 * N;
         A(tree.left, nodes); A(tree.right, nodes); A(tree.parent, nodes);
@@ -375,7 +383,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         A(aabbPart, (size_t)2 * aabb_max_partials()); A(aabbBox, 2);
     }
     if (flags & ARSIM_CONTACT_SIS) {
-        A(sisDistA, N); A(sisPartA, N); A(sisDistB, N); A(sisBodyB, N); A(sisPartB, N);
+        A(sisDistA, N); A(sisPartA, N); A(sisDistB, N); A(sisBodyB, N); A(sisPartB, N); A(sceneBodyD, (size_t)S);
         if ((rc = alloc_sis(N)) != 0) return rc;
     }
 #undef A
@@ -470,6 +478,15 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     ARSIM_HIP(hipMemcpyAsync(sceneCountD, hCount.data(), 4 * (size_t)S, hipMemcpyHostToDevice, st));
     if (tree.sceneRoot) launch_fill_u32((unsigned *)tree.sceneRoot, 0xffffffffu, (size_t)S, st);
     launch_fill_u32((unsigned *)overflowD, 0u, 1, st);
+    if (longCnt) launch_fill_u32(longCnt, 0u, 1, st);
+    maxSceneBodies = 0;
+    for (int s = 0; s < S; ++s) maxSceneBodies = std::max(maxSceneBodies, bCount[s]);
+    if (sceneBodyD) {
+        std::vector<int2> sb(S);
+        for (int s = 0; s < S; ++s) sb[s] = make_int2(bStart[s], bCount[s]);
+        ARSIM_HIP(hipMemcpyAsync(sceneBodyD, sb.data(), sizeof(int2) * S, hipMemcpyHostToDevice, st));
+        ARSIM_HIP(hipStreamSynchronize(st));   // sb is a host temporary
+    }
     ARSIM_HIP(hipGetLastError());
     ARSIM_HIP(hipStreamSynchronize(st));
     first = true;
@@ -516,7 +533,9 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         // updateUniformGridSIS (RigidBodies/UniformGridUpdate.cpp:734-797): integrate and pose, the grid
         // broadphase, deepest contacts (static set first), the contact list, the sweeps, the pose again
         const SortPlan pl = sort_plan(sortWs, N, keyBytes, 0, keyBits, stt);
-        const ZeroJobs z = sort_zero_jobs(pl);
+        ZeroJobs z = sort_zero_jobs(pl);
+        z.p[3] = longCnt;   // the cell table's long-run count
+        z.count[3] = 1;
         launch_body_integrate(bb, B, ph, &z, st);
         rec();
         KeyOut ko{keysA, nullptr, nullptr};
@@ -528,7 +547,7 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
             return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
         rec();
         launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart,
-                           segBits, numSegs, N, st);
+                           segBits, numSegs, N, long_fill(), st);
         rec();
         if (numStatic)
             launch_sis_find_static(sortedPos, vA, N, stPos, nullptr, stTable, nullptr, geoms[0], ph.r, false, sisDistA,
@@ -543,7 +562,8 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         rec();
         const SisBodies bd{rbPos, rbVel, rbAng, Iw, mass};
         const SisSolvePhys sp{p0.ARfriction, p0.gravity.y};
-        if (launch_sis_solve(ct, bd, sp, sisCap, S, B, kSisIterations, sisW, stt, st))
+        if (launch_sis_solve(ct, bd, sp, sisCap, S, B, S > 1 ? sceneBodyD : nullptr, maxSceneBodies, kSisIterations,
+                             sisW, stt, st))
             return fail(ARSIM_ERR_INVALID_ARG, "engine: contact sort rejected");
         rec();
         KeyOut none{};
@@ -555,7 +575,9 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         // the sort's workspace is cleared by the body launch and its histograms are built while the
         // keys are generated, so the sort itself is only its digit passes
         const SortPlan pl = sort_plan(sortWs, N, keyBytes, 0, keyBits, stt);
-        const ZeroJobs z = sort_zero_jobs(pl);
+        ZeroJobs z = sort_zero_jobs(pl);
+        z.p[3] = longCnt;   // the cell table's long-run count
+        z.count[3] = 1;
         launch_body_integrate(bb, B, ph, &z, st);
         rec();
         KeyOut ko{keysA, nullptr, nullptr};
@@ -567,7 +589,7 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
             return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
         rec();
         launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart,
-                           segBits, numSegs, N, st);
+                           segBits, numSegs, N, long_fill(), st);
         rec();
         DemOut o{};
         o.pForce = pForce; o.vel = vel; o.relPos = relPos;
@@ -678,8 +700,9 @@ int Engine::enqueue_broadphase(int *pairsD, int *countsD, int maxPer) {
         unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
         if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st))
             return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
+        launch_fill_u32(longCnt, 0u, 1, st);
         launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart,
-                           segBits, numSegs, N, st);
+                           segBits, numSegs, N, long_fill(), st);
         launch_dem_grid_seg(sortedPos, sortedVel, kA, vA, segStart, segBits, N, ph, sv, S > 1, o, tu.demBlock, st);
         prevSorted = kA;
         lastKeys = kA;

@@ -53,6 +53,9 @@ struct Engine {
     unsigned char *keyBufA = nullptr, *keyBufB = nullptr, *sortWsBuf = nullptr;
     unsigned *valsA = nullptr, *valsB = nullptr;
     unsigned *segStart = nullptr;           // grid: CSR offsets over segments of 2^segBits cells
+    unsigned *longCnt = nullptr;            // grid: run count of the long-run table fill (zeroed each step)
+    uint4 *longList = nullptr;
+    unsigned longCap = 0;
     int segBits = 3;
     long long numSegs = 0;
     float4 *sortedPos = nullptr, *sortedVel = nullptr;
@@ -73,6 +76,8 @@ struct Engine {
     int2 *cBodies = nullptr;
     float *cBias = nullptr;
     unsigned char *sisWsBuf = nullptr;
+    int2 *sceneBodyD = nullptr;             // (first body, body count) per scene
+    int maxSceneBodies = 0;
     SisWork sisW{};
 
     // step state
@@ -97,6 +102,7 @@ struct Engine {
     BodyBufs body_bufs() const;
     StateBufs state_bufs() const;
     PartBufs part_bufs() const;
+    LongFill long_fill() const { return LongFill{longCnt, longList, longCap}; }
     size_t scene_bytes_estimate() const;
 
   private:

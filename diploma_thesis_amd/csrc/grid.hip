@@ -106,18 +106,51 @@ __global__ void k_dem_grid(const float4 *__restrict__ sPos, const float4 *__rest
 // segStart[s] = first sorted index whose key >= s << segBits (segStart[numSegs] = n).  Every entry
 // is rewritten each step, so nothing has to be cleared.  With segBits = 0 it is cellStart with
 // cellEnd[c] = segStart[c + 1], one table instead of two.  numSegs < 2^32.
+// The long runs handed over by wave_fill: the table is cut into windows of W entries and each
+// workgroup takes windows blockIdx, blockIdx + gridDim, ...; in each window it fills the part of
+// every listed run that falls inside.  With no runs listed the kernel only reads the count.
+constexpr unsigned long long kFillWindow = 65536ull;
+constexpr unsigned kFillBlocks = 128u;
+__global__ __launch_bounds__(256) void k_fill_long(unsigned *__restrict__ table, unsigned long long size,
+                                                   LongFill lf) {
+    const unsigned cnt = min(*lf.count, lf.cap);
+    if (cnt == 0u) return;
+    const unsigned long long windows = (size + kFillWindow - 1) / kFillWindow;
+    for (unsigned long long wdx = blockIdx.x; wdx < windows; wdx += gridDim.x) {
+        const unsigned long long w0 = wdx * kFillWindow;
+        const unsigned long long w1 = w0 + kFillWindow < size ? w0 + kFillWindow : size;
+        for (unsigned r = 0; r < cnt; ++r) {
+            const uint4 e = lf.list[r];
+            const unsigned long long a = e.x > w0 ? e.x : w0, b = e.y < w1 ? e.y : w1;
+            if (a >= b) continue;
+            // scalar head and tail, 16-byte stores for the aligned middle
+            const unsigned long long a4 = (a + 3ull) & ~3ull, b4 = b & ~3ull;
+            if (a4 >= b4) {
+                for (unsigned long long k = a + threadIdx.x; k < b; k += 256ull) table[k] = e.z;
+                continue;
+            }
+            if (threadIdx.x < a4 - a) table[a + threadIdx.x] = e.z;
+            if (threadIdx.x < b - b4) table[b4 + threadIdx.x] = e.z;
+            const uint4 v4 = make_uint4(e.z, e.z, e.z, e.z);
+            for (unsigned long long k = a4 + 4ull * threadIdx.x; k < b4; k += 1024ull)
+                *reinterpret_cast<uint4 *>(table + k) = v4;
+        }
+    }
+}
+
 __global__ void k_reorder_seg(const unsigned *__restrict__ keys, const unsigned *__restrict__ idx,
                               const float4 *__restrict__ pos, const float4 *__restrict__ vel,
                               const int *__restrict__ rb, const uint16_t *__restrict__ scene,
                               const float4 *__restrict__ rec, float4 *__restrict__ sPos, float4 *__restrict__ sVel,
-                              unsigned *__restrict__ segStart, int segBits, long long numSegs, int n) {
+                              unsigned *__restrict__ segStart, int segBits, long long numSegs, int n,
+                              LongFill lf) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
     const unsigned k = valid ? keys[i] : 0u;
     const unsigned s = k >> segBits;
     const unsigned sp1 = (valid && i > 0) ? (keys[i - 1] >> segBits) + 1u : 0u;   // first segment not yet started
-    wave_fill(segStart, valid && s + 1u > sp1, sp1, s + 1u, (unsigned)i);
-    wave_fill(segStart, i == n - 1, s + 1u, (unsigned)numSegs + 1u, (unsigned)n);
+    wave_fill(segStart, valid && s + 1u > sp1, sp1, s + 1u, (unsigned)i, lf);
+    wave_fill(segStart, i == n - 1, s + 1u, (unsigned)numSegs + 1u, (unsigned)n, lf);
     if (!valid) return;
     const unsigned o = idx[i];
     if (rec) {
@@ -298,11 +331,21 @@ __global__ void k_dem_grid_seg(const float4 *__restrict__ sPos, const float4 *__
 
 void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, const float4 *pos, const float4 *vel,
                         const int *rb, const uint16_t *scene, const float4 *rec, float4 *sortedPos,
-                        float4 *sortedVel, unsigned *segStart, int segBits, long long numSegs, int n, hipStream_t st) {
+                        float4 *sortedVel, unsigned *segStart, int segBits, long long numSegs, int n,
+                        const LongFill &lf, hipStream_t st) {
     if (n <= 0) return;
     const int block = 256;
     hipLaunchKernelGGL(k_reorder_seg, dim3((n + block - 1) / block), dim3(block), 0, st, sortedKeys, sortedIdx, pos,
-                       vel, rb, scene, rec, sortedPos, sortedVel, segStart, segBits, numSegs, n);
+                       vel, rb, scene, rec, sortedPos, sortedVel, segStart, segBits, numSegs, n, lf);
+    if (lf.list) launch_fill_long(segStart, (unsigned long long)numSegs + 1ull, lf, st);
+}
+
+unsigned fill_long_capacity(unsigned long long tableSize) { return (unsigned)(tableSize / kLongRange + 2ull); }
+
+void launch_fill_long(unsigned *table, unsigned long long tableSize, const LongFill &lf, hipStream_t st) {
+    unsigned long long blocks = (tableSize + kFillWindow - 1) / kFillWindow;
+    if (blocks > kFillBlocks) blocks = kFillBlocks;
+    hipLaunchKernelGGL(k_fill_long, dim3((unsigned)blocks), dim3(256), 0, st, table, tableSize, lf);
 }
 
 void launch_dem_grid_seg(const float4 *sortedPos, const float4 *sortedVel, const unsigned *sortedKeys,

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "arsim_device.hpp"
+#include "csr_fill.hpp"
 
 namespace arsim {
 
@@ -67,8 +68,8 @@ struct KeyOut {
 
 // Up to three device ranges zeroed by extra blocks of the body-integration launch.
 struct ZeroJobs {
-    unsigned *p[3] = {nullptr, nullptr, nullptr};
-    size_t count[3] = {0, 0, 0};
+    unsigned *p[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t count[4] = {0, 0, 0, 0};
 };
 
 // ---------------------------------------------------------------- particle / body stages (body.hip)
@@ -129,7 +130,12 @@ void launch_dem_grid(const float4 *sortedPos, const float4 *sortedVel, const uns
 // rec: the key kernel's gather record (pos / vel / rb / scene are then not read), or NULL
 void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, const float4 *pos, const float4 *vel,
                         const int *rb, const uint16_t *scene, const float4 *rec, float4 *sortedPos,
-                        float4 *sortedVel, unsigned *segStart, int segBits, long long numSegs, int n, hipStream_t st);
+                        float4 *sortedVel, unsigned *segStart, int segBits, long long numSegs, int n,
+                        const LongFill &lf, hipStream_t st);
+// Capacity of the long-run list of a table of tableSize entries (runs are disjoint and longer than
+// kLongRange), and the launch that fills the listed runs.
+unsigned fill_long_capacity(unsigned long long tableSize);
+void launch_fill_long(unsigned *table, unsigned long long tableSize, const LongFill &lf, hipStream_t st);
 void launch_dem_grid_seg(const float4 *sortedPos, const float4 *sortedVel, const unsigned *sortedKeys,
                          const unsigned *sortedIdx, const unsigned *segStart, int segBits, int n, const DevPhys &ph,
                          const SceneView &sv, bool sceneInVelW, const DemOut &o, int block, hipStream_t st);
@@ -176,7 +182,8 @@ void launch_range_image(const uint16_t *img, int W, int H, const RangeCamera &ca
                         float4 *norm, float *radii, float *vao, hipStream_t st);
 void launch_static_hash(const float4 *pos, int m, const SceneGeom &g, unsigned *keys, hipStream_t st);
 void launch_static_reorder(const unsigned *keys, const unsigned *idx, const float4 *pos, const float4 *norm, int m,
-                           unsigned numCells, float4 *sPos, float4 *sNorm, unsigned *table, hipStream_t st);
+                           unsigned numCells, float4 *sPos, float4 *sNorm, unsigned *table, const LongFill &lf,
+                           hipStream_t st);
 // refLayout: cs / ce are the reference's static cellStart / cellEnd, normals are read through stIdx and
 // every particle gets pForce += F, pTorque += relPos x F; otherwise cs is the exact static cell table
 // (ce unused), normals are sorted and only body particles get pForce += F.
@@ -250,8 +257,10 @@ SisWork sis_work_layout(void *base, int cap, int numScenes, int numBodyParticles
 // perScene: the particles' scene ids give each scene's contact range (no static set then).
 void launch_sis_gather(const SisGatherIn &in, int nb, const SisContacts &out, const SisWork &w, bool perScene,
                        hipStream_t st);
-// Level schedule and `iterations` sweeps per scene; cap = capacity of the contact list.
+// Level schedule and `iterations` sweeps per scene; cap = capacity of the contact list; sceneBodies
+// (start, count) per scene, NULL for one scene of numBodies bodies; maxSceneBodies the largest count.
 int launch_sis_solve(const SisContacts &ct, const SisBodies &bd, const SisSolvePhys &ph, int cap, int numScenes,
-                     int numBodies, int iterations, const SisWork &w, const SortTuning &tu, hipStream_t st);
+                     int numBodies, const int2 *sceneBodies, int maxSceneBodies, int iterations, const SisWork &w,
+                     const SortTuning &tu, hipStream_t st);
 
 }  // namespace arsim

@@ -14,6 +14,7 @@ namespace {
 
 constexpr int kSisBlock = 1024;   // one workgroup per scene for levels / buckets / sweeps
 constexpr int kSisWaves = kSisBlock / 64;
+constexpr int kSisLdsBodies = 2048;   // 64 KiB of (vel, ang) per scene in LDS
 
 __device__ __forceinline__ float dot4(float4 a, float4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
 
@@ -367,30 +368,34 @@ __global__ __launch_bounds__(kSisBlock) void k_sis_levels(const int *__restrict_
 }
 
 // Contacts of one scene grouped by level: off[L] = first position of level L in order[], for
-// L = 0 .. maxLvl + 1; the scene's offsets start at off + range.x + scene.
+// L = 0 .. maxLvl + 1; the scene's offsets start at off + range.x + scene.  The per-level counters
+// live in LDS when the levels fit, in global memory (cur) otherwise.
+constexpr int kBucketLds = 12288;
 __global__ __launch_bounds__(kSisBlock) void k_sis_bucket(const int *__restrict__ lvl,
                                                           const int2 *__restrict__ sceneRange,
                                                           const int *__restrict__ numContacts,
                                                           const int *__restrict__ maxLvl, int *__restrict__ offAll,
                                                           int *__restrict__ curAll, int *__restrict__ order) {
+    __shared__ int sCnt[kBucketLds];
     __shared__ unsigned sWave[kSisWaves];
     __shared__ int sCarry;
     const int s = blockIdx.x;
     const int2 rg = scene_range(sceneRange, numContacts, s);
     const int levels = maxLvl[s] + 1;
+    const bool lds = levels + 1 <= kBucketLds;
     int *off = offAll + rg.x + s;
-    int *cur = curAll + rg.x + s;
-    for (int L = threadIdx.x; L <= levels; L += kSisBlock) off[L] = 0;
+    int *cnt = lds ? sCnt : curAll + rg.x + s;
+    for (int L = threadIdx.x; L <= levels; L += kSisBlock) cnt[L] = 0;
     __syncthreads();
-    for (int c = rg.x + (int)threadIdx.x; c < rg.y; c += kSisBlock) atomicAdd(&off[lvl[c] + 1], 1);
+    for (int c = rg.x + (int)threadIdx.x; c < rg.y; c += kSisBlock) atomicAdd(&cnt[lvl[c] + 1], 1);
     __syncthreads();
-    // inclusive scan of off[0 .. levels] (off[0] = 0): off[L] = contacts below level L
+    // inclusive scan of cnt[0 .. levels] (cnt[0] = 0): the first position of each level
     if (threadIdx.x == 0) sCarry = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int base = 0; base <= levels; base += kSisBlock) {
         const int L = base + (int)threadIdx.x;
-        const unsigned v = L <= levels ? (unsigned)off[L] : 0u;
+        const unsigned v = L <= levels ? (unsigned)cnt[L] : 0u;
         unsigned x = v;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -407,13 +412,13 @@ __global__ __launch_bounds__(kSisBlock) void k_sis_bucket(const int *__restrict_
         const int carry = sCarry;
         if (L <= levels) {
             off[L] = carry + (int)(before + x);
-            cur[L] = carry + (int)(before + x);
+            cnt[L] = carry + (int)(before + x);
         }
         __syncthreads();
         if (threadIdx.x == 0) sCarry = carry + (int)tot;
         __syncthreads();
     }
-    for (int c = rg.x + (int)threadIdx.x; c < rg.y; c += kSisBlock) order[rg.x + atomicAdd(&cur[lvl[c]], 1)] = c;
+    for (int c = rg.x + (int)threadIdx.x; c < rg.y; c += kSisBlock) order[rg.x + atomicAdd(&cnt[lvl[c]], 1)] = c;
 }
 
 __device__ __forceinline__ void ld_mat(const float *__restrict__ Iw, int b, float m[9]) {
@@ -439,7 +444,30 @@ __device__ __forceinline__ f3 divs3(f3 a, float s) { return f3{a.x / s, a.y / s,
 //  body contact (:1714-1860): the normal impulse with both bodies' terms, applied with opposite
 //    signs (its friction impulses are not applied there, so they are not formed here).
 // Velocities are stored with w = 0.
-__device__ void sis_contact(int c, const SisContacts &ct, const SisBodies &bd, const SisSolvePhys &ph) {
+// Where the sweeps keep the bodies' linear / angular velocities: global memory, or a copy of the
+// scene's bodies in LDS (index b - b0).
+struct VelGlobal {
+    float4 *vel, *ang;
+    __device__ float4 v(int b) const { return vel[b]; }
+    __device__ float4 w(int b) const { return ang[b]; }
+    __device__ void set(int b, f3 lv, f3 av) const {
+        vel[b] = f4(lv, 0.0f);
+        ang[b] = f4(av, 0.0f);
+    }
+};
+struct VelLds {
+    float4 *s;   // (vel, ang) pairs
+    int b0;
+    __device__ float4 v(int b) const { return s[2 * (b - b0)]; }
+    __device__ float4 w(int b) const { return s[2 * (b - b0) + 1]; }
+    __device__ void set(int b, f3 lv, f3 av) const {
+        s[2 * (b - b0)] = f4(lv, 0.0f);
+        s[2 * (b - b0) + 1] = f4(av, 0.0f);
+    }
+};
+
+template <class VA>
+__device__ void sis_contact(int c, const SisContacts &ct, const SisBodies &bd, const VA &va, const SisSolvePhys &ph) {
     const float4 n4 = ct.normal[c], pt = ct.point[c];
     const int2 bb = ct.bodies[c];
     const float bias = ct.bias[c];
@@ -453,7 +481,7 @@ __device__ void sis_contact(int c, const SisContacts &ct, const SisBodies &bd, c
         float I[9];
         ld_mat(bd.Iw, b, I);
         const float m = bd.mass[b];
-        f3 v = xyz(bd.vel[b]), w = xyz(bd.ang[b]);
+        f3 v = xyz(va.v(b)), w = xyz(va.w(b));
         float mc = 1.0f / m + sis_k(I, p, n);
         if (fabs((double)mc) < 0.00001) mc = 1.0f;
         const float vrel = dot3(add3(v, cross3(w, p)), n);
@@ -488,8 +516,7 @@ __device__ void sis_contact(int c, const SisContacts &ct, const SisBodies &bd, c
         const f3 J = add3(add3(scl3(ci, n), scl3(cf, t)), scl3(cf2, t2));
         v = add3(v, divs3(J, m));
         w = add3(w, mat_vec(I, cross3(p, J)));
-        bd.vel[b] = f4(v, 0.0f);
-        bd.ang[b] = f4(w, 0.0f);
+        va.set(b, v, w);
     } else {
         const int b1 = bb.x, b2 = bb.y;
         const float4 c1 = bd.pos[b1], c2 = bd.pos[b2];
@@ -499,7 +526,7 @@ __device__ void sis_contact(int c, const SisContacts &ct, const SisBodies &bd, c
         ld_mat(bd.Iw, b1, I1);
         ld_mat(bd.Iw, b2, I2);
         const float m1 = bd.mass[b1], m2 = bd.mass[b2];
-        f3 v1 = xyz(bd.vel[b1]), w1 = xyz(bd.ang[b1]), v2 = xyz(bd.vel[b2]), w2 = xyz(bd.ang[b2]);
+        f3 v1 = xyz(va.v(b1)), w1 = xyz(va.w(b1)), v2 = xyz(va.v(b2)), w2 = xyz(va.w(b2));
         float mc = 1.0f / m1 + 1.0f / m2 + sis_k(I1, p1, n) + sis_k(I2, p2, n);
         if (fabs((double)mc) < 0.00001) mc = 1.0f;
         const float vrel = dot3(add3(v1, cross3(w1, p1)), n) - dot3(add3(v2, cross3(w2, p2)), n);
@@ -512,32 +539,53 @@ __device__ void sis_contact(int c, const SisContacts &ct, const SisBodies &bd, c
         const f3 J = scl3(ci, n);
         v1 = add3(v1, divs3(J, m1));
         w1 = add3(w1, mat_vec(I1, cross3(p1, J)));
-        bd.vel[b1] = f4(v1, 0.0f);
-        bd.ang[b1] = f4(w1, 0.0f);
+        va.set(b1, v1, w1);
         v2 = sub3(v2, divs3(J, m2));
         w2 = sub3(w2, mat_vec(I2, cross3(p2, J)));
-        bd.vel[b2] = f4(v2, 0.0f);
-        bd.ang[b2] = f4(w2, 0.0f);
+        va.set(b2, v2, w2);
     }
     ct.impulse[c] = acc;
 }
 
 // The sweeps of one scene (one workgroup per scene): level by level, a barrier between levels.
+// LDS: the scene's bodies' velocities live in LDS for the whole kernel (copied in and out), so a
+// level's round trip is an LDS access instead of a global one.
+template <bool LDS>
 __global__ __launch_bounds__(kSisBlock) void k_sis_solve(SisContacts ct, SisBodies bd, SisSolvePhys ph,
                                                          const int2 *__restrict__ sceneRange,
                                                          const int *__restrict__ numContacts,
+                                                         const int2 *__restrict__ sceneBodies, int numBodies,
                                                          const int *__restrict__ maxLvl,
                                                          const int *__restrict__ offAll,
                                                          const int *__restrict__ order, int iterations) {
+    extern __shared__ float4 sVel[];
     const int s = blockIdx.x;
     const int2 rg = scene_range(sceneRange, numContacts, s);
     const int levels = maxLvl[s] + 1;
     const int *off = offAll + rg.x + s;
+    const int2 br = sceneBodies ? sceneBodies[s] : make_int2(0, numBodies);
+    if (LDS) {
+        for (int k = threadIdx.x; k < br.y; k += kSisBlock) {
+            sVel[2 * k] = bd.vel[br.x + k];
+            sVel[2 * k + 1] = bd.ang[br.x + k];
+        }
+        __syncthreads();
+    }
+    const VelLds vl{sVel, br.x};
+    const VelGlobal vg{bd.vel, bd.ang};
     for (int it = 0; it < iterations; ++it)
         for (int L = 0; L < levels; ++L) {
             const int a = off[L], e = off[L + 1];
-            for (int k = a + (int)threadIdx.x; k < e; k += kSisBlock) sis_contact(order[rg.x + k], ct, bd, ph);
+            for (int k = a + (int)threadIdx.x; k < e; k += kSisBlock) {
+                if (LDS) sis_contact(order[rg.x + k], ct, bd, vl, ph);
+                else sis_contact(order[rg.x + k], ct, bd, vg, ph);
+            }
             __syncthreads();
+        }
+    if (LDS)
+        for (int k = threadIdx.x; k < br.y; k += kSisBlock) {
+            bd.vel[br.x + k] = sVel[2 * k];
+            bd.ang[br.x + k] = sVel[2 * k + 1];
         }
 }
 
@@ -625,7 +673,8 @@ void launch_sis_gather(const SisGatherIn &in, int nb, const SisContacts &out, co
 }
 
 int launch_sis_solve(const SisContacts &ct, const SisBodies &bd, const SisSolvePhys &ph, int cap, int numScenes,
-                     int numBodies, int iterations, const SisWork &w, const SortTuning &tu, hipStream_t st) {
+                     int numBodies, const int2 *sceneBodies, int maxSceneBodies, int iterations, const SisWork &w,
+                     const SortTuning &tu, hipStream_t st) {
     if (cap <= 0 || numScenes <= 0) return 0;
     const int ne = 2 * cap;
     const unsigned invalid = (unsigned)numBodies;
@@ -641,9 +690,15 @@ int launch_sis_solve(const SisContacts &ct, const SisBodies &bd, const SisSolveP
                        w.maxLvl);
     hipLaunchKernelGGL(k_sis_bucket, dim3(numScenes), dim3(kSisBlock), 0, st, w.lvl, rg, w.numContacts, w.maxLvl,
                        w.off, w.cur, w.order);
-    if (iterations > 0)
-        hipLaunchKernelGGL(k_sis_solve, dim3(numScenes), dim3(kSisBlock), 0, st, ct, bd, ph, rg, w.numContacts,
-                           w.maxLvl, w.off, w.order, iterations);
+    if (iterations > 0) {
+        if (maxSceneBodies <= kSisLdsBodies)
+            hipLaunchKernelGGL(k_sis_solve<true>, dim3(numScenes), dim3(kSisBlock),
+                               (size_t)maxSceneBodies * 2 * sizeof(float4), st, ct, bd, ph, rg, w.numContacts,
+                               sceneBodies, numBodies, w.maxLvl, w.off, w.order, iterations);
+        else
+            hipLaunchKernelGGL(k_sis_solve<false>, dim3(numScenes), dim3(kSisBlock), 0, st, ct, bd, ph, rg,
+                               w.numContacts, sceneBodies, numBodies, w.maxLvl, w.off, w.order, iterations);
+    }
     return 0;
 }
 

@@ -74,13 +74,13 @@ __global__ void k_static_hash(const float4 *__restrict__ pos, int m, SceneGeom g
 __global__ void k_static_reorder(const unsigned *__restrict__ keys, const unsigned *__restrict__ idx,
                                  const float4 *__restrict__ pos, const float4 *__restrict__ norm, int m,
                                  unsigned numCells, float4 *__restrict__ sPos, float4 *__restrict__ sNorm,
-                                 unsigned *__restrict__ table) {
+                                 unsigned *__restrict__ table, LongFill lf) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < m;
     const unsigned k = valid ? keys[i] : 0u;
     const unsigned kp1 = (valid && i > 0) ? keys[i - 1] + 1u : 0u;
-    wave_fill(table, valid && k + 1u > kp1, kp1, k + 1u, (unsigned)i);
-    wave_fill(table, i == m - 1, k + 1u, numCells + 1u, (unsigned)m);
+    wave_fill(table, valid && k + 1u > kp1, kp1, k + 1u, (unsigned)i, lf);
+    wave_fill(table, i == m - 1, k + 1u, numCells + 1u, (unsigned)m, lf);
     if (!valid) return;
     const unsigned o = idx[i];
     sPos[i] = pos[o];
@@ -158,10 +158,12 @@ void launch_static_hash(const float4 *pos, int m, const SceneGeom &g, unsigned *
 }
 
 void launch_static_reorder(const unsigned *keys, const unsigned *idx, const float4 *pos, const float4 *norm, int m,
-                           unsigned numCells, float4 *sPos, float4 *sNorm, unsigned *table, hipStream_t st) {
+                           unsigned numCells, float4 *sPos, float4 *sNorm, unsigned *table, const LongFill &lf,
+                           hipStream_t st) {
     if (m <= 0) return;
     hipLaunchKernelGGL(k_static_reorder, dim3((m + 255) / 256), dim3(256), 0, st, keys, idx, pos, norm, m, numCells,
-                       sPos, sNorm, table);
+                       sPos, sNorm, table, lf);
+    if (lf.list) launch_fill_long(table, (unsigned long long)numCells + 1ull, lf, st);
 }
 
 void launch_dem_static(const float4 *sPos, const float4 *sVel, const unsigned *sIdx, int n, const float4 *stPos,

@@ -164,6 +164,70 @@ def batch_mix(num_scenes=512, lo=1024, hi=65536, seed=0, method=capi.METHOD_GRID
     return [mix_scene(int(n), seed * 100003 + k, method) for k, n in enumerate(counts)]
 
 
+def stacked_cubes(nx=32, nz=32, layers=2, seed=0, k=4, pitch=None, floor_pitch=0.5):
+    """Sequential-impulse workload: `layers` layers of nx x nz cubes of k^3 particles (spacing 2r), each
+    resting on the static floor or on the cube below (0.1 r penetration), with a random yaw and small
+    random velocities; returns (scene, floor points, floor normals).  Bodies only, ARSIM_CONTACT_SIS set."""
+    rng = np.random.default_rng(seed)
+    base = mix_scene(1024, seed)
+    r = float(base.params.particleRadius)
+    g = (np.arange(k, dtype=np.float32) - (k - 1) / 2) * (2 * r)
+    offs = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    m = offs.shape[0]
+    half = (k - 1) * r + r                        # half extent of a cube, particle radius included
+    pitch = pitch or 2 * half + 0.4 * r
+    floor_y = 1.0
+    cms = []
+    for layer in range(layers):
+        y = floor_y + r + half - 0.1 * r + layer * (2 * half - 0.1 * r)
+        for i in range(nx):
+            for j in range(nz):
+                cms.append((2 * half + i * pitch, y, 2 * half + j * pitch))
+    cms = np.asarray(cms, np.float32)
+    nb = len(cms)
+    yaw = rng.uniform(-0.05, 0.05, nb)
+    q = np.zeros((nb, 4), np.float32)
+    q[:, 1], q[:, 3] = np.sin(yaw / 2), np.cos(yaw / 2)
+    inertia = (offs.astype(np.float64) ** 2).sum() * np.eye(3) - offs.T.astype(np.float64) @ offs
+    inv = np.linalg.inv(inertia).astype(np.float32)
+    sc = base.copy()
+    sc.rb = np.repeat(np.arange(nb, dtype=np.int32), m)
+    sc.body_frame = np.zeros((nb * m, 4), np.float32)
+    sc.body_frame[:, :3] = np.tile(offs, (nb, 1))
+    sc.pos = np.zeros((nb * m, 4), np.float32)
+    c, sn = np.cos(yaw), np.sin(yaw)
+    for b in range(nb):
+        R = np.array([[c[b], 0, sn[b]], [0, 1, 0], [-sn[b], 0, c[b]]], np.float32)
+        sc.pos[b * m:(b + 1) * m, :3] = cms[b] + offs @ R.T
+    sc.vel = np.zeros_like(sc.pos)
+    sc.rb_pos = np.zeros((nb, 4), np.float32)
+    sc.rb_pos[:, :3] = cms
+    sc.rb_vel = np.zeros((nb, 4), np.float32)
+    sc.rb_vel[:, :3] = rng.normal(scale=0.05, size=(nb, 3))
+    sc.rb_ang = np.zeros((nb, 4), np.float32)
+    sc.rb_quat = q
+    sc.rb_inertia = np.tile(inv.T.reshape(1, 9), (nb, 1))
+    sc.rb_mass = np.full(nb, float(m), np.float32)
+    sc.ppo = np.full(nb, m, np.int32)
+    ext = 4 * half + max(nx, nz) * pitch
+    size = 1 << max(6, int(math.ceil(math.log2(ext))))
+    p = sc.params
+    p.gridSize.x = p.gridSize.y = p.gridSize.z = size
+    p.numCells = size ** 3
+    p.numBodies = nb
+    p.ARrestitution, p.RBrestitution, p.ARfriction, p.RBfriction = 0.5, 0.5, 0.3, 0.1
+    sc.cfg.boxMax[:] = [ext, ext, ext]
+    sc.cfg.flags = capi.CONTACT_SIS
+    sc.cfg.dt = 1e-3
+    xs = np.arange(0.0, ext, floor_pitch, dtype=np.float32)
+    X, Z = np.meshgrid(xs, xs, indexing="ij")
+    fp = np.zeros((X.size, 4), np.float32)
+    fp[:, 0], fp[:, 1], fp[:, 2] = X.ravel(), floor_y, Z.ravel()
+    fn = np.zeros_like(fp)
+    fn[:, 1] = 1.0
+    return sc, fp, fn
+
+
 class Template:
     """A particle template loaded as the reference's initObject does (arsim_template_load)."""
 

@@ -109,3 +109,34 @@ def test_engine_cell_index_is_cell_start():
     assert counts.tolist() == [s.n for s in mix]
     expect = np.searchsorted(keys, np.arange(total + 1, dtype=np.int64), "left").astype(np.uint32)
     assert np.array_equal(table, expect)
+
+
+def test_engine_cell_table_with_long_empty_runs():
+    """A scene in one corner of a 256^3 grid: the empty runs between its z slabs (~55,000 cells) and the
+    table's tail (~16 M cells) are filled by the long-run pass, over graph replays too."""
+    print("engine: cell table, long empty runs", flush=True)
+    sc = scenes.mix_scene(6000, 91, capi.METHOD_GRID)
+    p = sc.params
+    p.gridSize.x = p.gridSize.y = p.gridSize.z = 256
+    p.numCells = 256 ** 3
+    total = 256 ** 3
+    with Engine(sc, use_graph=True) as e:
+        for steps in (1, 3):
+            e.step(steps)
+            e.sync()
+            kp, tp = C.c_void_p(), C.c_void_p()
+            nul = None
+            capi.check(L.arsim_engine_debug_buffers(e.h, C.byref(kp), nul, C.byref(tp), nul, nul, nul, nul, nul, nul,
+                                                    nul))
+            keys = np.zeros(sc.n, np.uint32)
+            table = np.zeros(total + 1, np.uint32)
+            capi.check(L.arsim_memcpy_dtoh(capi.ptr(keys), kp, keys.nbytes, None))
+            capi.check(L.arsim_memcpy_dtoh(capi.ptr(table), tp, table.nbytes, None))
+            gaps = np.diff(np.unique(keys).astype(np.int64))
+            assert gaps.max() > 4096 and total - int(keys[-1]) > 4096
+            expect = np.searchsorted(keys, np.arange(total + 1, dtype=np.int64), "left").astype(np.uint32)
+            assert np.array_equal(table, expect)
+        g = e.download()
+    st = oracle.State(sc, np.float32)
+    st.step(4)
+    assert_state_equal(g, st, "sparse grid")
