@@ -232,6 +232,7 @@ struct SisWork {
     size_t sortWsBytes;
     int *pred, *lvl, *order, *off, *cur, *maxLvl;
     int2 *sceneRange;
+    float4 *prep;                  // the sweeps' records in level order (planes of cap float4)
     unsigned *blockCnt;
     int *numContacts;
     size_t end;

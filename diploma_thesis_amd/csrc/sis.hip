@@ -436,14 +436,6 @@ __device__ __forceinline__ f3 gnormalize(f3 v) {
 }
 __device__ __forceinline__ f3 divs3(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
 
-// One contact of one sweep: SequentialImpulseSolver (Particles/updateFunctions.cpp:1580-1874).
-//  static contact (:1603-1713): normal impulse accumulated and clamped to [0, 100]; friction along
-//    t = normalize((1,0,0) - ((1,0,0) . n) n) (or (0,1,0) when n.x is within 1e-4 of +-1) and
-//    t2 = normalize(t x n), each accumulated and clamped to +-ARfriction * mc * |g.y|;
-//    v += J / m, w += I^-1 (p x J), J = jn n + jt t + jt2 t2;
-//  body contact (:1714-1860): the normal impulse with both bodies' terms, applied with opposite
-//    signs (its friction impulses are not applied there, so they are not formed here).
-// Velocities are stored with w = 0.
 // Where the sweeps keep the bodies' linear / angular velocities: global memory, or a copy of the
 // scene's bodies in LDS (index b - b0).
 struct VelGlobal {
@@ -466,24 +458,90 @@ struct VelLds {
     }
 };
 
-template <class VA>
-__device__ void sis_contact(int c, const SisContacts &ct, const SisBodies &bd, const VA &va, const SisSolvePhys &ph) {
+// Sweep records in level order.  k_sis_prep writes, for position q of order[] (contact c =
+// order[q]), the contact's data and the terms that stay fixed over the sweeps of a step, formed with
+// the expressions the serial loop evaluates in every sweep (so they are the same bits), into planes
+// of kSisPlanes float4 (plane k at rec + k * cap):
+//   0: (n, bias)   1: (p1, mc)   2: static (t, kt), body (p2, 0)   3: static (t2, kt2)
+//   4: (b1 bits, b2 bits, m1, static: fb / body: m2)
+//   5..9: I1 (9 floats) then I2 (9 floats), packed
+//   10: the accumulated impulses (normal, friction, friction 2, 0)
+// mc already has the |mc| < 1e-5 -> 1 substitution applied.  A sweep then reads its level's records
+// with coalesced loads (a level's positions are consecutive) instead of gathering by contact.
+constexpr int kSisPlanes = 11;
+__global__ void k_sis_prep(SisContacts ct, SisBodies bd, SisSolvePhys ph, const int *__restrict__ numContacts,
+                           const int *__restrict__ order, int cap, float4 *__restrict__ rec) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= numContacts[0]) return;
+    const int c = order[q];
+    const size_t P = (size_t)cap;
     const float4 n4 = ct.normal[c], pt = ct.point[c];
     const int2 bb = ct.bodies[c];
-    const float bias = ct.bias[c];
-    float4 acc = ct.impulse[c];
     const f3 n = xyz(n4);
-    const float upper = 100.0f;
+    float I1[9], I2[9];
+    ld_mat(bd.Iw, bb.x, I1);
+    const float m1 = bd.mass[bb.x];
+    const float4 c1 = bd.pos[bb.x];
+    const f3 p1 = f3{pt.x - c1.x, pt.y - c1.y, pt.z - c1.z};
+    rec[q] = make_float4(n.x, n.y, n.z, ct.bias[c]);
     if (bb.y < 0) {
-        const int b = bb.x;
-        const float4 cm = bd.pos[b];
-        const f3 p = f3{pt.x - cm.x, pt.y - cm.y, pt.z - cm.z};
-        float I[9];
-        ld_mat(bd.Iw, b, I);
-        const float m = bd.mass[b];
-        f3 v = xyz(va.v(b)), w = xyz(va.w(b));
-        float mc = 1.0f / m + sis_k(I, p, n);
+        float mc = 1.0f / m1 + sis_k(I1, p1, n);
         if (fabs((double)mc) < 0.00001) mc = 1.0f;
+        const float fb = ph.arFric * mc * fabsf(ph.gy);
+        f3 t = f3{1.0f, 0.0f, 0.0f};
+        if (fabs((double)(t.x - n.x)) < 0.0001) t = f3{0.0f, 1.0f, 0.0f};
+        else if (fabs((double)(t.x + n.x)) < 0.0001) t = f3{0.0f, 1.0f, 0.0f};
+        t = sub3(t, scl3(dot3(t, n), n));
+        t = gnormalize(t);
+        const float kt = 1.0f / m1 + sis_k(I1, p1, t);
+        const f3 t2 = gnormalize(cross3(t, n));
+        const float kt2 = 1.0f / m1 + sis_k(I1, p1, t2);
+        rec[P + q] = f4(p1, mc);
+        rec[2 * P + q] = f4(t, kt);
+        rec[3 * P + q] = f4(t2, kt2);
+        rec[4 * P + q] = make_float4(__int_as_float(bb.x), __int_as_float(-1), m1, fb);
+        for (int k = 0; k < 9; ++k) I2[k] = 0.0f;
+    } else {
+        ld_mat(bd.Iw, bb.y, I2);
+        const float m2 = bd.mass[bb.y];
+        const float4 c2 = bd.pos[bb.y];
+        const f3 p2 = f3{pt.x - c2.x, pt.y - c2.y, pt.z - c2.z};
+        float mc = 1.0f / m1 + 1.0f / m2 + sis_k(I1, p1, n) + sis_k(I2, p2, n);
+        if (fabs((double)mc) < 0.00001) mc = 1.0f;
+        rec[P + q] = f4(p1, mc);
+        rec[2 * P + q] = f4(p2, 0.0f);
+        rec[4 * P + q] = make_float4(__int_as_float(bb.x), __int_as_float(bb.y), m1, m2);
+    }
+    rec[5 * P + q] = make_float4(I1[0], I1[1], I1[2], I1[3]);
+    rec[6 * P + q] = make_float4(I1[4], I1[5], I1[6], I1[7]);
+    rec[7 * P + q] = make_float4(I1[8], I2[0], I2[1], I2[2]);
+    rec[8 * P + q] = make_float4(I2[3], I2[4], I2[5], I2[6]);
+    rec[9 * P + q] = make_float4(I2[7], I2[8], 0.0f, 0.0f);
+    rec[10 * P + q] = ct.impulse[c];
+}
+
+// One contact of one sweep, record q: SequentialImpulseSolver (Particles/updateFunctions.cpp:1580-1874).
+//  static contact (:1603-1713): normal impulse accumulated and clamped to [0, 100]; friction along
+//    t and t2, each accumulated and clamped to +-fb; v += J / m, w += I^-1 (p x J),
+//    J = jn n + jt t + jt2 t2;
+//  body contact (:1714-1860): the normal impulse with both bodies' terms, applied with opposite
+//    signs (its friction impulses are not applied there, so they are not formed here).
+// Velocities are stored with w = 0.
+template <class VA>
+__device__ void sis_contact(size_t q, float4 *__restrict__ rec, size_t P, const VA &va) {
+    const float4 r0 = rec[q], r1 = rec[P + q], r2 = rec[2 * P + q], r4 = rec[4 * P + q];
+    const float4 i5 = rec[5 * P + q], i6 = rec[6 * P + q], i7 = rec[7 * P + q];
+    float4 acc = rec[10 * P + q];
+    const f3 n = xyz(r0);
+    const float bias = r0.w, mc = r1.w;
+    const int b1 = __float_as_int(r4.x), b2 = __float_as_int(r4.y);
+    const float I1[9] = {i5.x, i5.y, i5.z, i5.w, i6.x, i6.y, i6.z, i6.w, i7.x};
+    const float upper = 100.0f;
+    if (b2 < 0) {
+        const float4 r3 = rec[3 * P + q];
+        const f3 p = xyz(r1), t = xyz(r2), t2 = xyz(r3);
+        const float kt = r2.w, kt2 = r3.w, m = r4.z, fb = r4.w;
+        f3 v = xyz(va.v(b1)), w = xyz(va.w(b1));
         const float vrel = dot3(add3(v, cross3(w, p)), n);
         float ci = -(vrel + bias) / mc;
         float tmp = acc.x + ci;
@@ -491,22 +549,13 @@ __device__ void sis_contact(int c, const SisContacts &ct, const SisBodies &bd, c
         else if (tmp > upper) tmp = upper;
         ci = tmp - acc.x;
         acc.x = tmp;
-        const float fb = ph.arFric * mc * fabsf(ph.gy);
         const f3 vel = add3(v, cross3(w, p));
-        f3 t = f3{1.0f, 0.0f, 0.0f};
-        if (fabs((double)(t.x - n.x)) < 0.0001) t = f3{0.0f, 1.0f, 0.0f};
-        else if (fabs((double)(t.x + n.x)) < 0.0001) t = f3{0.0f, 1.0f, 0.0f};
-        t = sub3(t, scl3(dot3(t, n), n));
-        t = gnormalize(t);
-        const float kt = 1.0f / m + sis_k(I, p, t);
         float cf = -dot3(vel, t) / kt;
         float tf = acc.y + cf;
         if (tf < -fb) tf = -fb;
         else if (tf > fb) tf = fb;
         cf = tf - acc.y;
         acc.y = tf;
-        const f3 t2 = gnormalize(cross3(t, n));
-        const float kt2 = 1.0f / m + sis_k(I, p, t2);
         float cf2 = -dot3(vel, t2) / kt2;
         float tf2 = acc.z + cf2;
         if (tf2 < -fb) tf2 = -fb;
@@ -515,20 +564,14 @@ __device__ void sis_contact(int c, const SisContacts &ct, const SisBodies &bd, c
         acc.z = tf2;
         const f3 J = add3(add3(scl3(ci, n), scl3(cf, t)), scl3(cf2, t2));
         v = add3(v, divs3(J, m));
-        w = add3(w, mat_vec(I, cross3(p, J)));
-        va.set(b, v, w);
+        w = add3(w, mat_vec(I1, cross3(p, J)));
+        va.set(b1, v, w);
     } else {
-        const int b1 = bb.x, b2 = bb.y;
-        const float4 c1 = bd.pos[b1], c2 = bd.pos[b2];
-        const f3 p1 = f3{pt.x - c1.x, pt.y - c1.y, pt.z - c1.z};
-        const f3 p2 = f3{pt.x - c2.x, pt.y - c2.y, pt.z - c2.z};
-        float I1[9], I2[9];
-        ld_mat(bd.Iw, b1, I1);
-        ld_mat(bd.Iw, b2, I2);
-        const float m1 = bd.mass[b1], m2 = bd.mass[b2];
+        const float4 i8 = rec[8 * P + q], i9 = rec[9 * P + q];
+        const float I2[9] = {i7.y, i7.z, i7.w, i8.x, i8.y, i8.z, i8.w, i9.x, i9.y};
+        const f3 p1 = xyz(r1), p2 = xyz(r2);
+        const float m1 = r4.z, m2 = r4.w;
         f3 v1 = xyz(va.v(b1)), w1 = xyz(va.w(b1)), v2 = xyz(va.v(b2)), w2 = xyz(va.w(b2));
-        float mc = 1.0f / m1 + 1.0f / m2 + sis_k(I1, p1, n) + sis_k(I2, p2, n);
-        if (fabs((double)mc) < 0.00001) mc = 1.0f;
         const float vrel = dot3(add3(v1, cross3(w1, p1)), n) - dot3(add3(v2, cross3(w2, p2)), n);
         float ci = -(vrel + bias) / mc;
         float tmp = acc.x + ci;
@@ -544,15 +587,15 @@ __device__ void sis_contact(int c, const SisContacts &ct, const SisBodies &bd, c
         w2 = sub3(w2, mat_vec(I2, cross3(p2, J)));
         va.set(b2, v2, w2);
     }
-    ct.impulse[c] = acc;
+    rec[10 * P + q] = acc;
 }
 
 // The sweeps of one scene (one workgroup per scene): level by level, a barrier between levels.
 // LDS: the scene's bodies' velocities live in LDS for the whole kernel (copied in and out), so a
 // level's round trip is an LDS access instead of a global one.
 template <bool LDS>
-__global__ __launch_bounds__(kSisBlock) void k_sis_solve(SisContacts ct, SisBodies bd, SisSolvePhys ph,
-                                                         const int2 *__restrict__ sceneRange,
+__global__ __launch_bounds__(kSisBlock) void k_sis_solve(float4 *__restrict__ rec, int cap, float4 *__restrict__ impulse,
+                                                         SisBodies bd, const int2 *__restrict__ sceneRange,
                                                          const int *__restrict__ numContacts,
                                                          const int2 *__restrict__ sceneBodies, int numBodies,
                                                          const int *__restrict__ maxLvl,
@@ -563,6 +606,7 @@ __global__ __launch_bounds__(kSisBlock) void k_sis_solve(SisContacts ct, SisBodi
     const int2 rg = scene_range(sceneRange, numContacts, s);
     const int levels = maxLvl[s] + 1;
     const int *off = offAll + rg.x + s;
+    const size_t P = (size_t)cap;
     const int2 br = sceneBodies ? sceneBodies[s] : make_int2(0, numBodies);
     if (LDS) {
         for (int k = threadIdx.x; k < br.y; k += kSisBlock) {
@@ -577,8 +621,8 @@ __global__ __launch_bounds__(kSisBlock) void k_sis_solve(SisContacts ct, SisBodi
         for (int L = 0; L < levels; ++L) {
             const int a = off[L], e = off[L + 1];
             for (int k = a + (int)threadIdx.x; k < e; k += kSisBlock) {
-                if (LDS) sis_contact(order[rg.x + k], ct, bd, vl, ph);
-                else sis_contact(order[rg.x + k], ct, bd, vg, ph);
+                if (LDS) sis_contact((size_t)(rg.x + k), rec, P, vl);
+                else sis_contact((size_t)(rg.x + k), rec, P, vg);
             }
             __syncthreads();
         }
@@ -587,6 +631,7 @@ __global__ __launch_bounds__(kSisBlock) void k_sis_solve(SisContacts ct, SisBodi
             bd.vel[br.x + k] = sVel[2 * k];
             bd.ang[br.x + k] = sVel[2 * k + 1];
         }
+    for (int k = rg.x + (int)threadIdx.x; k < rg.y; k += kSisBlock) impulse[order[k]] = rec[10 * P + k];
 }
 
 }  // namespace
@@ -655,6 +700,7 @@ SisWork sis_work_layout(void *base, int cap, int numScenes, int numBodyParticles
     w.cur = (int *)take(((size_t)cap + numScenes + 1) * 4);
     w.maxLvl = (int *)take((size_t)numScenes * 4);
     w.sceneRange = (int2 *)take((size_t)numScenes * 8);
+    w.prep = (float4 *)take((size_t)kSisPlanes * cap * 16);
     w.blockCnt = (unsigned *)take(2 * (size_t)(nblk + 1) * 4);
     w.numContacts = (int *)take(4);
     w.end = o;
@@ -691,13 +737,15 @@ int launch_sis_solve(const SisContacts &ct, const SisBodies &bd, const SisSolveP
     hipLaunchKernelGGL(k_sis_bucket, dim3(numScenes), dim3(kSisBlock), 0, st, w.lvl, rg, w.numContacts, w.maxLvl,
                        w.off, w.cur, w.order);
     if (iterations > 0) {
+        hipLaunchKernelGGL(k_sis_prep, dim3((cap + 255) / 256), dim3(256), 0, st, ct, bd, ph, w.numContacts, w.order,
+                           cap, w.prep);
         if (maxSceneBodies <= kSisLdsBodies)
             hipLaunchKernelGGL(k_sis_solve<true>, dim3(numScenes), dim3(kSisBlock),
-                               (size_t)maxSceneBodies * 2 * sizeof(float4), st, ct, bd, ph, rg, w.numContacts,
-                               sceneBodies, numBodies, w.maxLvl, w.off, w.order, iterations);
-        else
-            hipLaunchKernelGGL(k_sis_solve<false>, dim3(numScenes), dim3(kSisBlock), 0, st, ct, bd, ph, rg,
+                               (size_t)maxSceneBodies * 2 * sizeof(float4), st, w.prep, cap, ct.impulse, bd, rg,
                                w.numContacts, sceneBodies, numBodies, w.maxLvl, w.off, w.order, iterations);
+        else
+            hipLaunchKernelGGL(k_sis_solve<false>, dim3(numScenes), dim3(kSisBlock), 0, st, w.prep, cap, ct.impulse,
+                               bd, rg, w.numContacts, sceneBodies, numBodies, w.maxLvl, w.off, w.order, iterations);
     }
     return 0;
 }
