@@ -6,6 +6,8 @@
 // of its predecessors' levels.  Contacts of one level touch disjoint bodies and everything they
 // read was written by lower levels, so running a sweep level by level, with a barrier between
 // levels, reads and writes every body value exactly as the serial loop does.
+#include <climits>
+
 #include "kernels.hpp"
 
 namespace arsim {
@@ -323,43 +325,76 @@ __device__ __forceinline__ int block_max(int v, int *sRed) {
     return m;
 }
 
+// Block-wide inclusive max-scan of x over the threads in segments that start where head is set.
+__device__ __forceinline__ int block_seg_max_scan(int x, bool head, int *sAgg, int *sHead) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    bool f = head;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        const int g = __shfl_up((int)f, d, 64);
+        if (lane >= d) {
+            if (!f) x = max(x, y);
+            f = f || g;
+        }
+    }
+    if (lane == 63) {
+        sAgg[wave] = x;
+        sHead[wave] = f ? 1 : 0;
+    }
+    __syncthreads();
+    // carry into this wave: the running maximum of the segment open at the end of the wave before
+    int carry = INT_MIN;
+    for (int w = 0; w < wave; ++w) carry = sHead[w] ? sAgg[w] : max(carry, sAgg[w]);
+    __syncthreads();
+    return f ? x : max(x, carry);
+}
+
 // Levels of one scene's contacts (one workgroup per scene), chunk by chunk in contact order:
-// a predecessor before the chunk has its final level already; within the chunk the levels are
-// relaxed (level = 1 + max over predecessors) until nothing changes, which takes as many passes
-// as the longest dependency chain inside the chunk.
+// level(c) = 1 + max(level of its predecessors), 0 without any.  A predecessor before the chunk has
+// its final level already.  Inside the chunk, a contact whose predecessor is the contact just before
+// it continues that contact's run (typically the contacts of one body): along a run
+// L(c) = max(B(c), L(c - 1) + 1), with B(c) = 1 + the levels of its other predecessors, so
+// L(c) = c + max over the run up to c of (B(k) - k), a segmented max-scan.  The scan is repeated with
+// the updated B until nothing changes, which takes as many passes as the longest path has jumps
+// between runs.
 __global__ __launch_bounds__(kSisBlock) void k_sis_levels(const int *__restrict__ pred,
                                                           const int2 *__restrict__ sceneRange,
                                                           const int *__restrict__ numContacts, int *__restrict__ lvl,
                                                           int *__restrict__ maxLvl) {
     __shared__ int sL[kSisBlock];
+    __shared__ int sAgg[kSisWaves], sHead[kSisWaves];
     __shared__ int sRed[kSisWaves];
     const int2 rg = scene_range(sceneRange, numContacts, blockIdx.x);
+    const int t = (int)threadIdx.x;
     int mx = -1;
     for (int c0 = rg.x; c0 < rg.y; c0 += kSisBlock) {
-        const int c = c0 + (int)threadIdx.x;
+        const int c = c0 + t;
         const bool ok = c < rg.y;
-        int p1 = -1, p2 = -1, base = 0;
+        int p1 = -1, p2 = -1, outer = 0;
         if (ok) {
             p1 = pred[2 * c];
             p2 = pred[2 * c + 1];
-            if (p1 >= 0 && p1 < c0) base = max(base, lvl[p1] + 1);
-            if (p2 >= 0 && p2 < c0) base = max(base, lvl[p2] + 1);
+            if (p1 >= 0 && p1 < c0) outer = max(outer, lvl[p1] + 1);
+            if (p2 >= 0 && p2 < c0) outer = max(outer, lvl[p2] + 1);
         }
-        const bool in1 = p1 >= c0, in2 = p2 >= c0;
-        sL[threadIdx.x] = base;
+        // the run link (the previous contact is a predecessor) and the other in-chunk predecessors
+        const bool link = ok && t > 0 && (p1 == c - 1 || p2 == c - 1);
+        const bool side1 = p1 >= c0 && p1 != c - 1, side2 = p2 >= c0 && p2 != c - 1;
+        sL[t] = 0;
         __syncthreads();
         while (true) {
-            int v = base;
-            if (in1) v = max(v, sL[p1 - c0] + 1);
-            if (in2) v = max(v, sL[p2 - c0] + 1);
-            const bool changed = v != sL[threadIdx.x];
-            __syncthreads();
-            if (changed) sL[threadIdx.x] = v;
+            int b = outer;
+            if (side1) b = max(b, sL[p1 - c0] + 1);
+            if (side2) b = max(b, sL[p2 - c0] + 1);
+            const int L = t + block_seg_max_scan(b - t, !link, sAgg, sHead);
+            const bool changed = ok && L != sL[t];
+            if (changed) sL[t] = L;
             if (!__syncthreads_or(changed)) break;
         }
         if (ok) {
-            lvl[c] = sL[threadIdx.x];
-            mx = max(mx, sL[threadIdx.x]);
+            lvl[c] = sL[t];
+            mx = max(mx, sL[t]);
         }
         __syncthreads();
     }
