@@ -14,16 +14,18 @@ __device__ __forceinline__ void fill_range(unsigned *p, long long a, long long b
     for (; k < b; ++k) p[k] = v;
 }
 
-// Ranges longer than kLongRange entries that a kernel hands to k_fill_long (launch_fill_long):
-// list[k] = (a, b, v, 0) for k < *count.  One wave filling a run of millions of empty cells (a scene
-// that occupies a small part of its grid) would take milliseconds; k_fill_long spreads such runs
-// over many workgroups.  *count must be zero before the kernel that appends runs.
+// Ranges longer than kLongRange entries that a kernel hands to k_fill_long (launch_fill_long), cut
+// into chunks of at most kLongChunk entries: list[k] = (a, b, v, 0) for k < *count.  One wave filling
+// a run of millions of empty cells (a scene that occupies a small part of its grid) would take
+// milliseconds; k_fill_long gives every chunk to a workgroup.  *count must be zero before the kernel
+// that appends runs.
 struct LongFill {
     unsigned *count;
     uint4 *list;
     unsigned cap;
 };
 constexpr unsigned kLongRange = 4096u;
+constexpr unsigned kLongChunk = 65536u;
 
 // Fills p[a, b) with v for every lane that has a range: short ranges by the lane itself, long
 // ones (empty-cell runs, the tail of the table) by the whole wave with coalesced stores, and runs
@@ -31,9 +33,13 @@ constexpr unsigned kLongRange = 4096u;
 __device__ __forceinline__ void wave_fill(unsigned *p, bool has, unsigned a, unsigned b, unsigned v,
                                           const LongFill &lf = LongFill{nullptr, nullptr, 0u}) {
     if (lf.list && has && b - a > kLongRange) {
-        const unsigned slot = atomicAdd(lf.count, 1u);
-        if (slot < lf.cap) {
-            lf.list[slot] = make_uint4(a, b, v, 0u);
+        const unsigned nch = (b - a + kLongChunk - 1u) / kLongChunk;
+        const unsigned slot = atomicAdd(lf.count, nch);
+        if (slot + nch <= lf.cap) {
+            for (unsigned q = 0; q < nch; ++q) {
+                const unsigned c0 = a + q * kLongChunk;
+                lf.list[slot + q] = make_uint4(c0, b - c0 > kLongChunk ? c0 + kLongChunk : b, v, 0u);
+            }
             has = false;
         }
     }

@@ -106,35 +106,26 @@ __global__ void k_dem_grid(const float4 *__restrict__ sPos, const float4 *__rest
 // segStart[s] = first sorted index whose key >= s << segBits (segStart[numSegs] = n).  Every entry
 // is rewritten each step, so nothing has to be cleared.  With segBits = 0 it is cellStart with
 // cellEnd[c] = segStart[c + 1], one table instead of two.  numSegs < 2^32.
-// The long runs handed over by wave_fill: the table is cut into windows of W entries and each
-// workgroup takes windows blockIdx, blockIdx + gridDim, ...; in each window it fills the part of
-// every listed run that falls inside.  With no runs listed the kernel only reads the count.
-constexpr unsigned long long kFillWindow = 65536ull;
-constexpr unsigned kFillBlocks = 128u;
-__global__ __launch_bounds__(256) void k_fill_long(unsigned *__restrict__ table, unsigned long long size,
-                                                   LongFill lf) {
+// The long runs handed over by wave_fill, one chunk of at most kLongChunk entries per workgroup
+// (workgroup g takes chunks g, g + G, ...).  With nothing listed the kernel only reads the count.
+constexpr unsigned kFillBlocks = 512u;
+__global__ __launch_bounds__(256) void k_fill_long(unsigned *__restrict__ table, unsigned long long size, LongFill lf) {
     const unsigned cnt = min(*lf.count, lf.cap);
-    if (cnt == 0u) return;
-    const unsigned long long windows = (size + kFillWindow - 1) / kFillWindow;
-    for (unsigned long long wdx = blockIdx.x; wdx < windows; wdx += gridDim.x) {
-        const unsigned long long w0 = wdx * kFillWindow;
-        const unsigned long long w1 = w0 + kFillWindow < size ? w0 + kFillWindow : size;
-        for (unsigned r = 0; r < cnt; ++r) {
-            const uint4 e = lf.list[r];
-            const unsigned long long a = e.x > w0 ? e.x : w0, b = e.y < w1 ? e.y : w1;
-            if (a >= b) continue;
-            // scalar head and tail, 16-byte stores for the aligned middle
-            const unsigned long long a4 = (a + 3ull) & ~3ull, b4 = b & ~3ull;
-            if (a4 >= b4) {
-                for (unsigned long long k = a + threadIdx.x; k < b; k += 256ull) table[k] = e.z;
-                continue;
-            }
-            if (threadIdx.x < a4 - a) table[a + threadIdx.x] = e.z;
-            if (threadIdx.x < b - b4) table[b4 + threadIdx.x] = e.z;
-            const uint4 v4 = make_uint4(e.z, e.z, e.z, e.z);
-            for (unsigned long long k = a4 + 4ull * threadIdx.x; k < b4; k += 1024ull)
-                *reinterpret_cast<uint4 *>(table + k) = v4;
+    for (unsigned r = blockIdx.x; r < cnt; r += gridDim.x) {
+        const uint4 e = lf.list[r];
+        // (the capacity bound means every listed entry was written; the clamp keeps stores in the table)
+        const unsigned a = e.x, b = (unsigned long long)e.y < size ? e.y : (unsigned)size;
+        if (a >= b) continue;
+        // scalar head and tail, 16-byte stores for the aligned middle
+        const unsigned a4 = (a + 3u) & ~3u, b4 = b & ~3u;
+        if (a4 >= b4) {
+            for (unsigned k = a + threadIdx.x; k < b; k += 256u) table[k] = e.z;
+            continue;
         }
+        if (threadIdx.x < a4 - a) table[a + threadIdx.x] = e.z;
+        if (threadIdx.x < b - b4) table[b4 + threadIdx.x] = e.z;
+        const uint4 v4 = make_uint4(e.z, e.z, e.z, e.z);
+        for (unsigned k = a4 + 4u * threadIdx.x; k < b4; k += 1024u) *reinterpret_cast<uint4 *>(table + k) = v4;
     }
 }
 
@@ -340,11 +331,16 @@ void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, c
     if (lf.list) launch_fill_long(segStart, (unsigned long long)numSegs + 1ull, lf, st);
 }
 
-unsigned fill_long_capacity(unsigned long long tableSize) { return (unsigned)(tableSize / kLongRange + 2ull); }
+// Runs are disjoint and longer than kLongRange, so there are fewer than size / kLongRange of them, and
+// each has at most len / kLongChunk + 1 chunks.
+unsigned fill_long_capacity(unsigned long long tableSize) {
+    return (unsigned)(tableSize / kLongRange + tableSize / kLongChunk + 4ull);
+}
 
 void launch_fill_long(unsigned *table, unsigned long long tableSize, const LongFill &lf, hipStream_t st) {
-    unsigned long long blocks = (tableSize + kFillWindow - 1) / kFillWindow;
+    unsigned long long blocks = (tableSize + kLongChunk - 1) / kLongChunk;
     if (blocks > kFillBlocks) blocks = kFillBlocks;
+    if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_fill_long, dim3((unsigned)blocks), dim3(256), 0, st, table, tableSize, lf);
 }
 

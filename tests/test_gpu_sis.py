@@ -201,3 +201,31 @@ def test_engine_sis_refuses_grains():
     sc.cfg.flags = capi.CONTACT_SIS
     with pytest.raises(capi.ArSimError):
         Engine(sc)
+
+
+def test_engine_sis_without_contacts_and_single_body():
+    print("sis: no contacts, one body", flush=True)
+    far = dense_bodies(8, 4.0, 30.0, seed=31, vel_sigma=0.0)
+    far.rb_pos[:, :3] = np.float32([[4 + 8 * (b % 4), 4 + 8 * (b // 4), 20.0] for b in range(8)])
+    one = dense_bodies(1, 10.0, 1.0, seed=32)
+    for sc, what in ((far, "no contacts"), (one, "one body")):
+        with Engine(sc) as e:
+            e.step(5)
+            e.sync()
+            g = e.download()
+        st = oracle.State(sc, np.float32)
+        st.step(5)
+        assert_state_equal(g, st, what)
+
+
+def test_engine_sis_more_bodies_than_lds_holds():
+    # 2100 bodies of 8 particles: the sweeps keep velocities in global memory (more than 2048 bodies)
+    print("sis: 2100 bodies, velocities in global memory", flush=True)
+    sc = dense_bodies(2100, 6.0, 26.0, seed=33, k=2)
+    with Engine(sc) as e:
+        e.step(3)
+        e.sync()
+        g = e.download()
+    st = oracle.State(sc, np.float32)
+    st.step(3)
+    assert_state_equal(g, st, "2100 bodies")
