@@ -34,6 +34,9 @@ ALGO_BYTES = {
     "pose_key": (68.0 + 24.0, 0.0),     # body integrate + pose 68, hash 24
     "sort": (4.0 + 16.0 * 3, 0.0),      # onesweep, u32 key + u32 value, 3 digit passes over the 24-bit grid key
     "reduce_apply": (32.0, 0.0),
+    # bucket sort: one onesweep pass on the top digit, then the two remaining passes fused with the reorder
+    "sort_top_digit": (4.0 + 16.0, 0.0),
+    "bucket_sort_reorder": (72.0 + 2 * 16.0, 4.0),
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -206,7 +209,7 @@ def main():
     for _ in range(reps):
         for name, ms in eng.profile():
             stage_ms[name] = stage_ms.get(name, 0.0) + ms / reps
-    dom = max((k for k in stage_ms if k != "sort"), key=lambda k: stage_ms[k])
+    dom = max((k for k in stage_ms if not k.startswith("sort")), key=lambda k: stage_ms[k])
     cells = int(sc.params.numCells)
 
     def algo_bytes(stage):

@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["sort.hip", "body.hip", "grid.hip", "lbvh.hip", "static.hip", "sis.hip", "capi.cpp", "engine.cpp", "serve.cpp", "scenegen.cpp",
+SOURCES = ["sort.hip", "body.hip", "grid.hip", "lbvh.hip", "static.hip", "sis.hip", "bucket.hip", "capi.cpp", "engine.cpp", "serve.cpp", "scenegen.cpp",
            "sceneio.cpp"]
 
 

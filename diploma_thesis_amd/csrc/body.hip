@@ -207,7 +207,7 @@ __device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int 
     if (ko.index) ko.index[i] = (unsigned)i;
     if (shist) {
         for (int pp = 0; pp < ko.histPasses; ++pp) {
-            const int sh0 = pp * ko.histRbits;
+            const int sh0 = ko.histBase + pp * ko.histRbits;
             const int bits = ko.histEndBit - sh0 < ko.histRbits ? ko.histEndBit - sh0 : ko.histRbits;
             const unsigned d = (unsigned)(key >> sh0) & ((1u << bits) - 1u);
             hist_add(shist + pp * 256, d);

@@ -1,0 +1,250 @@
+// bucket.hip -- the grid key's sort finished per top-digit bucket, fused with the reorder and the
+// cell table.
+//
+// After one stable onesweep pass on the key's top 8 bits, the keys of one top digit d are a
+// contiguous bucket, and the cells of that bucket are the table range [d << lowBits, (d + 1) << lowBits).
+// One workgroup per bucket then
+//   1. sorts the bucket by the low bits, stably: a bucket that fits in LDS is sorted there as packed
+//      words (low key << 16 | position in the bucket), so one 32-bit word carries key and order;
+//      a larger bucket is sorted in global memory by the same stable passes (slow, but correct);
+//   2. writes the sorted keys / indices and the gathered sorted copies (the reorder);
+//   3. writes its part of the cell table: table[c] = first sorted index with key >= c.
+// The result is the same as the stable LSD sort followed by the reorder, for any distribution.
+#include "csr_fill.hpp"
+#include "kernels.hpp"
+
+namespace arsim {
+
+namespace {
+
+constexpr int kBkThreads = 1024;
+constexpr int kBkWaves = kBkThreads / 64;
+constexpr int kBkCap = 8192;     // bucket size sorted in LDS (2 x 32 KiB of packed words; gfx950 has 160 KiB)
+constexpr int kBkSplit = 2;      // workgroups per bucket: each sorts it, then writes its share of the outputs
+
+// Lanes of the wave whose digit equals this lane's (among lanes with ok set).
+__device__ __forceinline__ uint64_t match_digit(unsigned d, int bits, bool ok) {
+    uint64_t peers = __ballot(ok);
+    for (int b = 0; b < bits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+__device__ __forceinline__ unsigned block_excl_scan(unsigned v, unsigned *sWave) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sWave[wave] = x;
+    __syncthreads();
+    unsigned before = 0u;
+    for (int w = 0; w < wave; ++w) before += sWave[w];
+    __syncthreads();
+    return before + x - v;
+}
+
+__device__ __forceinline__ unsigned block_sum(unsigned v, unsigned *sWave) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) sWave[wave] = v;
+    __syncthreads();
+    unsigned s = 0u;
+    for (int w = 0; w < kBkWaves; ++w) s += sWave[w];
+    __syncthreads();
+    return s;
+}
+
+// Item accessors of the stable pass: packed words in LDS, or (key, value) pairs in global memory.
+struct LdsItems {
+    unsigned *a;
+    __device__ unsigned digit(int i, int shift, unsigned mask) const { return (a[i] >> shift) & mask; }
+    __device__ void move_to(const LdsItems &dst, int pos, int i) const { dst.a[pos] = a[i]; }
+};
+struct GlobalItems {
+    unsigned *k, *v;
+    __device__ unsigned digit(int i, int shift, unsigned mask) const { return (k[i] >> shift) & mask; }
+    __device__ void move_to(const GlobalItems &dst, int pos, int i) const {
+        dst.k[pos] = k[i];
+        dst.v[pos] = v[i];
+    }
+};
+
+// One stable counting-sort pass of cnt items by the digit at `shift` (bits wide), src -> dst.
+// Wave w owns a contiguous segment of the items and walks it in order 64 at a time; a lane's rank
+// among equal digits of its group comes from ballots, so the order of equal digits is kept.
+template <class IT>
+__device__ void stable_pass(const IT &src, const IT &dst, int cnt, int shift, int bits, unsigned (*sCur)[256],
+                            unsigned *sWave) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned mask = (1u << bits) - 1u;
+    const int radix = 1 << bits;
+    const int seg = ((cnt + kBkThreads - 1) / kBkThreads) * 64;
+    const int s0 = min(wave * seg, cnt), s1 = min(s0 + seg, cnt);
+    const uint64_t lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (int k = threadIdx.x; k < kBkWaves * 256; k += kBkThreads) (&sCur[0][0])[k] = 0u;
+    __syncthreads();
+    for (int g = s0; g < s1; g += 64) {
+        const int i = g + lane;
+        const bool ok = i < s1;
+        const unsigned d = ok ? src.digit(i, shift, mask) : 0u;
+        const uint64_t peers = match_digit(d, bits, ok);
+        if (ok && (peers & lower) == 0ull) sCur[wave][d] += (unsigned)__popcll(peers);
+    }
+    __syncthreads();
+    // cursors: digits in order, and within a digit the waves in order
+    {
+        const int t = threadIdx.x;
+        unsigned c[kBkWaves], tot = 0u;
+        for (int w = 0; w < kBkWaves; ++w) {
+            c[w] = t < radix ? sCur[w][t] : 0u;
+            tot += c[w];
+        }
+        const unsigned excl = block_excl_scan(tot, sWave);
+        if (t < radix) {
+            unsigned run = excl;
+            for (int w = 0; w < kBkWaves; ++w) {
+                sCur[w][t] = run;
+                run += c[w];
+            }
+        }
+    }
+    __syncthreads();
+    for (int g = s0; g < s1; g += 64) {
+        const int i = g + lane;
+        const bool ok = i < s1;
+        const unsigned d = ok ? src.digit(i, shift, mask) : 0u;
+        const uint64_t peers = match_digit(d, bits, ok);
+        const unsigned prior = (unsigned)__popcll(peers & lower);
+        const unsigned base = ok ? sCur[wave][d] : 0u;
+        if (ok) src.move_to(dst, (int)(base + prior), i);
+        if (ok && prior == 0u) sCur[wave][d] = base + (unsigned)__popcll(peers);
+    }
+    __syncthreads();
+}
+
+// Fills table[a, b) with v using the whole workgroup (16-byte stores for the aligned middle).
+__device__ __forceinline__ void block_fill(unsigned *table, unsigned long long a, unsigned long long b, unsigned v) {
+    if (a >= b) return;
+    const unsigned long long a4 = (a + 3ull) & ~3ull, b4 = b & ~3ull;
+    if (a4 >= b4) {
+        for (unsigned long long k = a + threadIdx.x; k < b; k += kBkThreads) table[k] = v;
+        return;
+    }
+    if (threadIdx.x < a4 - a) table[a + threadIdx.x] = v;
+    if (threadIdx.x < b - b4) table[b4 + threadIdx.x] = v;
+    const uint4 v4 = make_uint4(v, v, v, v);
+    for (unsigned long long k = a4 + 4ull * threadIdx.x; k < b4; k += 4ull * kBkThreads)
+        *reinterpret_cast<uint4 *>(table + k) = v4;
+}
+
+__global__ __launch_bounds__(kBkThreads) void k_bucket_sort(unsigned *__restrict__ kIn, unsigned *__restrict__ vIn,
+                                                            unsigned *__restrict__ kOut, unsigned *__restrict__ vOut,
+                                                            const unsigned *__restrict__ hist, int lowBits,
+                                                            const float4 *__restrict__ rec,
+                                                            float4 *__restrict__ sPos, float4 *__restrict__ sVel,
+                                                            unsigned *__restrict__ table, unsigned numCells) {
+    __shared__ unsigned sA[kBkCap], sB[kBkCap];
+    __shared__ unsigned sCur[kBkWaves][256];
+    __shared__ unsigned sWave[kBkWaves];
+    const int t = threadIdx.x;
+    const int d = blockIdx.x / kBkSplit, part = blockIdx.x % kBkSplit;
+    const unsigned lowMask = (1u << lowBits) - 1u;
+    const unsigned start = block_sum(t < d && t < 256 ? hist[t] : 0u, sWave);
+    const int cnt = d < 256 ? (int)hist[d] : 0;
+    const bool inLds = cnt <= kBkCap;
+    // a bucket too large for LDS is sorted in global memory by its first workgroup alone
+    if (!inLds && part != 0) return;
+    const int nparts = inLds ? kBkSplit : 1;
+    const int i0 = (int)((long long)cnt * part / nparts), i1 = (int)((long long)cnt * (part + 1) / nparts);
+    unsigned *sorted = sA;   // LDS: packed words in sorted order
+    if (inLds) {
+        for (int i = t; i < cnt; i += kBkThreads) sA[i] = ((kIn[start + i] & lowMask) << 16) | (unsigned)i;
+        __syncthreads();
+        unsigned *src = sA, *dst = sB;
+        for (int sh = 0; sh < lowBits; sh += 8) {
+            stable_pass(LdsItems{src}, LdsItems{dst}, cnt, 16 + sh, min(8, lowBits - sh), sCur, sWave);
+            unsigned *x = src;
+            src = dst;
+            dst = x;
+        }
+        sorted = src;
+        const unsigned topBits = (unsigned)d << lowBits;
+        for (int i = i0 + t; i < i1; i += kBkThreads) {
+            const unsigned w = sorted[i];
+            const unsigned key = topBits | (w >> 16), val = vIn[start + (w & 0xffffu)];
+            kOut[start + i] = key;
+            vOut[start + i] = val;
+            sPos[start + i] = rec[2 * (size_t)val];
+            sVel[start + i] = rec[2 * (size_t)val + 1];
+        }
+    } else {
+        GlobalItems a{kIn + start, vIn + start}, b{kOut + start, vOut + start};
+        int passes = 0;
+        for (int sh = 0; sh < lowBits; sh += 8, ++passes) {
+            if ((passes & 1) == 0) stable_pass(a, b, cnt, sh, min(8, lowBits - sh), sCur, sWave);
+            else stable_pass(b, a, cnt, sh, min(8, lowBits - sh), sCur, sWave);
+        }
+        const GlobalItems &res = (passes & 1) ? b : a;
+        for (int i = t; i < cnt; i += kBkThreads) {
+            const unsigned key = res.k[i], val = res.v[i];
+            kOut[start + i] = key;
+            vOut[start + i] = val;
+            sPos[start + i] = rec[2 * (size_t)val];
+            sVel[start + i] = rec[2 * (size_t)val + 1];
+        }
+    }
+    __syncthreads();
+    // the bucket's cells [c0, c1): table[c] = start + (number of bucket keys below c)
+    const unsigned long long c0 = (unsigned long long)d << lowBits;
+    const unsigned long long cEnd = (unsigned long long)numCells + 1ull;
+    const unsigned long long c1 = c0 + (1ull << lowBits) < cEnd ? c0 + (1ull << lowBits) : cEnd;
+    if (cnt == 0) {
+        const unsigned long long n = c1 - c0;
+        block_fill(table, c0 + n * part / kBkSplit, c0 + n * (part + 1) / kBkSplit, start);
+        return;
+    }
+    auto low = [&](int i) -> unsigned { return inLds ? (sorted[i] >> 16) : (kOut[start + i] & lowMask); };
+    // head: cells up to the first key; tail: cells past the last key
+    const unsigned kFirst = low(0), kLast = low(cnt - 1);
+    if (part == 0) block_fill(table, c0, c0 + kFirst + 1ull < c1 ? c0 + kFirst + 1ull : c1, start);
+    if (part == nparts - 1) block_fill(table, c0 + kLast + 1ull, c1, start + (unsigned)cnt);
+    // between keys: the cells (k[i-1], k[i]] start at sorted position i (a lane per key, waves help
+    // with long runs); this workgroup's share of the positions
+    const int rounds = (i1 - i0 + kBkThreads - 1) / kBkThreads;
+    for (int r = 0; r < rounds; ++r) {
+        const int i = i0 + r * kBkThreads + t;
+        const bool ok = i > 0 && i < i1;
+        const unsigned kp = ok ? low(i - 1) : 0u, ki = ok ? low(i) : 0u;
+        const bool has = ok && ki != kp;
+        const unsigned a = (unsigned)c0 + kp + 1u, b = (unsigned)c0 + ki + 1u;
+        wave_fill(table, has, a, b, start + (unsigned)i);
+    }
+}
+
+}  // namespace
+
+bool bucket_sort_fits(long long n, int keyBits, unsigned long long numCells) {
+    if (keyBits < 9 || keyBits > 24) return false;
+    const int lowBits = keyBits - 8;
+    const unsigned long long buckets = (numCells >> lowBits) + 1ull;
+    // a bucket averages n / 256 keys; the LDS path takes up to kBkCap, larger ones go to global memory
+    return (double)n / (double)(buckets < 256ull ? buckets : 256ull) <= 0.75 * kBkCap;
+}
+
+void launch_bucket_sort(unsigned *kIn, unsigned *vIn, unsigned *kOut, unsigned *vOut, const unsigned *hist,
+                        int keyBits, const float4 *rec, float4 *sortedPos, float4 *sortedVel, unsigned *table,
+                        unsigned numCells, hipStream_t st) {
+    const int lowBits = keyBits - 8;
+    const unsigned buckets = (unsigned)((numCells >> lowBits) + 1u);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(buckets * kBkSplit), dim3(kBkThreads), 0, st, kIn, vIn, kOut, vOut, hist,
+                       lowBits, rec, sortedPos, sortedVel, table, numCells);
+}
+
+}  // namespace arsim

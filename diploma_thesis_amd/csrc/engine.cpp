@@ -49,11 +49,12 @@ ZeroJobs sort_zero_jobs(const SortPlan &pl) {
 }
 
 // Lets the key kernel accumulate the sort's digit histograms.
-void attach_hist(KeyOut &ko, const SortPlan &pl, const SortTuning &stt, int keyBits) {
+void attach_hist(KeyOut &ko, const SortPlan &pl, const SortTuning &stt, int keyBits, int beginBit = 0) {
     ko.hist = pl.hist;
     ko.histPasses = pl.passes;
     ko.histRbits = stt.radixBits;
     ko.histEndBit = keyBits;
+    ko.histBase = beginBit;
 }
 
 }  // namespace
@@ -210,15 +211,20 @@ PartBufs Engine::part_bufs() const {
     return p;
 }
 
-const char *const *Engine::stage_names(int method, unsigned flags, int *count) {
+const char *const *Engine::stage_names(int method, unsigned flags, bool bucket, int *count) {
+    // with the bucket sort the "sort" stage is the top-digit pass and the reorder also finishes the sort
     static const char *const sis[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "sis_find",
                                       "sis_gather", "sis_solve", "pose"};
-    if (flags & ARSIM_CONTACT_SIS) { *count = 8; return sis; }
+    static const char *const sisB[] = {"body_integrate", "pose_key", "sort_top_digit", "bucket_sort_reorder",
+                                       "sis_find", "sis_gather", "sis_solve", "pose"};
+    if (flags & ARSIM_CONTACT_SIS) { *count = 8; return bucket ? sisB : sis; }
     static const char *const grid[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "dem_grid",
                                        "reduce_apply"};
+    static const char *const gridB[] = {"body_integrate", "pose_key", "sort_top_digit", "bucket_sort_reorder",
+                                        "dem_grid", "reduce_apply"};
     static const char *const lbvh[] = {"body_integrate", "pose_key", "sort", "lbvh_build", "lbvh_refit", "dem_lbvh",
                                        "reduce_apply"};
-    if (method == ARSIM_METHOD_GRID) { *count = 6; return grid; }
+    if (method == ARSIM_METHOD_GRID) { *count = 6; return bucket ? gridB : grid; }
     *count = 7;
     return lbvh;
 }
@@ -332,6 +338,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         // beyond that, segments of 2^segBits cells keep the table at that size
         segBits = keyBits > 27 ? keyBits - 27 : 0;
         numSegs = (long long)((numCells - 1) >> segBits) + 1;
+        bucketSort = segBits == 0 && bucket_sort_fits(nTot, keyBits, numCells);
         if ((flags & ARSIM_CONTACT_SIS) && segBits != 0)
             return fail(ARSIM_ERR_UNSUPPORTED, "engine: the sequential-impulse model needs at most 2^27 cells");
         long long base = 0;
@@ -516,6 +523,45 @@ int Engine::reset_grid_tables() {
     return ARSIM_OK;
 }
 
+// Body integration, pose + keys, the sort and the reorder / cell table of the grid methods (the DEM
+// and the SIS steps share them).  Records the stage boundaries through rec; on return *kAOut / *vAOut
+// hold the sorted keys and particle indices.
+template <class Rec>
+int Engine::enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, const SceneView &sv, const SortTuning &tuIn,
+                                    unsigned **kAOut, unsigned **vAOut, Rec &rec) {
+    SortTuning stt = tuIn;
+    // the bucket path sorts the top 8 bits in one onesweep pass and the rest per bucket
+    const int beginBit = bucketSort ? keyBits - 8 : 0;
+    if (bucketSort) stt.radixBits = 8;
+    const SortPlan pl = sort_plan(sortWs, N, keyBytes, beginBit, keyBits, stt);
+    ZeroJobs z = sort_zero_jobs(pl);
+    z.p[3] = longCnt;   // the cell table's long-run count
+    z.count[3] = 1;
+    launch_body_integrate(bb, B, ph, &z, st);
+    rec();
+    KeyOut ko{keysA, nullptr, nullptr};
+    attach_hist(ko, pl, stt, keyBits, beginBit);
+    launch_pose_key(pb, bb, N, ph, sv, kKeyGrid, ko, true, true, st);
+    rec();
+    unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
+    if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, beginBit, keyBits, true, sortWs, sortWsBytes, stt, st, true))
+        return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
+    rec();
+    if (bucketSort) {
+        launch_bucket_sort(kA, vA, kB, vB, pl.hist, keyBits, gatherRec, sortedPos, sortedVel, segStart,
+                           (unsigned)numCells, st);
+        std::swap(kA, kB);
+        std::swap(vA, vB);
+    } else {
+        launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart,
+                           segBits, numSegs, N, long_fill(), st);
+    }
+    rec();
+    *kAOut = kA;
+    *vAOut = vA;
+    return ARSIM_OK;
+}
+
 int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
     const BodyBufs bb = body_bufs();
     const PartBufs pb = part_bufs();
@@ -532,23 +578,8 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
     if (flags & ARSIM_CONTACT_SIS) {
         // updateUniformGridSIS (RigidBodies/UniformGridUpdate.cpp:734-797): integrate and pose, the grid
         // broadphase, deepest contacts (static set first), the contact list, the sweeps, the pose again
-        const SortPlan pl = sort_plan(sortWs, N, keyBytes, 0, keyBits, stt);
-        ZeroJobs z = sort_zero_jobs(pl);
-        z.p[3] = longCnt;   // the cell table's long-run count
-        z.count[3] = 1;
-        launch_body_integrate(bb, B, ph, &z, st);
-        rec();
-        KeyOut ko{keysA, nullptr, nullptr};
-        attach_hist(ko, pl, stt, keyBits);
-        launch_pose_key(pb, bb, N, ph, sv, kKeyGrid, ko, true, true, st);
-        rec();
-        unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
-        if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st, true))
-            return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
-        rec();
-        launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart,
-                           segBits, numSegs, N, long_fill(), st);
-        rec();
+        unsigned *kA = nullptr, *vA = nullptr;
+        if (int rc = enqueue_grid_broadphase(bb, pb, sv, stt, &kA, &vA, rec)) return rc;
         if (numStatic)
             launch_sis_find_static(sortedPos, vA, N, stPos, nullptr, stTable, nullptr, geoms[0], ph.r, false, sisDistA,
                                    sisPartA, nullptr, st);
@@ -574,23 +605,8 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
     } else if (method == ARSIM_METHOD_GRID) {
         // the sort's workspace is cleared by the body launch and its histograms are built while the
         // keys are generated, so the sort itself is only its digit passes
-        const SortPlan pl = sort_plan(sortWs, N, keyBytes, 0, keyBits, stt);
-        ZeroJobs z = sort_zero_jobs(pl);
-        z.p[3] = longCnt;   // the cell table's long-run count
-        z.count[3] = 1;
-        launch_body_integrate(bb, B, ph, &z, st);
-        rec();
-        KeyOut ko{keysA, nullptr, nullptr};
-        attach_hist(ko, pl, stt, keyBits);
-        launch_pose_key(pb, bb, N, ph, sv, kKeyGrid, ko, true, true, st);
-        rec();
-        unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
-        if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st, true))
-            return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
-        rec();
-        launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart,
-                           segBits, numSegs, N, long_fill(), st);
-        rec();
+        unsigned *kA = nullptr, *vA = nullptr;
+        if (int rc = enqueue_grid_broadphase(bb, pb, sv, stt, &kA, &vA, rec)) return rc;
         DemOut o{};
         o.pForce = pForce; o.vel = vel; o.relPos = relPos;
         launch_dem_grid_seg(sortedPos, sortedVel, kA, vA, segStart, segBits, N, ph, sv, S > 1, o, tu.demBlock, st);
@@ -952,7 +968,7 @@ int arsim_engine_profile_step(arsim_engine *e, float *ms, const char **names, in
     if (!e) return fail(ARSIM_ERR_INVALID_ARG, "null engine");
     Engine &E = e->e;
     int count = 0;
-    const char *const *nm = Engine::stage_names(E.method, E.flags, &count);
+    const char *const *nm = Engine::stage_names(E.method, E.flags, E.bucketSort, &count);
     std::vector<hipEvent_t> ev(count + 1);
     for (auto &x : ev) ARSIM_HIP(hipEventCreate(&x));
     int ns = 0;

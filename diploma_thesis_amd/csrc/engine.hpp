@@ -57,6 +57,7 @@ struct Engine {
     uint4 *longList = nullptr;
     unsigned longCap = 0;
     int segBits = 3;
+    bool bucketSort = false;                // grid: sort finished per top-digit bucket (bucket.hip)
     long long numSegs = 0;
     float4 *sortedPos = nullptr, *sortedVel = nullptr;
     void *sortWs = nullptr;
@@ -97,7 +98,7 @@ struct Engine {
     int set_static(const float *pos4, const float *norm4, int n);
     void free_static();
     int reset_grid_tables();
-    static const char *const *stage_names(int method, unsigned flags, int *count);
+    static const char *const *stage_names(int method, unsigned flags, bool bucket, int *count);
     SceneView scene_view() const;
     BodyBufs body_bufs() const;
     StateBufs state_bufs() const;
@@ -108,6 +109,9 @@ struct Engine {
   private:
     template <class T> int alloc(T **p, size_t count);
     int alloc_sis(int cap);
+    template <class Rec>
+    int enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, const SceneView &sv, const SortTuning &stt,
+                                unsigned **kAOut, unsigned **vAOut, Rec &rec);
 };
 
 }  // namespace arsim

@@ -26,7 +26,8 @@ int sort_pairs(K **keys, unsigned **vals, K **alt, unsigned **altVals, int n, in
                bool valsIota, void *ws, size_t wsBytes, const SortTuning &tu, hipStream_t st, bool prepared = false);
 // Where the sort keeps its digit histograms, tile counters and look-back words inside the workspace.
 // A caller may zero {hist, counters, statusA} and accumulate the histograms itself (for instance while
-// it generates the keys) and then call sort_pairs(..., prepared = true), which skips both steps.
+// it generates the keys) and then call sort_pairs(..., prepared = true), which skips both steps; the
+// histogram of pass p is that of the digit at beginBit + p * radixBits.
 struct SortPlan {
     unsigned *hist, *counters, *statusA, *statusB;
     int numTiles, passes, radix;
@@ -64,6 +65,7 @@ struct KeyOut {
     const float4 *box;            // dynamic Morton box (min, max) on the device, or NULL
     unsigned *hist = nullptr;     // optional: the sort's digit histograms, accumulated here
     int histPasses = 0, histRbits = 8, histEndBit = 32;
+    int histBase = 0;             // bit of the first digit
 };
 
 // Up to three device ranges zeroed by extra blocks of the body-integration launch.
@@ -132,6 +134,15 @@ void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, c
                         const int *rb, const uint16_t *scene, const float4 *rec, float4 *sortedPos,
                         float4 *sortedVel, unsigned *segStart, int segBits, long long numSegs, int n,
                         const LongFill &lf, hipStream_t st);
+// The grid sort finished per top-digit bucket (bucket.hip): after one stable pass on bits
+// [keyBits - 8, keyBits) of (kIn, vIn), one workgroup per bucket sorts it by the low bits, writes the
+// sorted keys / indices to (kOut, vOut), the gathered sorted copies and its range of the exact cell
+// table.  hist = the top digit's histogram.  bucket_sort_fits: keyBits in [9, 24] and buckets that
+// mostly fit in LDS for n keys.
+bool bucket_sort_fits(long long n, int keyBits, unsigned long long numCells);
+void launch_bucket_sort(unsigned *kIn, unsigned *vIn, unsigned *kOut, unsigned *vOut, const unsigned *hist,
+                        int keyBits, const float4 *rec, float4 *sortedPos, float4 *sortedVel, unsigned *table,
+                        unsigned numCells, hipStream_t st);
 // Capacity of the long-run list of a table of tableSize entries (runs are disjoint and longer than
 // kLongRange), and the launch that fills the listed runs.
 unsigned fill_long_capacity(unsigned long long tableSize);

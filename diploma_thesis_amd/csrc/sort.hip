@@ -282,7 +282,6 @@ int sort_pairs(K **keys, unsigned **vals, K **alt, unsigned **altVals, int n, in
     const int keyBits = (int)sizeof(K) * 8;
     if (beginBit < 0 || endBit > keyBits || beginBit >= endBit) return -1;
     if (wsBytes < sort_workspace_bytes(n, (int)sizeof(K))) return -2;
-    if (prepared && beginBit != 0) return -1;
     const int ipt = tu.ipt, rbits = tu.radixBits;
     const SortPlan pl = sort_plan(ws, n, (int)sizeof(K), beginBit, endBit, tu);
     const int numTiles = pl.numTiles;

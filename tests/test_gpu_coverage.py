@@ -140,3 +140,49 @@ def test_engine_cell_table_with_long_empty_runs():
     st = oracle.State(sc, np.float32)
     st.step(4)
     assert_state_equal(g, st, "sparse grid")
+
+
+def test_bucket_sort_overflowing_bucket():
+    """7,000 grains in one slab of a 64^3 grid: their keys share the top 8 bits, so one bucket of the
+    per-bucket sort holds more keys than LDS does and is sorted in global memory."""
+    print("engine: bucket sort, one bucket past the LDS capacity", flush=True)
+    sc = scenes.mix_scene(8000, 95, capi.METHOD_GRID)
+    rng = np.random.default_rng(95)
+    g = np.where(sc.rb < 0)[0]
+    assert len(g) >= 1900
+    sc2 = sc.copy()
+    # put every grain, and fill up to 7,000 more with a second copy of grains, into z in [10, 11), y in [1, 15)
+    k = len(g)
+    sc2.pos[g, 0] = rng.uniform(1.0, 39.0, k).astype(np.float32)
+    sc2.pos[g, 1] = rng.uniform(1.0, 15.0, k).astype(np.float32)
+    sc2.pos[g, 2] = rng.uniform(10.05, 10.95, k).astype(np.float32)
+    extra = 7000 - k
+    add_pos = np.zeros((extra, 4), np.float32)
+    add_pos[:, 0] = rng.uniform(1.0, 39.0, extra)
+    add_pos[:, 1] = rng.uniform(1.0, 15.0, extra)
+    add_pos[:, 2] = rng.uniform(10.05, 10.95, extra)
+    sc2.pos = np.concatenate([sc2.pos, add_pos])
+    sc2.vel = np.concatenate([sc2.vel, np.zeros_like(add_pos)])
+    sc2.rb = np.concatenate([sc2.rb, np.full(extra, -1, np.int32)])
+    sc2.body_frame = np.concatenate([sc2.body_frame, np.zeros_like(add_pos)])
+    p = sc2.params
+    keys, _ = oracle.calc_hash(sc2.pos, p)
+    top = keys >> 10
+    assert np.bincount(top).max() > 6144 and sc2.n / 256 < 4608
+    with Engine(sc2, use_graph=False) as e:
+        e.step(2)
+        e.sync()
+        kp, tp = C.c_void_p(), C.c_void_p()
+        nul = None
+        capi.check(L.arsim_engine_debug_buffers(e.h, C.byref(kp), nul, C.byref(tp), nul, nul, nul, nul, nul, nul, nul))
+        total = int(p.numCells)
+        kk = np.zeros(sc2.n, np.uint32)
+        table = np.zeros(total + 1, np.uint32)
+        capi.check(L.arsim_memcpy_dtoh(capi.ptr(kk), kp, kk.nbytes, None))
+        capi.check(L.arsim_memcpy_dtoh(capi.ptr(table), tp, table.nbytes, None))
+        g2 = e.download()
+    assert np.all(kk[1:] >= kk[:-1])
+    assert np.array_equal(table, np.searchsorted(kk, np.arange(total + 1, dtype=np.int64), "left").astype(np.uint32))
+    st = oracle.State(sc2, np.float32)
+    st.step(2)
+    assert_state_equal(g2, st, "overflowing bucket")
