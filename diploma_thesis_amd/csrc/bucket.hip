@@ -19,8 +19,12 @@ namespace {
 
 constexpr int kBkThreads = 1024;
 constexpr int kBkWaves = kBkThreads / 64;
-constexpr int kBkCap = 8192;     // bucket size sorted in LDS (2 x 32 KiB of packed words; gfx950 has 160 KiB)
+// Bucket size sorted in LDS: two buffers of packed words plus the 16 KiB cursor table and the wave
+// sums stay within 80 KiB, half of a gfx950 CU's 160 KiB, so two workgroups share a CU.
+constexpr int kBkCap = 8160;
 constexpr int kBkSplit = 2;      // workgroups per bucket: each sorts it, then writes its share of the outputs
+constexpr int kBkLdsBytes = 2 * kBkCap * 4 + kBkWaves * 256 * 4 + kBkWaves * 4;
+static_assert(kBkLdsBytes <= 80 * 1024, "two bucket workgroups must fit in one CU's LDS");
 
 // Lanes of the wave whose digit equals this lane's (among lanes with ok set).
 __device__ __forceinline__ uint64_t match_digit(unsigned d, int bits, bool ok) {
@@ -163,7 +167,33 @@ __global__ __launch_bounds__(kBkThreads) void k_bucket_sort(unsigned *__restrict
     if (!inLds && part != 0) return;
     const int nparts = inLds ? kBkSplit : 1;
     const int i0 = (int)((long long)cnt * part / nparts), i1 = (int)((long long)cnt * (part + 1) / nparts);
-    unsigned *sorted = sA;   // LDS: packed words in sorted order
+    // the bucket's cells [c0, c1): table[c] = start + (number of bucket keys below c)
+    const unsigned long long c0 = (unsigned long long)d << lowBits;
+    const unsigned long long cEnd = (unsigned long long)numCells + 1ull;
+    const unsigned long long c1 = c0 + (1ull << lowBits) < cEnd ? c0 + (1ull << lowBits) : cEnd;
+    if (cnt == 0) {
+        const unsigned long long n = c1 - c0;
+        block_fill(table, c0 + n * part / kBkSplit, c0 + n * (part + 1) / kBkSplit, start);
+        return;
+    }
+    // low(i) = low bits of the i-th sorted key of the bucket
+    auto fill_table = [&](auto low) {
+        // head: cells up to the first key; tail: cells past the last key
+        const unsigned kFirst = low(0), kLast = low(cnt - 1);
+        if (part == 0) block_fill(table, c0, c0 + kFirst + 1ull < c1 ? c0 + kFirst + 1ull : c1, start);
+        if (part == nparts - 1) block_fill(table, c0 + kLast + 1ull, c1, start + (unsigned)cnt);
+        // between keys: the cells (k[i-1], k[i]] start at sorted position i (a lane per key, waves help
+        // with long runs); this workgroup's share of the positions
+        const int rounds = (i1 - i0 + kBkThreads - 1) / kBkThreads;
+        for (int r = 0; r < rounds; ++r) {
+            const int i = i0 + r * kBkThreads + t;
+            const bool ok = i > 0 && i < i1;
+            const unsigned kp = ok ? low(i - 1) : 0u, ki = ok ? low(i) : 0u;
+            const bool has = ok && ki != kp;
+            const unsigned a = (unsigned)c0 + kp + 1u, b = (unsigned)c0 + ki + 1u;
+            wave_fill(table, has, a, b, start + (unsigned)i);
+        }
+    };
     if (inLds) {
         for (int i = t; i < cnt; i += kBkThreads) sA[i] = ((kIn[start + i] & lowMask) << 16) | (unsigned)i;
         __syncthreads();
@@ -174,12 +204,34 @@ __global__ __launch_bounds__(kBkThreads) void k_bucket_sort(unsigned *__restrict
             src = dst;
             dst = x;
         }
-        sorted = src;
+        const unsigned *sorted = src;   // packed words in sorted order
         const unsigned topBits = (unsigned)d << lowBits;
-        for (int i = i0 + t; i < i1; i += kBkThreads) {
+        // the gathers of the first two rounds of positions are issued before the table is filled, so
+        // that their latency overlaps the table's stores; a lane past the part's end loads sorted
+        // position 0 (the bucket is not empty) and stores nothing, so the loads need no branch
+        const int ia = i0 + t, ib = ia + kBkThreads;
+        const bool ha = ia < i1, hb = ib < i1;
+        const unsigned wa = sorted[ha ? ia : 0], wb = sorted[hb ? ib : 0];
+        const unsigned va = vIn[start + (wa & 0xffffu)], vb = vIn[start + (wb & 0xffffu)];
+        const float4 pa = rec[2 * (size_t)va], qa = rec[2 * (size_t)va + 1];
+        const float4 pb = rec[2 * (size_t)vb], qb = rec[2 * (size_t)vb + 1];
+        fill_table([&](int i) -> unsigned { return sorted[i] >> 16; });
+        if (ha) {
+            kOut[start + ia] = topBits | (wa >> 16);
+            vOut[start + ia] = va;
+            sPos[start + ia] = pa;
+            sVel[start + ia] = qa;
+        }
+        if (hb) {
+            kOut[start + ib] = topBits | (wb >> 16);
+            vOut[start + ib] = vb;
+            sPos[start + ib] = pb;
+            sVel[start + ib] = qb;
+        }
+        for (int i = ib + kBkThreads; i < i1; i += kBkThreads) {
             const unsigned w = sorted[i];
-            const unsigned key = topBits | (w >> 16), val = vIn[start + (w & 0xffffu)];
-            kOut[start + i] = key;
+            const unsigned val = vIn[start + (w & 0xffffu)];
+            kOut[start + i] = topBits | (w >> 16);
             vOut[start + i] = val;
             sPos[start + i] = rec[2 * (size_t)val];
             sVel[start + i] = rec[2 * (size_t)val + 1];
@@ -199,32 +251,8 @@ __global__ __launch_bounds__(kBkThreads) void k_bucket_sort(unsigned *__restrict
             sPos[start + i] = rec[2 * (size_t)val];
             sVel[start + i] = rec[2 * (size_t)val + 1];
         }
-    }
-    __syncthreads();
-    // the bucket's cells [c0, c1): table[c] = start + (number of bucket keys below c)
-    const unsigned long long c0 = (unsigned long long)d << lowBits;
-    const unsigned long long cEnd = (unsigned long long)numCells + 1ull;
-    const unsigned long long c1 = c0 + (1ull << lowBits) < cEnd ? c0 + (1ull << lowBits) : cEnd;
-    if (cnt == 0) {
-        const unsigned long long n = c1 - c0;
-        block_fill(table, c0 + n * part / kBkSplit, c0 + n * (part + 1) / kBkSplit, start);
-        return;
-    }
-    auto low = [&](int i) -> unsigned { return inLds ? (sorted[i] >> 16) : (kOut[start + i] & lowMask); };
-    // head: cells up to the first key; tail: cells past the last key
-    const unsigned kFirst = low(0), kLast = low(cnt - 1);
-    if (part == 0) block_fill(table, c0, c0 + kFirst + 1ull < c1 ? c0 + kFirst + 1ull : c1, start);
-    if (part == nparts - 1) block_fill(table, c0 + kLast + 1ull, c1, start + (unsigned)cnt);
-    // between keys: the cells (k[i-1], k[i]] start at sorted position i (a lane per key, waves help
-    // with long runs); this workgroup's share of the positions
-    const int rounds = (i1 - i0 + kBkThreads - 1) / kBkThreads;
-    for (int r = 0; r < rounds; ++r) {
-        const int i = i0 + r * kBkThreads + t;
-        const bool ok = i > 0 && i < i1;
-        const unsigned kp = ok ? low(i - 1) : 0u, ki = ok ? low(i) : 0u;
-        const bool has = ok && ki != kp;
-        const unsigned a = (unsigned)c0 + kp + 1u, b = (unsigned)c0 + ki + 1u;
-        wave_fill(table, has, a, b, start + (unsigned)i);
+        __syncthreads();
+        fill_table([&](int i) -> unsigned { return kOut[start + i] & lowMask; });
     }
 }
 

@@ -143,20 +143,20 @@ def test_engine_cell_table_with_long_empty_runs():
 
 
 def test_bucket_sort_overflowing_bucket():
-    """7,000 grains in one slab of a 64^3 grid: their keys share the top 8 bits, so one bucket of the
-    per-bucket sort holds more keys than LDS does and is sorted in global memory."""
+    """9,000 grains in one slab of a 64^3 grid: their keys share the top 8 bits, so one bucket of the
+    per-bucket sort holds more keys than LDS does (8,160) and is sorted in global memory."""
     print("engine: bucket sort, one bucket past the LDS capacity", flush=True)
     sc = scenes.mix_scene(8000, 95, capi.METHOD_GRID)
     rng = np.random.default_rng(95)
     g = np.where(sc.rb < 0)[0]
     assert len(g) >= 1900
     sc2 = sc.copy()
-    # put every grain, and fill up to 7,000 more with a second copy of grains, into z in [10, 11), y in [1, 15)
+    # put every grain, and more grains up to 9,000 in all, into z in [10, 11), y in [1, 15)
     k = len(g)
     sc2.pos[g, 0] = rng.uniform(1.0, 39.0, k).astype(np.float32)
     sc2.pos[g, 1] = rng.uniform(1.0, 15.0, k).astype(np.float32)
     sc2.pos[g, 2] = rng.uniform(10.05, 10.95, k).astype(np.float32)
-    extra = 7000 - k
+    extra = 9000 - k
     add_pos = np.zeros((extra, 4), np.float32)
     add_pos[:, 0] = rng.uniform(1.0, 39.0, extra)
     add_pos[:, 1] = rng.uniform(1.0, 15.0, extra)
@@ -168,7 +168,7 @@ def test_bucket_sort_overflowing_bucket():
     p = sc2.params
     keys, _ = oracle.calc_hash(sc2.pos, p)
     top = keys >> 10
-    assert np.bincount(top).max() > 6144 and sc2.n / 256 < 4608
+    assert np.bincount(top).max() > 8160 and sc2.n / 256 < 0.75 * 8160
     with Engine(sc2, use_graph=False) as e:
         e.step(2)
         e.sync()
