@@ -198,6 +198,9 @@ __device__ __forceinline__ void scan_cells(const unsigned *__restrict__ keys, co
 // Contacts of one sorted particle with the sorted run [j, e) that holds up to three consecutive
 // cells, the second starting at b1 and the third at b2 (exact cell offsets: no key is read).
 // A cell's partial is added to F when a non-empty cell ends, as in the 27-cell loop.
+// Candidates are loaded two at a time (the second index clamped into the run), so that a wave has
+// two loads in flight per lane: the loop is bound by load latency, not by the force arithmetic.
+// Three or more per round trip cost more occupancy (registers) than they hide.
 template <bool EMIT>
 __device__ __forceinline__ void scan_run(const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
                                          const unsigned *__restrict__ sIdx, unsigned j, unsigned b1, unsigned b2,
@@ -205,25 +208,27 @@ __device__ __forceinline__ void scan_run(const float4 *__restrict__ sPos, const 
                                          unsigned orig, const DemOut &o) {
     const unsigned j0 = j;
     f3 csum = mk3(0.f, 0.f, 0.f);
-    // the next candidate is loaded while the current one is tested
-    float4 Qn = j < e ? sPos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
-    for (; j < e; ++j) {
-        if (j != j0 && (j == b1 || j == b2)) {
+    auto visit = [&](unsigned jj, const float4 &Q) {
+        if (jj != j0 && (jj == b1 || jj == b2)) {
             F = add3(F, csum);
             csum = mk3(0.f, 0.f, 0.f);
         }
-        const float4 Q = Qn;
-        if (j + 1 < e) Qn = sPos[j + 1];
-        if (!candidate(Q, pa, ra, (int)j == i, ph)) continue;
+        if (!candidate(Q, pa, ra, (int)jj == i, ph)) return;
         if (EMIT) {
             if (in_contact(pa, xyz(Q), ph)) {
-                if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = (int)sIdx[j];
+                if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = (int)sIdx[jj];
                 ++cnt;
             }
         } else {
             f3 fj;
-            if (pair_force(pa, xyz(Q), va, xyz(sVel[j]), ph, fj)) csum = add3(csum, fj);
+            if (pair_force(pa, xyz(Q), va, xyz(sVel[jj]), ph, fj)) csum = add3(csum, fj);
         }
+    };
+    for (; j < e; j += 2u) {
+        const float4 Q0 = sPos[j];
+        const float4 Q1 = sPos[j + 1u < e ? j + 1u : j];
+        visit(j, Q0);
+        if (j + 1u < e) visit(j + 1u, Q1);
     }
     if (e > j0) F = add3(F, csum);
 }
