@@ -26,13 +26,18 @@ constexpr int kBkSplit = 2;      // workgroups per bucket: each sorts it, then w
 constexpr int kBkLdsBytes = 2 * kBkCap * 4 + kBkWaves * 256 * 4 + kBkWaves * 4;
 static_assert(kBkLdsBytes <= 80 * 1024, "two bucket workgroups must fit in one CU's LDS");
 
-// Lanes of the wave whose digit equals this lane's (among lanes with ok set).
+// Lanes of the wave whose digit (bits <= 8 wide) equals this lane's, among lanes with ok set.  Per
+// bit: the ballot of lanes with the bit set, kept as is by lanes with the bit set and complemented
+// by the others (xnor with the lane's bit spread to a mask).
 __device__ __forceinline__ uint64_t match_digit(unsigned d, int bits, bool ok) {
     uint64_t peers = __ballot(ok);
-    for (int b = 0; b < bits; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        if (b >= bits) break;
+        const unsigned t = 0u - ((d >> b) & 1u);
+        const uint64_t m = __ballot(t != 0u);
+        const unsigned lo = ~((unsigned)m ^ t), hi = ~((unsigned)(m >> 32) ^ t);
+        peers &= ((uint64_t)hi << 32) | lo;
     }
     return peers;
 }
@@ -81,8 +86,10 @@ struct GlobalItems {
 };
 
 // One stable counting-sort pass of cnt items by the digit at `shift` (bits wide), src -> dst.
-// Wave w owns a contiguous segment of the items and walks it in order 64 at a time; a lane's rank
-// among equal digits of its group comes from ballots, so the order of equal digits is kept.
+// Wave w owns a contiguous segment of the items: it counts its digits with LDS atomics, then walks
+// the segment in order 64 at a time, a lane's rank among equal digits of its group coming from
+// ballots, so the order of equal digits is kept.  (The ballots are the pass's VALU cost: a wave64
+// instruction takes four cycles, so the count phase does without them.)
 template <class IT>
 __device__ void stable_pass(const IT &src, const IT &dst, int cnt, int shift, int bits, unsigned (*sCur)[256],
                             unsigned *sWave) {
@@ -94,13 +101,8 @@ __device__ void stable_pass(const IT &src, const IT &dst, int cnt, int shift, in
     const uint64_t lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     for (int k = threadIdx.x; k < kBkWaves * 256; k += kBkThreads) (&sCur[0][0])[k] = 0u;
     __syncthreads();
-    for (int g = s0; g < s1; g += 64) {
-        const int i = g + lane;
-        const bool ok = i < s1;
-        const unsigned d = ok ? src.digit(i, shift, mask) : 0u;
-        const uint64_t peers = match_digit(d, bits, ok);
-        if (ok && (peers & lower) == 0ull) sCur[wave][d] += (unsigned)__popcll(peers);
-    }
+    // the counts need no order: an LDS atomic per item
+    for (int i = s0 + lane; i < s1; i += 64) atomicAdd(&sCur[wave][src.digit(i, shift, mask)], 1u);
     __syncthreads();
     // cursors: digits in order, and within a digit the waves in order
     {
@@ -148,7 +150,8 @@ __device__ __forceinline__ void block_fill(unsigned *table, unsigned long long a
         *reinterpret_cast<uint4 *>(table + k) = v4;
 }
 
-__global__ __launch_bounds__(kBkThreads) void k_bucket_sort(unsigned *__restrict__ kIn, unsigned *__restrict__ vIn,
+// eight waves per SIMD (two workgroups per CU): registers are held to that occupancy
+__global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bucket_sort(unsigned *__restrict__ kIn, unsigned *__restrict__ vIn,
                                                             unsigned *__restrict__ kOut, unsigned *__restrict__ vOut,
                                                             const unsigned *__restrict__ hist, int lowBits,
                                                             const float4 *__restrict__ rec,
