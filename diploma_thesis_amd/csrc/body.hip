@@ -429,11 +429,13 @@ void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, con
                      int mode, const KeyOut &ko, bool doBodies, bool doGrains, hipStream_t st) {
     if (numParticles <= 0) return;
     const int block = 256;
-    // with digit histograms, at most 512 blocks (2 per CU): each flushes its LDS histogram once with
-    // one global atomic per non-empty digit, and at 2^20 keys those atomics cost more than the
-    // occupancy that more blocks would add (512 blocks: 28.8 us; 1024: 29.4; 2048: 32.2; 384: 31.2)
+    // With digit histograms the blocks are capped: each flushes its LDS histogram once with one global
+    // atomic per non-empty digit, and those atomics cost more than the occupancy that more blocks
+    // would add.  Measured: 2^20 keys 512 blocks 28.8 us, 1024 29.4, 2048 32.2, 384 31.2; 4.2 M keys
+    // 1024 blocks 98 us, 512 129, 2048 109; 8.9 M keys 1024 blocks 250 us, 512 350, 2048 257.
     const long long need = blocks_for(numParticles, block);
-    dim3 grid((unsigned)(ko.hist && mode != kKeyNone && need > 512 ? 512 : need));
+    const long long cap = need <= 4096 ? 512 : 1024;
+    dim3 grid((unsigned)(ko.hist && mode != kKeyNone && need > cap ? cap : need));
     int db = doBodies ? 1 : 0, dg = doGrains ? 1 : 0;
     switch (mode) {
         case kKeyGrid:
