@@ -144,6 +144,7 @@ def test_profile_and_autotune_leave_state_consistent():
 
 def test_serve_small_stream():
     print("serve: small stream", flush=True)
+    work = []
     for one in (0, 1):
         cfg = capi.ArSimServeConfig(seed=1, numRequests=6, arrivalRate=200.0, minParticles=2048, maxParticles=6000,
                                     minSteps=5, maxSteps=20, hbmBudgetBytes=8e9, maxResident=64, oneAtATime=one,
@@ -151,6 +152,9 @@ def test_serve_small_stream():
         rep = capi.ArSimServeReport()
         capi.check(capi.lib.arsim_serve_run(C.byref(cfg), C.byref(rep)), "serve_run")
         assert rep.completed == 6 and rep.particleUpdates > 0 and rep.stepLatencyP99Ms >= rep.stepLatencyP50Ms
+        work.append(rep.particleUpdates)
+    # every request's steps are counted once, however the batch retires and admits scenes
+    assert work[0] == work[1]
 
 
 @pytest.mark.parametrize("method", [capi.METHOD_GRID, capi.METHOD_LBVH30])
