@@ -186,3 +186,34 @@ def test_bucket_sort_overflowing_bucket():
     st = oracle.State(sc2, np.float32)
     st.step(2)
     assert_state_equal(g2, st, "overflowing bucket")
+
+
+@pytest.mark.parametrize("g", [8, 16, 32, 128])
+def test_bucket_sort_key_widths(g):
+    """The bucket path at key widths 9, 12, 15 and 21 bits (grids 8^3 ... 128^3): low-key sorts of one
+    pass (1 and 4 bits), of a full and a partial pass (7 bits) and of two passes (13 bits).  The cell
+    table is exact CSR and the state matches the oracle after graph-replayed steps."""
+    print(f"engine: bucket sort, grid {g}^3", flush=True)
+    sc = scenes.mix_scene(6000, 100 + g, capi.METHOD_GRID)
+    p = sc.params
+    p.gridSize.x = p.gridSize.y = p.gridSize.z = g
+    p.numCells = g ** 3
+    total = g ** 3
+    with Engine(sc, use_graph=True) as e:
+        e.step(3)
+        e.sync()
+        kp, tp = C.c_void_p(), C.c_void_p()
+        nul = None
+        capi.check(L.arsim_engine_debug_buffers(e.h, C.byref(kp), nul, C.byref(tp), nul, nul, nul, nul, nul, nul, nul))
+        keys = np.zeros(sc.n, np.uint32)
+        table = np.zeros(total + 1, np.uint32)
+        capi.check(L.arsim_memcpy_dtoh(capi.ptr(keys), kp, keys.nbytes, None))
+        capi.check(L.arsim_memcpy_dtoh(capi.ptr(table), tp, table.nbytes, None))
+        got = e.download()
+        stages = [name for name, _ in e.profile()]   # one more step, after the state was read
+    assert "bucket_sort_reorder" in stages, stages
+    assert np.all(keys[1:] >= keys[:-1]) and int(keys[-1]) < total
+    assert np.array_equal(table, np.searchsorted(keys, np.arange(total + 1, dtype=np.int64), "left").astype(np.uint32))
+    st = oracle.State(sc, np.float32)
+    st.step(3)
+    assert_state_equal(got, st, f"grid {g}")
