@@ -169,7 +169,7 @@ __device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int 
             p.pos[i] = P;
             Vfin = f4(v, Vv.w);
             haveV = true;
-            p.vel[i] = Vfin;
+            if (!ko.grainVelDead) p.vel[i] = Vfin;
         }
     }
     if (MODE == kKeyNone) return;

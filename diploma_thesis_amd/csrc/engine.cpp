@@ -528,7 +528,7 @@ int Engine::reset_grid_tables() {
 // hold the sorted keys and particle indices.
 template <class Rec>
 int Engine::enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, const SceneView &sv, const SortTuning &tuIn,
-                                    unsigned **kAOut, unsigned **vAOut, Rec &rec) {
+                                    bool grainVelDead, unsigned **kAOut, unsigned **vAOut, Rec &rec) {
     SortTuning stt = tuIn;
     // the bucket path sorts the top 8 bits in one onesweep pass and the rest per bucket
     const int beginBit = bucketSort ? keyBits - 8 : 0;
@@ -541,6 +541,7 @@ int Engine::enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, cons
     rec();
     KeyOut ko{keysA, nullptr, nullptr};
     attach_hist(ko, pl, stt, keyBits, beginBit);
+    ko.grainVelDead = grainVelDead;
     launch_pose_key(pb, bb, N, ph, sv, kKeyGrid, ko, true, true, st);
     rec();
     unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
@@ -579,7 +580,7 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         // updateUniformGridSIS (RigidBodies/UniformGridUpdate.cpp:734-797): integrate and pose, the grid
         // broadphase, deepest contacts (static set first), the contact list, the sweeps, the pose again
         unsigned *kA = nullptr, *vA = nullptr;
-        if (int rc = enqueue_grid_broadphase(bb, pb, sv, stt, &kA, &vA, rec)) return rc;
+        if (int rc = enqueue_grid_broadphase(bb, pb, sv, stt, false, &kA, &vA, rec)) return rc;
         if (numStatic)
             launch_sis_find_static(sortedPos, vA, N, stPos, nullptr, stTable, nullptr, geoms[0], ph.r, false, sisDistA,
                                    sisPartA, nullptr, st);
@@ -606,7 +607,8 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         // the sort's workspace is cleared by the body launch and its histograms are built while the
         // keys are generated, so the sort itself is only its digit passes
         unsigned *kA = nullptr, *vA = nullptr;
-        if (int rc = enqueue_grid_broadphase(bb, pb, sv, stt, &kA, &vA, rec)) return rc;
+        // the DEM below writes every grain's velocity, so the pose kernel's copy is not stored
+        if (int rc = enqueue_grid_broadphase(bb, pb, sv, stt, true, &kA, &vA, rec)) return rc;
         DemOut o{};
         o.pForce = pForce; o.vel = vel; o.relPos = relPos;
         launch_dem_grid_seg(sortedPos, sortedVel, kA, vA, segStart, segBits, N, ph, sv, S > 1, o, tu.demBlock, st);

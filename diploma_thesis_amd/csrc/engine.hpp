@@ -111,7 +111,7 @@ struct Engine {
     int alloc_sis(int cap);
     template <class Rec>
     int enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, const SceneView &sv, const SortTuning &stt,
-                                unsigned **kAOut, unsigned **vAOut, Rec &rec);
+                                bool grainVelDead, unsigned **kAOut, unsigned **vAOut, Rec &rec);
 };
 
 }  // namespace arsim

@@ -66,9 +66,10 @@ struct KeyOut {
     unsigned *hist = nullptr;     // optional: the sort's digit histograms, accumulated here
     int histPasses = 0, histRbits = 8, histEndBit = 32;
     int histBase = 0;             // bit of the first digit
+    bool grainVelDead = false;    // the step's DEM rewrites every grain's velocity: skip writing it here
 };
 
-// Up to three device ranges zeroed by extra blocks of the body-integration launch.
+// Up to four device ranges zeroed by extra blocks of the body-integration launch.
 struct ZeroJobs {
     unsigned *p[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t count[4] = {0, 0, 0, 0};
