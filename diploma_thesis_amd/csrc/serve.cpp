@@ -122,9 +122,10 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
         reqs[i].bytes = scene_bytes(reqs[i].n, (int)(0.75 * reqs[i].n / 64), method, 18);
     }
 
-    // ---- clients: request inputs are prepared ahead of their arrival by a few producer threads
-    // (a request arrives with its scene), at most kLookahead requests ahead of the server
-    const int kLookahead = 256;
+    // ---- clients: request inputs are prepared ahead of their arrival by producer threads (a request
+    // arrives with its scene), at most kLookahead requests ahead of the server.  Eight threads keep
+    // up with arrivals 10x the specified rate; four left the server waiting 5.6 s for scenes.
+    const int kLookahead = 1024;
     std::vector<std::atomic<int>> ready(cfg->numRequests);
     for (auto &x : ready) x.store(0);
     std::atomic<int> nextGen{0}, served{0}, genError{0};
@@ -143,7 +144,7 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
         }
     };
     std::vector<std::thread> producers;
-    for (int k = 0; k < 4; ++k) producers.emplace_back(producer);
+    for (int k = 0; k < 8; ++k) producers.emplace_back(producer);
     struct Joiner {   // on every exit: stop the producers and free scenes nobody took
         std::vector<std::thread> &t;
         std::atomic<int> &stop;
