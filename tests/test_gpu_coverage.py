@@ -245,3 +245,20 @@ def test_benchmark_scene_two_steps_bit_exact():
     st = oracle.State(sc, np.float32)
     st.step(2)
     assert_state_equal(got, st, "benchmark scene")
+
+
+def test_benchmark_batch_one_step_bit_exact():
+    """The benchmark's batch: 512 scenes log-uniform in [1024, 65536] particles (8.85 M) stepped as one
+    graph-replayed batch.  Every scene's state after a step equals the oracle's step of that scene
+    alone (the classic sort and segment reorder at this size, long empty runs of the cell table)."""
+    print("engine: 512-scene benchmark batch, 1 step", flush=True)
+    mix = scenes.batch_mix(512, seed=0, method=capi.METHOD_GRID)
+    assert sum(s.n for s in mix) > 8_000_000
+    with Engine(mix, use_graph=True) as e:
+        e.step(1)
+        e.sync()
+        got = [e.download(k) for k in range(len(mix))]
+    for k, sc in enumerate(mix):
+        st = oracle.State(sc, np.float32)
+        st.step(1)
+        assert_state_equal(got[k], st, f"batch scene {k}")
