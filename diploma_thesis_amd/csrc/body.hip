@@ -77,8 +77,8 @@ __global__ void k_body_integrate(BodyBufs b, int B, DevPhys ph, ZeroJobs z, int 
 // (particleKernels.cu:50-129) for grains, then the broadphase key of the updated position:
 // calcHashD (particleKernels.cu:170-189) or generateMortonCodes (BVHkernels.cu:670-716).
 template <int MODE>
-__device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko,
-                                             int doBodies, int doGrains, unsigned *shist);
+__device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
+                                                 KeyOut ko, int doBodies, int doGrains, unsigned *shist);
 
 // Adds 1 to an LDS histogram bin for every active lane.  Spatially ordered keys share digits across
 // a wave, so lanes are grouped by digit (8 ballots find the lanes with the same 8-bit digit) and
@@ -117,10 +117,11 @@ __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N,
     }
 }
 
+// Returns the key (0 without one); a grid key is stored only when ko.keys is set.
 template <int MODE>
-__device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko,
-                                             int doBodies, int doGrains, unsigned *shist) {
-    if (i >= N) return;
+__device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
+                                                 KeyOut ko, int doBodies, int doGrains, unsigned *shist) {
+    if (i >= N) return 0;
     const int rbi = p.rb ? p.rb[i] : -1;
     const int s = p.scene ? (int)p.scene[i] : 0;
     SceneGeom g;
@@ -172,7 +173,7 @@ __device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int 
             if (!ko.grainVelDead) p.vel[i] = Vfin;
         }
     }
-    if (MODE == kKeyNone) return;
+    if (MODE == kKeyNone) return 0;
     if (p.rec) {
         if (!haveV) Vfin = p.vel[i];
         p.rec[2 * (size_t)i] = make_float4(P.x, P.y, P.z, __int_as_float(rbi));
@@ -183,7 +184,7 @@ __device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int 
         int gx, gy, gz;
         grid_pos(xyz(P), g, gx, gy, gz);
         key = g.cellBase + grid_hash(gx, gy, gz, g);
-        ((unsigned *)ko.keys)[i] = (unsigned)key;
+        if (ko.keys) ((unsigned *)ko.keys)[i] = (unsigned)key;
     } else {
         float lo0 = g.mLo[0], lo1 = g.mLo[1], lo2 = g.mLo[2];
         float e0 = g.mExt[0], e1 = g.mExt[1], e2 = g.mExt[2];
@@ -211,6 +212,102 @@ __device__ __forceinline__ void pose_key_one(int i, PartBufs p, BodyBufs b, int 
             const int bits = ko.histEndBit - sh0 < ko.histRbits ? ko.histEndBit - sh0 : ko.histRbits;
             const unsigned d = (unsigned)(key >> sh0) & ((1u << bits) - 1u);
             hist_add(shist + pp * 256, d);
+        }
+    }
+    return key;
+}
+
+// Pose and grid key of one tile of kTileKeys consecutive particles, the tile then ranked by the key's
+// top digit in LDS and written in digit order (stable: particle order within a digit), with the
+// tile's per-digit count and first slot in tileInfo[d * numTiles + tile] = count << 16 | first.  This
+// stands in for the first (top-digit) pass of the grid sort: the bucket kernel gathers each digit's
+// keys from every tile's run, in tile order.  Wave w owns particles [tile + 256 w, tile + 256 w + 256),
+// four per lane, so (wave, item, lane) order is particle order.
+constexpr int kTileThreads = 1024;
+constexpr int kTileWaves = kTileThreads / 64;
+constexpr int kTileItems = kTileKeys / kTileThreads;
+
+__device__ __forceinline__ uint64_t match8(unsigned d, bool ok) {
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const unsigned t = 0u - ((d >> b) & 1u);
+        const uint64_t m = __ballot(t != 0u);
+        const unsigned lo = ~((unsigned)m ^ t), hi = ~((unsigned)(m >> 32) ^ t);
+        peers &= ((uint64_t)hi << 32) | lo;
+    }
+    return peers;
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_pose_key_tiles(PartBufs p, BodyBufs b, int N, DevPhys ph,
+                                                                 SceneView sv, TileKeyOut to, int grainVelDead) {
+    __shared__ unsigned sWaveHist[kTileWaves][256];
+    __shared__ unsigned sDigitStart[256];
+    __shared__ unsigned sScan[kTileWaves];
+    const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int k = t; k < kTileWaves * 256; k += kTileThreads) (&sWaveHist[0][0])[k] = 0u;
+    const long long tileBase = (long long)blockIdx.x * kTileKeys;
+    const long long waveBase = tileBase + (long long)wave * (kTileItems * 64);
+    KeyOut ko{};
+    ko.grainVelDead = grainVelDead != 0;
+    unsigned keys[kTileItems];
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k) {
+        const long long i = waveBase + k * 64 + lane;
+        keys[k] = i < N ? (unsigned)pose_key_one<kKeyGrid>((int)i, p, b, N, ph, sv, ko, 1, 1, nullptr) : 0u;
+    }
+    __syncthreads();
+    // ranks within the wave, in particle order
+    const uint64_t lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    unsigned rank[kTileItems];
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k) {
+        const bool valid = waveBase + k * 64 + lane < N;
+        const unsigned d = valid ? keys[k] >> to.topShift : 0u;
+        const uint64_t peers = match8(d, valid);
+        const unsigned prior = (unsigned)__popcll(peers & lower);
+        const unsigned old = sWaveHist[wave][d];
+        rank[k] = old + prior;
+        if (valid && prior == 0u) sWaveHist[wave][d] = old + (unsigned)__popcll(peers);
+    }
+    __syncthreads();
+    // per digit: exclusive over waves and the tile's count, then the digits' first slots
+    unsigned cnt = 0u;
+    if (t < 256) {
+        unsigned run = 0u;
+#pragma unroll
+        for (int w = 0; w < kTileWaves; ++w) {
+            const unsigned c = sWaveHist[w][t];
+            sWaveHist[w][t] = run;
+            run += c;
+        }
+        cnt = run;
+    }
+    unsigned x = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sScan[wave] = x;
+    __syncthreads();
+    unsigned before = 0u;
+    for (int w = 0; w < wave; ++w) before += sScan[w];
+    const unsigned first = before + x - cnt;
+    if (t < 256) {
+        sDigitStart[t] = first;
+        to.info[(size_t)t * to.numTiles + blockIdx.x] = (cnt << 16) | first;
+        if (cnt) atomicAdd(&to.hist[t], cnt);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k) {
+        const long long i = waveBase + k * 64 + lane;
+        if (i < N) {
+            const unsigned d = keys[k] >> to.topShift;
+            const unsigned pos = sDigitStart[d] + sWaveHist[wave][d] + rank[k];
+            to.keys[tileBase + pos] = keys[k];
+            to.vals[tileBase + pos] = (unsigned)i;
         }
     }
 }
@@ -454,6 +551,15 @@ void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, con
             hipLaunchKernelGGL(k_pose_key<kKeyNone>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
             break;
     }
+}
+
+int pose_tile_count(long long numParticles) { return (int)((numParticles + kTileKeys - 1) / kTileKeys); }
+
+void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,
+                           const SceneView &sv, const TileKeyOut &to, bool grainVelDead, hipStream_t st) {
+    if (numParticles <= 0) return;
+    hipLaunchKernelGGL(k_pose_key_tiles, dim3(pose_tile_count(numParticles)), dim3(kTileThreads), 0, st, p, b,
+                       numParticles, ph, sv, to, grainVelDead ? 1 : 0);
 }
 
 void launch_reduce_apply(const float4 *pForce, const float4 *pTorque, const float4 *relPos, const BodyBufs &b,

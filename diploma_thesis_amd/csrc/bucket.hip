@@ -1,14 +1,15 @@
 // bucket.hip -- the grid key's sort finished per top-digit bucket, fused with the reorder and the
 // cell table.
 //
-// After one stable onesweep pass on the key's top 8 bits, the keys of one top digit d are a
-// contiguous bucket, and the cells of that bucket are the table range [d << lowBits, (d + 1) << lowBits).
-// One workgroup per bucket then
-//   1. sorts the bucket by the low bits, stably: a bucket that fits in LDS is sorted there as packed
+// The pose kernel leaves every tile of 4096 particles ranked by the key's top 8 bits (tile-local,
+// stable).  The keys of one top digit d, the bucket, are then the runs of d in every tile taken in
+// tile order, and the cells of that bucket are the table range [d << lowBits, (d + 1) << lowBits).
+// Two workgroups per bucket then
+//   1. sort the bucket by the low bits, stably: a bucket that fits in LDS is sorted there as packed
 //      words (low key << 16 | position in the bucket), so one 32-bit word carries key and order;
 //      a larger bucket is sorted in global memory by the same stable passes (slow, but correct);
-//   2. writes the sorted keys / indices and the gathered sorted copies (the reorder);
-//   3. writes its part of the cell table: table[c] = first sorted index with key >= c.
+//   2. write the sorted keys / indices and the gathered sorted copies (the reorder), a half each;
+//   3. write their part of the cell table: table[c] = first sorted index with key >= c.
 // The result is the same as the stable LSD sort followed by the reorder, for any distribution.
 #include "csr_fill.hpp"
 #include "kernels.hpp"
@@ -19,11 +20,12 @@ namespace {
 
 constexpr int kBkThreads = 1024;
 constexpr int kBkWaves = kBkThreads / 64;
-// Bucket size sorted in LDS: two buffers of packed words plus the 16 KiB cursor table and the wave
-// sums stay within 80 KiB, half of a gfx950 CU's 160 KiB, so two workgroups share a CU.
-constexpr int kBkCap = 8160;
+// Bucket size sorted in LDS: two buffers of packed words plus the 16 KiB cursor table, the wave sums
+// and the tile map stay within 80 KiB, half of a gfx950 CU's 160 KiB, so two workgroups share a CU.
+constexpr int kBkCap = 7776;
 constexpr int kBkSplit = 2;      // workgroups per bucket: each sorts it, then writes its share of the outputs
-constexpr int kBkLdsBytes = 2 * kBkCap * 4 + kBkWaves * 256 * 4 + kBkWaves * 4;
+constexpr int kBkMaxTiles = 384;  // pose tiles a bucket gathers from (2^20 keys: 256)
+constexpr int kBkLdsBytes = 2 * kBkCap * 4 + kBkWaves * 256 * 4 + kBkWaves * 4 + 2 * kBkMaxTiles * 4;
 static_assert(kBkLdsBytes <= 80 * 1024, "two bucket workgroups must fit in one CU's LDS");
 
 // Lanes of the wave whose digit (bits <= 8 wide) equals this lane's, among lanes with ok set.  Per
@@ -151,15 +153,15 @@ __device__ __forceinline__ void block_fill(unsigned *table, unsigned long long a
 }
 
 // eight waves per SIMD (two workgroups per CU): registers are held to that occupancy
-__global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bucket_sort(unsigned *__restrict__ kIn, unsigned *__restrict__ vIn,
-                                                            unsigned *__restrict__ kOut, unsigned *__restrict__ vOut,
-                                                            const unsigned *__restrict__ hist, int lowBits,
-                                                            const float4 *__restrict__ rec,
-                                                            float4 *__restrict__ sPos, float4 *__restrict__ sVel,
-                                                            unsigned *__restrict__ table, unsigned numCells) {
+__global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bucket_sort(
+    const unsigned *__restrict__ tKeys, const unsigned *__restrict__ tVals, const unsigned *__restrict__ tInfo,
+    int numTiles, unsigned *__restrict__ kOut, unsigned *__restrict__ vOut, const unsigned *__restrict__ hist,
+    int lowBits, const float4 *__restrict__ rec, float4 *__restrict__ sPos, float4 *__restrict__ sVel,
+    unsigned *__restrict__ table, unsigned numCells) {
     __shared__ unsigned sA[kBkCap], sB[kBkCap];
     __shared__ unsigned sCur[kBkWaves][256];
     __shared__ unsigned sWave[kBkWaves];
+    __shared__ unsigned sOff[kBkMaxTiles], sSrc[kBkMaxTiles];   // tile map: bucket offset and source slot
     const int t = threadIdx.x;
     const int d = blockIdx.x / kBkSplit, part = blockIdx.x % kBkSplit;
     const unsigned lowMask = (1u << lowBits) - 1u;
@@ -179,6 +181,32 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
         block_fill(table, c0 + n * part / kBkSplit, c0 + n * (part + 1) / kBkSplit, start);
         return;
     }
+    // The bucket's keys are the runs of digit d in every tile, in tile order: tile k's run starts at
+    // bucket position sOff[k] and at slot sSrc[k] of the tile-ordered arrays.
+    {
+        unsigned c = 0u, f = 0u;
+        if (t < numTiles) {
+            const unsigned w = tInfo[(size_t)d * numTiles + t];
+            c = w >> 16;
+            f = w & 0xffffu;
+        }
+        const unsigned off = block_excl_scan(c, sWave);
+        if (t < numTiles) {
+            sOff[t] = off;
+            sSrc[t] = (unsigned)t * kTileKeys + f;
+        }
+        __syncthreads();
+    }
+    // slot of bucket position i in the tile-ordered arrays: the last tile whose run starts at or before i
+    auto slot = [&](unsigned i) -> unsigned {
+        int lo = 0, hi = numTiles;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (sOff[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        return sSrc[lo] + (i - sOff[lo]);
+    };
     // low(i) = low bits of the i-th sorted key of the bucket
     auto fill_table = [&](auto low) {
         // head: cells up to the first key; tail: cells past the last key
@@ -198,7 +226,7 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
         }
     };
     if (inLds) {
-        for (int i = t; i < cnt; i += kBkThreads) sA[i] = ((kIn[start + i] & lowMask) << 16) | (unsigned)i;
+        for (int i = t; i < cnt; i += kBkThreads) sA[i] = ((tKeys[slot((unsigned)i)] & lowMask) << 16) | (unsigned)i;
         __syncthreads();
         unsigned *src = sA, *dst = sB;
         for (int sh = 0; sh < lowBits; sh += 8) {
@@ -215,7 +243,7 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
         const int ia = i0 + t, ib = ia + kBkThreads;
         const bool ha = ia < i1, hb = ib < i1;
         const unsigned wa = sorted[ha ? ia : 0], wb = sorted[hb ? ib : 0];
-        const unsigned va = vIn[start + (wa & 0xffffu)], vb = vIn[start + (wb & 0xffffu)];
+        const unsigned va = tVals[slot(wa & 0xffffu)], vb = tVals[slot(wb & 0xffffu)];
         const float4 pa = rec[2 * (size_t)va], qa = rec[2 * (size_t)va + 1];
         const float4 pb = rec[2 * (size_t)vb], qb = rec[2 * (size_t)vb + 1];
         fill_table([&](int i) -> unsigned { return sorted[i] >> 16; });
@@ -233,24 +261,38 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
         }
         for (int i = ib + kBkThreads; i < i1; i += kBkThreads) {
             const unsigned w = sorted[i];
-            const unsigned val = vIn[start + (w & 0xffffu)];
+            const unsigned val = tVals[slot(w & 0xffffu)];
             kOut[start + i] = topBits | (w >> 16);
             vOut[start + i] = val;
             sPos[start + i] = rec[2 * (size_t)val];
             sVel[start + i] = rec[2 * (size_t)val + 1];
         }
     } else {
-        GlobalItems a{kIn + start, vIn + start}, b{kOut + start, vOut + start};
+        // a bucket too large for LDS, sorted in global memory by its first workgroup alone: its runs are
+        // gathered into (kOut, vOut) at [start, start + cnt), and the sorted-copy arrays' range (16 B
+        // per key) serves as the second buffer of the passes until the copies are written
+        for (int i = t; i < cnt; i += kBkThreads) {
+            const unsigned src = slot((unsigned)i);
+            kOut[start + i] = tKeys[src];
+            vOut[start + i] = tVals[src];
+        }
+        __syncthreads();
+        GlobalItems a{kOut + start, vOut + start}, b{reinterpret_cast<unsigned *>(sPos + start),
+                                                     reinterpret_cast<unsigned *>(sVel + start)};
         int passes = 0;
         for (int sh = 0; sh < lowBits; sh += 8, ++passes) {
             if ((passes & 1) == 0) stable_pass(a, b, cnt, sh, min(8, lowBits - sh), sCur, sWave);
             else stable_pass(b, a, cnt, sh, min(8, lowBits - sh), sCur, sWave);
         }
-        const GlobalItems &res = (passes & 1) ? b : a;
+        if (passes & 1) {
+            for (int i = t; i < cnt; i += kBkThreads) {
+                a.k[i] = b.k[i];
+                a.v[i] = b.v[i];
+            }
+            __syncthreads();
+        }
         for (int i = t; i < cnt; i += kBkThreads) {
-            const unsigned key = res.k[i], val = res.v[i];
-            kOut[start + i] = key;
-            vOut[start + i] = val;
+            const unsigned val = vOut[start + i];
             sPos[start + i] = rec[2 * (size_t)val];
             sVel[start + i] = rec[2 * (size_t)val + 1];
         }
@@ -263,19 +305,19 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
 
 bool bucket_sort_fits(long long n, int keyBits, unsigned long long numCells) {
     if (keyBits < 9 || keyBits > 24) return false;
+    if ((n + kTileKeys - 1) / kTileKeys > kBkMaxTiles) return false;
     const int lowBits = keyBits - 8;
     const unsigned long long buckets = (numCells >> lowBits) + 1ull;
     // a bucket averages n / 256 keys; the LDS path takes up to kBkCap, larger ones go to global memory
     return (double)n / (double)(buckets < 256ull ? buckets : 256ull) <= 0.75 * kBkCap;
 }
 
-void launch_bucket_sort(unsigned *kIn, unsigned *vIn, unsigned *kOut, unsigned *vOut, const unsigned *hist,
-                        int keyBits, const float4 *rec, float4 *sortedPos, float4 *sortedVel, unsigned *table,
-                        unsigned numCells, hipStream_t st) {
+void launch_bucket_sort(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, int keyBits, const float4 *rec,
+                        float4 *sortedPos, float4 *sortedVel, unsigned *table, unsigned numCells, hipStream_t st) {
     const int lowBits = keyBits - 8;
     const unsigned buckets = (unsigned)((numCells >> lowBits) + 1u);
-    hipLaunchKernelGGL(k_bucket_sort, dim3(buckets * kBkSplit), dim3(kBkThreads), 0, st, kIn, vIn, kOut, vOut, hist,
-                       lowBits, rec, sortedPos, sortedVel, table, numCells);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(buckets * kBkSplit), dim3(kBkThreads), 0, st, to.keys, to.vals, to.info,
+                       to.numTiles, kOut, vOut, to.hist, lowBits, rec, sortedPos, sortedVel, table, numCells);
 }
 
 }  // namespace arsim

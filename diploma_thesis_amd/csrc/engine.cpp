@@ -212,19 +212,26 @@ PartBufs Engine::part_bufs() const {
 }
 
 const char *const *Engine::stage_names(int method, unsigned flags, bool bucket, int *count) {
-    // with the bucket sort the "sort" stage is the top-digit pass and the reorder also finishes the sort
+    // with the bucket sort the pose stage also ranks each tile by the top digit (the sort's first pass)
+    // and the bucket stage finishes the sort with the reorder
     static const char *const sis[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "sis_find",
                                       "sis_gather", "sis_solve", "pose"};
-    static const char *const sisB[] = {"body_integrate", "pose_key", "sort_top_digit", "bucket_sort_reorder",
-                                       "sis_find", "sis_gather", "sis_solve", "pose"};
-    if (flags & ARSIM_CONTACT_SIS) { *count = 8; return bucket ? sisB : sis; }
+    static const char *const sisB[] = {"body_integrate", "pose_key", "bucket_sort_reorder", "sis_find",
+                                       "sis_gather", "sis_solve", "pose"};
+    if (flags & ARSIM_CONTACT_SIS) {
+        *count = bucket ? 7 : 8;
+        return bucket ? sisB : sis;
+    }
     static const char *const grid[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "dem_grid",
                                        "reduce_apply"};
-    static const char *const gridB[] = {"body_integrate", "pose_key", "sort_top_digit", "bucket_sort_reorder",
-                                        "dem_grid", "reduce_apply"};
+    static const char *const gridB[] = {"body_integrate", "pose_key", "bucket_sort_reorder", "dem_grid",
+                                        "reduce_apply"};
     static const char *const lbvh[] = {"body_integrate", "pose_key", "sort", "lbvh_build", "lbvh_refit", "dem_lbvh",
                                        "reduce_apply"};
-    if (method == ARSIM_METHOD_GRID) { *count = 6; return bucket ? gridB : grid; }
+    if (method == ARSIM_METHOD_GRID) {
+        *count = bucket ? 5 : 6;
+        return bucket ? gridB : grid;
+    }
     *count = 7;
     return lbvh;
 }
@@ -373,6 +380,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     A(overflowD, 4); A(flagD, 4);
     if (method == ARSIM_METHOD_GRID) {
         A(segStart, (size_t)numSegs + 1);
+        A(tileInfo, (size_t)256 * pose_tile_count(N));
         A(sortedPos, N); A(sortedVel, N);
         longCap = fill_long_capacity((unsigned long long)numSegs + 1ull);
         A(longCnt, 2); A(longList, (size_t)longCap);
@@ -529,10 +537,28 @@ int Engine::reset_grid_tables() {
 template <class Rec>
 int Engine::enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, const SceneView &sv, const SortTuning &tuIn,
                                     bool grainVelDead, unsigned **kAOut, unsigned **vAOut, Rec &rec) {
+    if (bucketSort) {
+        // the pose kernel ranks each tile of particles by the key's top digit (the sort's first pass);
+        // the bucket kernel gathers each digit's runs, finishes the sort there, and writes the sorted
+        // copies and the cell table
+        unsigned *hist = (unsigned *)sortWs;   // the top digit's histogram (256 words)
+        ZeroJobs z{};
+        z.p[0] = hist;
+        z.count[0] = 256;
+        launch_body_integrate(bb, B, ph, &z, st);
+        rec();
+        const TileKeyOut to{(unsigned *)keysB, valsB, tileInfo, hist, pose_tile_count(N), keyBits - 8};
+        launch_pose_key_tiles(pb, bb, N, ph, sv, to, grainVelDead, st);
+        rec();
+        launch_bucket_sort(to, (unsigned *)keysA, valsA, keyBits, gatherRec, sortedPos, sortedVel, segStart,
+                           (unsigned)numCells, st);
+        rec();
+        *kAOut = (unsigned *)keysA;
+        *vAOut = valsA;
+        return ARSIM_OK;
+    }
     SortTuning stt = tuIn;
-    // the bucket path sorts the top 8 bits in one onesweep pass and the rest per bucket
-    const int beginBit = bucketSort ? keyBits - 8 : 0;
-    if (bucketSort) stt.radixBits = 8;
+    const int beginBit = 0;
     const SortPlan pl = sort_plan(sortWs, N, keyBytes, beginBit, keyBits, stt);
     ZeroJobs z = sort_zero_jobs(pl);
     z.p[3] = longCnt;   // the cell table's long-run count
@@ -548,15 +574,8 @@ int Engine::enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, cons
     if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, beginBit, keyBits, true, sortWs, sortWsBytes, stt, st, true))
         return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
     rec();
-    if (bucketSort) {
-        launch_bucket_sort(kA, vA, kB, vB, pl.hist, keyBits, gatherRec, sortedPos, sortedVel, segStart,
-                           (unsigned)numCells, st);
-        std::swap(kA, kB);
-        std::swap(vA, vB);
-    } else {
-        launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart,
-                           segBits, numSegs, N, long_fill(), st);
-    }
+    launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart, segBits,
+                       numSegs, N, long_fill(), st);
     rec();
     *kAOut = kA;
     *vAOut = vA;

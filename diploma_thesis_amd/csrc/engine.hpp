@@ -53,6 +53,7 @@ struct Engine {
     unsigned char *keyBufA = nullptr, *keyBufB = nullptr, *sortWsBuf = nullptr;
     unsigned *valsA = nullptr, *valsB = nullptr;
     unsigned *segStart = nullptr;           // grid: CSR offsets over segments of 2^segBits cells
+    unsigned *tileInfo = nullptr;           // grid, bucket sort: per-tile digit runs (count << 16 | first)
     unsigned *longCnt = nullptr;            // grid: run count of the long-run table fill (zeroed each step)
     uint4 *longList = nullptr;
     unsigned longCap = 0;

@@ -135,15 +135,26 @@ void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, c
                         const int *rb, const uint16_t *scene, const float4 *rec, float4 *sortedPos,
                         float4 *sortedVel, unsigned *segStart, int segBits, long long numSegs, int n,
                         const LongFill &lf, hipStream_t st);
-// The grid sort finished per top-digit bucket (bucket.hip): after one stable pass on bits
-// [keyBits - 8, keyBits) of (kIn, vIn), one workgroup per bucket sorts it by the low bits, writes the
-// sorted keys / indices to (kOut, vOut), the gathered sorted copies and its range of the exact cell
-// table.  hist = the top digit's histogram.  bucket_sort_fits: keyBits in [9, 24] and buckets that
-// mostly fit in LDS for n keys.
+// Grid keys ranked per tile of kTileKeys particles inside the pose kernel (the first pass of the bucket
+// sort, body.hip): the tile's keys and particle indices in top-digit order (stable), tileInfo[d *
+// numTiles + tile] = count << 16 | first slot of digit d in the tile, and the top digit's histogram
+// accumulated into hist (zeroed beforehand).  topShift = keyBits - 8.
+constexpr int kTileKeys = 4096;
+struct TileKeyOut {
+    unsigned *keys, *vals, *info, *hist;
+    int numTiles, topShift;
+};
+int pose_tile_count(long long numParticles);
+void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,
+                           const SceneView &sv, const TileKeyOut &to, bool grainVelDead, hipStream_t st);
+// The grid sort finished per top-digit bucket (bucket.hip): each bucket gathers its keys from every
+// tile's run of its digit (tile order keeps the sort stable), two workgroups sort it by the low bits,
+// write the sorted keys / indices to (kOut, vOut), the gathered sorted copies and their range of the
+// exact cell table.  bucket_sort_fits: keyBits in [9, 24], at most 384 tiles, and buckets that mostly
+// fit in LDS for n keys.
 bool bucket_sort_fits(long long n, int keyBits, unsigned long long numCells);
-void launch_bucket_sort(unsigned *kIn, unsigned *vIn, unsigned *kOut, unsigned *vOut, const unsigned *hist,
-                        int keyBits, const float4 *rec, float4 *sortedPos, float4 *sortedVel, unsigned *table,
-                        unsigned numCells, hipStream_t st);
+void launch_bucket_sort(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, int keyBits, const float4 *rec,
+                        float4 *sortedPos, float4 *sortedVel, unsigned *table, unsigned numCells, hipStream_t st);
 // Capacity of the long-run list of a table of tableSize entries (runs are disjoint and longer than
 // kLongRange), and the launch that fills the listed runs.
 unsigned fill_long_capacity(unsigned long long tableSize);
