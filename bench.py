@@ -34,8 +34,8 @@ ALGO_BYTES = {
     "pose_key": (68.0 + 24.0, 0.0),     # body integrate + pose 68, hash 24
     "sort": (4.0 + 16.0 * 3, 0.0),      # onesweep, u32 key + u32 value, 3 digit passes over the 24-bit grid key
     "reduce_apply": (32.0, 0.0),
-    # bucket sort: one onesweep pass on the top digit, then the two remaining passes fused with the reorder
-    "sort_top_digit": (4.0 + 16.0, 0.0),
+    # bucket sort: the top digit ranked per tile inside pose_key, the two remaining passes fused with the
+    # reorder and the cell table
     "bucket_sort_reorder": (72.0 + 2 * 16.0, 4.0),
 }
 HBM_PEAK_GBS = 8000.0
