@@ -557,21 +557,20 @@ int Engine::enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, cons
         *vAOut = valsA;
         return ARSIM_OK;
     }
-    SortTuning stt = tuIn;
-    const int beginBit = 0;
-    const SortPlan pl = sort_plan(sortWs, N, keyBytes, beginBit, keyBits, stt);
+    const SortTuning stt = tuIn;
+    const SortPlan pl = sort_plan(sortWs, N, keyBytes, 0, keyBits, stt);
     ZeroJobs z = sort_zero_jobs(pl);
     z.p[3] = longCnt;   // the cell table's long-run count
     z.count[3] = 1;
     launch_body_integrate(bb, B, ph, &z, st);
     rec();
     KeyOut ko{keysA, nullptr, nullptr};
-    attach_hist(ko, pl, stt, keyBits, beginBit);
+    attach_hist(ko, pl, stt, keyBits);
     ko.grainVelDead = grainVelDead;
     launch_pose_key(pb, bb, N, ph, sv, kKeyGrid, ko, true, true, st);
     rec();
     unsigned *kA = (unsigned *)keysA, *kB = (unsigned *)keysB, *vA = valsA, *vB = valsB;
-    if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, beginBit, keyBits, true, sortWs, sortWsBytes, stt, st, true))
+    if (sort_pairs<unsigned>(&kA, &vA, &kB, &vB, N, 0, keyBits, true, sortWs, sortWsBytes, stt, st, true))
         return fail(ARSIM_ERR_INVALID_ARG, "engine: sort rejected");
     rec();
     launch_reorder_seg(kA, vA, pos, vel, rb, S > 1 ? scene : nullptr, gatherRec, sortedPos, sortedVel, segStart, segBits,
