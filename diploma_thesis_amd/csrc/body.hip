@@ -217,15 +217,13 @@ __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, 
     return key;
 }
 
-// Pose and grid key of one tile of kTileKeys consecutive particles, the tile then ranked by the key's
+// Pose and grid key of one tile of THREADS * 4 consecutive particles, the tile then ranked by the key's
 // top digit in LDS and written in digit order (stable: particle order within a digit), with the
 // tile's per-digit count and first slot in tileInfo[d * numTiles + tile] = count << 16 | first.  This
 // stands in for the first (top-digit) pass of the grid sort: the bucket kernel gathers each digit's
 // keys from every tile's run, in tile order.  Wave w owns particles [tile + 256 w, tile + 256 w + 256),
 // four per lane, so (wave, item, lane) order is particle order.
-constexpr int kTileThreads = 1024;
-constexpr int kTileWaves = kTileThreads / 64;
-constexpr int kTileItems = kTileKeys / kTileThreads;
+constexpr int kTileItems = 4;   // particles per thread
 
 __device__ __forceinline__ uint64_t match8(unsigned d, bool ok) {
     uint64_t peers = __ballot(ok);
@@ -239,14 +237,17 @@ __device__ __forceinline__ uint64_t match8(unsigned d, bool ok) {
     return peers;
 }
 
-__global__ __launch_bounds__(kTileThreads) void k_pose_key_tiles(PartBufs p, BodyBufs b, int N, DevPhys ph,
-                                                                 SceneView sv, TileKeyOut to, int grainVelDead) {
-    __shared__ unsigned sWaveHist[kTileWaves][256];
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
+                                                            TileKeyOut to, int grainVelDead) {
+    constexpr int WAVES = THREADS / 64;
+    constexpr int TILE = THREADS * kTileItems;
+    __shared__ unsigned sWaveHist[WAVES][256];
     __shared__ unsigned sDigitStart[256];
-    __shared__ unsigned sScan[kTileWaves];
+    __shared__ unsigned sScan[WAVES];
     const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
-    for (int k = t; k < kTileWaves * 256; k += kTileThreads) (&sWaveHist[0][0])[k] = 0u;
-    const long long tileBase = (long long)blockIdx.x * kTileKeys;
+    for (int k = t; k < WAVES * 256; k += THREADS) (&sWaveHist[0][0])[k] = 0u;
+    const long long tileBase = (long long)blockIdx.x * TILE;
     const long long waveBase = tileBase + (long long)wave * (kTileItems * 64);
     KeyOut ko{};
     ko.grainVelDead = grainVelDead != 0;
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pose_key_tiles(PartBufs p, Bod
     if (t < 256) {
         unsigned run = 0u;
 #pragma unroll
-        for (int w = 0; w < kTileWaves; ++w) {
+        for (int w = 0; w < WAVES; ++w) {
             const unsigned c = sWaveHist[w][t];
             sWaveHist[w][t] = run;
             run += c;
@@ -553,13 +554,23 @@ void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, con
     }
 }
 
-int pose_tile_count(long long numParticles) { return (int)((numParticles + kTileKeys - 1) / kTileKeys); }
+int pose_tile_keys(long long numParticles) { return numParticles <= 65536 ? 1024 : kTileKeys; }
+
+int pose_tile_count(long long numParticles) {
+    const int tk = pose_tile_keys(numParticles);
+    return (int)((numParticles + tk - 1) / tk);
+}
 
 void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,
                            const SceneView &sv, const TileKeyOut &to, bool grainVelDead, hipStream_t st) {
     if (numParticles <= 0) return;
-    hipLaunchKernelGGL(k_pose_key_tiles, dim3(pose_tile_count(numParticles)), dim3(kTileThreads), 0, st, p, b,
-                       numParticles, ph, sv, to, grainVelDead ? 1 : 0);
+    const int g = grainVelDead ? 1 : 0;
+    if (to.tileKeys == 1024)
+        hipLaunchKernelGGL(k_pose_key_tiles<256>, dim3(to.numTiles), dim3(256), 0, st, p, b, numParticles, ph, sv, to,
+                           g);
+    else
+        hipLaunchKernelGGL(k_pose_key_tiles<1024>, dim3(to.numTiles), dim3(1024), 0, st, p, b, numParticles, ph, sv,
+                           to, g);
 }
 
 void launch_reduce_apply(const float4 *pForce, const float4 *pTorque, const float4 *relPos, const BodyBufs &b,

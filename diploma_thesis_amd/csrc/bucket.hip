@@ -155,7 +155,7 @@ __device__ __forceinline__ void block_fill(unsigned *table, unsigned long long a
 // eight waves per SIMD (two workgroups per CU): registers are held to that occupancy
 __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bucket_sort(
     const unsigned *__restrict__ tKeys, const unsigned *__restrict__ tVals, const unsigned *__restrict__ tInfo,
-    int numTiles, unsigned *__restrict__ kOut, unsigned *__restrict__ vOut, const unsigned *__restrict__ hist,
+    int numTiles, int tileKeys, unsigned *__restrict__ kOut, unsigned *__restrict__ vOut, const unsigned *__restrict__ hist,
     int lowBits, const float4 *__restrict__ rec, float4 *__restrict__ sPos, float4 *__restrict__ sVel,
     unsigned *__restrict__ table, unsigned numCells) {
     __shared__ unsigned sA[kBkCap], sB[kBkCap];
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
         const unsigned off = block_excl_scan(c, sWave);
         if (t < numTiles) {
             sOff[t] = off;
-            sSrc[t] = (unsigned)t * kTileKeys + f;
+            sSrc[t] = (unsigned)t * (unsigned)tileKeys + f;
         }
         __syncthreads();
     }
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
 
 bool bucket_sort_fits(long long n, int keyBits, unsigned long long numCells) {
     if (keyBits < 9 || keyBits > 24) return false;
-    if ((n + kTileKeys - 1) / kTileKeys > kBkMaxTiles) return false;
+    if (pose_tile_count(n) > kBkMaxTiles) return false;
     const int lowBits = keyBits - 8;
     const unsigned long long buckets = (numCells >> lowBits) + 1ull;
     // a bucket averages n / 256 keys; the LDS path takes up to kBkCap, larger ones go to global memory
@@ -317,7 +317,7 @@ void launch_bucket_sort(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, in
     const int lowBits = keyBits - 8;
     const unsigned buckets = (unsigned)((numCells >> lowBits) + 1u);
     hipLaunchKernelGGL(k_bucket_sort, dim3(buckets * kBkSplit), dim3(kBkThreads), 0, st, to.keys, to.vals, to.info,
-                       to.numTiles, kOut, vOut, to.hist, lowBits, rec, sortedPos, sortedVel, table, numCells);
+                       to.numTiles, to.tileKeys, kOut, vOut, to.hist, lowBits, rec, sortedPos, sortedVel, table, numCells);
 }
 
 }  // namespace arsim

@@ -547,7 +547,8 @@ int Engine::enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, cons
         z.count[0] = 256;
         launch_body_integrate(bb, B, ph, &z, st);
         rec();
-        const TileKeyOut to{(unsigned *)keysB, valsB, tileInfo, hist, pose_tile_count(N), keyBits - 8};
+        const TileKeyOut to{(unsigned *)keysB, valsB, tileInfo, hist, pose_tile_count(N), keyBits - 8,
+                            pose_tile_keys(N)};
         launch_pose_key_tiles(pb, bb, N, ph, sv, to, grainVelDead, st);
         rec();
         launch_bucket_sort(to, (unsigned *)keysA, valsA, keyBits, gatherRec, sortedPos, sortedVel, segStart,

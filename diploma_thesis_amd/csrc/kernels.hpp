@@ -135,15 +135,17 @@ void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, c
                         const int *rb, const uint16_t *scene, const float4 *rec, float4 *sortedPos,
                         float4 *sortedVel, unsigned *segStart, int segBits, long long numSegs, int n,
                         const LongFill &lf, hipStream_t st);
-// Grid keys ranked per tile of kTileKeys particles inside the pose kernel (the first pass of the bucket
-// sort, body.hip): the tile's keys and particle indices in top-digit order (stable), tileInfo[d *
-// numTiles + tile] = count << 16 | first slot of digit d in the tile, and the top digit's histogram
-// accumulated into hist (zeroed beforehand).  topShift = keyBits - 8.
+// Grid keys ranked per tile of particles inside the pose kernel (the first pass of the bucket sort,
+// body.hip): the tile's keys and particle indices in top-digit order (stable), tileInfo[d * numTiles +
+// tile] = count << 16 | first slot of digit d in the tile, and the top digit's histogram accumulated
+// into hist (zeroed beforehand).  topShift = keyBits - 8.  Tiles hold pose_tile_keys(n) particles:
+// 4096 (kTileKeys), or 1024 for scenes of at most 65,536 particles, which would have too few tiles.
 constexpr int kTileKeys = 4096;
 struct TileKeyOut {
     unsigned *keys, *vals, *info, *hist;
-    int numTiles, topShift;
+    int numTiles, topShift, tileKeys;
 };
+int pose_tile_keys(long long numParticles);
 int pose_tile_count(long long numParticles);
 void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,
                            const SceneView &sv, const TileKeyOut &to, bool grainVelDead, hipStream_t st);
