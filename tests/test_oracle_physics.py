@@ -5,6 +5,12 @@
 - walls: overlap with the floor gives k (r - d) upward plus damping of the normal velocity;
 - integration: free flight under gravity, rotation of a body at constant ω by θ = |ω| dt,
   unit-norm quaternions, inverse inertia of a 4x4x4 cube = I / 160;
+- orientation (Kernels/rigidBodyKernels.cu:319-322): from a non-identity orientation the
+  increment composes on the world side (R_z(θ) R_x(90°), checked against Rodrigues matrices),
+  and the world inverse inertia is R I_b^-1 R^T for anisotropic I_b^-1 (a quarter turn about z
+  swaps two moments; 120° about (1,1,1) permutes them exactly; a generic rotation against
+  float64), and a torque on such a body gives the hand-computed Δω.  Reversing the quaternion
+  product or transposing the inertia product fails these tests;
 - segmented reduction: a single particle force at r gives Δv = F dt / m, Δω = I^-1 (r x F) dt;
 - FP32 mirror step vs FP64 step on the small scene: agreement to FP32 rounding.
 """
@@ -140,6 +146,119 @@ def test_body_rotation_closed_form(dtype):
     r = st.rel[0, :3]
     assert np.allclose(r, [math.cos(theta), math.sin(theta), 0], atol=2e-7)
     assert np.allclose(np.linalg.norm(st.qcum, axis=1), 1, atol=2e-7)
+
+
+def _rot(axis, angle):
+    """Rotation matrix about a unit axis (Rodrigues), float64: independent of quaternions."""
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + math.sin(angle) * K + (1 - math.cos(angle)) * (K @ K)
+
+
+def _quat(axis, angle):
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    s = math.sin(angle / 2)
+    return np.array([a[0] * s, a[1] * s, a[2] * s, math.cos(angle / 2)])
+
+
+def _body_state(dtype, q0, inv_inertia=None, omega=(0.0, 0.0, 0.0)):
+    """Oracle state of the small scene with body 0 at orientation q0 (x, y, z, w), inverse body
+    inertia inv_inertia (3x3) and angular velocity omega; no linear velocity."""
+    sc = scenes.config0()
+    st = oracle.State(sc, dtype)
+    st.qcum[0] = np.asarray(q0, dtype)
+    if inv_inertia is not None:
+        st.Ib[0] = np.asarray(inv_inertia, np.float64).T.reshape(9).astype(dtype)   # column-major
+    st.rb_ang[0, :3] = omega
+    st.rb_vel[0, :3] = 0
+    return sc, st
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_body_rotation_composes_in_world_frame(dtype):
+    # a body turned 90 degrees about x, then spun about the world z axis by theta in one step:
+    # the new orientation is R_z(theta) R_x(90) (the increment acts after the orientation it
+    # updates), so the body-frame x axis ends at (cos theta, sin theta, 0); the reversed
+    # product R_x(90) R_z(theta) would put it at (cos theta, 0, sin theta)
+    theta = 0.3
+    sc, st = _body_state(dtype, _quat((1, 0, 0), math.pi / 2), omega=(0.0, 0.0, theta / float(sc_dt())))
+    st.integrate_bodies()
+    R = _rot((0, 0, 1), theta) @ _rot((1, 0, 0), math.pi / 2)
+    Rm = st.model[0].reshape(4, 4).T[:3, :3]   # column-major mat4 of the cumulative rotation
+    tol = 2e-6 if dtype == np.float32 else 1e-12
+    assert np.allclose(Rm, R, atol=tol)
+    st.body_frame[0, :3] = [1.0, 0.0, 0.0]
+    st.body_frame[1, :3] = [0.0, 1.0, 0.0]
+    st.pose_grains()
+    assert np.allclose(st.rel[0, :3], [math.cos(theta), math.sin(theta), 0.0], atol=tol)
+    assert np.allclose(st.rel[1, :3], R @ [0.0, 1.0, 0.0], atol=tol)
+    assert np.allclose(st.rel[1, :3], [0.0, 0.0, 1.0], atol=tol)
+    # a second step about z again: R_z(2 theta) R_x(90)
+    st.integrate_bodies()
+    R2 = _rot((0, 0, 1), 2 * theta) @ _rot((1, 0, 0), math.pi / 2)
+    assert np.allclose(st.model[0].reshape(4, 4).T[:3, :3], R2, atol=2 * tol)
+
+
+def sc_dt():
+    return scenes.config0().cfg.dt
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_world_inverse_inertia_quarter_turn_about_z(dtype):
+    # I_b^-1 = diag(a, b, c) turned 90 degrees about z exchanges the x and y moments: diag(b, a, c)
+    a, b, c = 0.5, 1.0 / 3.0, 0.2
+    _, st = _body_state(dtype, _quat((0, 0, 1), math.pi / 2), np.diag([a, b, c]))
+    st.integrate_bodies()
+    Iw = st.Iw[0].reshape(3, 3).T
+    assert np.allclose(Iw, np.diag([b, a, c]), atol=2e-7 if dtype == np.float32 else 1e-14)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_world_inverse_inertia_is_r_i_rt(dtype):
+    # 120 degrees about (1,1,1) sends x -> y -> z -> x, and its quaternion (1/2, 1/2, 1/2, 1/2) is
+    # exact: R diag(a, b, c) R^T = diag(c, a, b) exactly, while R^T diag(a, b, c) R = diag(b, c, a)
+    a, b, c = 0.5, 1.0 / 3.0, 0.2
+    _, st = _body_state(dtype, [0.5, 0.5, 0.5, 0.5], np.diag([a, b, c]))
+    st.integrate_bodies()
+    Iw = st.Iw[0].reshape(3, 3).T
+    assert np.array_equal(Iw, np.diag([c, a, b]).astype(dtype))
+    # a generic orientation and a full symmetric inverse inertia against R I R^T in float64
+    rng = np.random.default_rng(5)
+    M = rng.normal(size=(3, 3))
+    Ib = M @ M.T + 3 * np.eye(3)
+    axis, ang = rng.normal(size=3), 1.1
+    _, st = _body_state(dtype, _quat(axis, ang), Ib)
+    st.integrate_bodies()
+    R = _rot(axis, ang)
+    Ib_used = st.Ib[0].reshape(3, 3).T.astype(np.float64)
+    assert np.allclose(st.Iw[0].reshape(3, 3).T, R @ Ib_used @ R.T, rtol=1e-5 if dtype == np.float32 else 1e-12,
+                       atol=1e-6 if dtype == np.float32 else 1e-13)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_torque_on_anisotropic_rotated_body(dtype):
+    # the body of test_world_inverse_inertia_is_r_i_rt (I_w^-1 = diag(c, a, b)); a particle at body
+    # offset (1, 2, 1/2) sits at r = (1/2, 1, 2) in the world; a force F there turns the body by
+    # dω = diag(c, a, b) (r x F) dt, computed here by hand
+    a, b, c = 0.5, 1.0 / 3.0, 0.2
+    sc, st = _body_state(dtype, [0.5, 0.5, 0.5, 0.5], np.diag([a, b, c]))
+    st.body_frame[3, :3] = [1.0, 2.0, 0.5]
+    st.integrate_bodies()
+    st.pose_grains()
+    assert np.array_equal(st.rel[3, :3], np.array([0.5, 1.0, 2.0], dtype))
+    st.force[:] = 0
+    st.torque[:] = 0
+    F = np.array([3.0, -1.0, 2.0])
+    r = np.array([0.5, 1.0, 2.0])
+    st.force[3, :3] = F
+    st.torque[3, :3] = np.cross(r, F)
+    st.reduce_apply()
+    dt = float(sc.cfg.dt)
+    tau = np.array([1.0 * 2.0 - 2.0 * -1.0, 2.0 * 3.0 - 0.5 * 2.0, 0.5 * -1.0 - 1.0 * 3.0])   # r x F = (4, 5, -3.5)
+    dw = np.array([c * tau[0], a * tau[1], b * tau[2]]) * dt
+    assert np.allclose(st.rb_ang[0, :3], dw, rtol=1e-6 if dtype == np.float32 else 1e-13, atol=0)
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
