@@ -89,7 +89,7 @@ class ArSimServeConfig(C.Structure):
         ("seed", C.c_uint64), ("numRequests", C.c_int), ("arrivalRate", C.c_double),
         ("minParticles", C.c_int), ("maxParticles", C.c_int), ("minSteps", C.c_int), ("maxSteps", C.c_int),
         ("hbmBudgetBytes", C.c_double), ("maxResident", C.c_int), ("oneAtATime", C.c_int),
-        ("timeScale", C.c_double), ("method", C.c_int),
+        ("timeScale", C.c_double), ("method", C.c_int), ("resultChecksums", C.c_void_p),
     ]
 
 
@@ -99,7 +99,7 @@ class ArSimServeReport(C.Structure):
         ("updatesPerSecond", C.c_double), ("stepLatencyP50Ms", C.c_double), ("stepLatencyP99Ms", C.c_double),
         ("completionLatencyP50Ms", C.c_double), ("completionLatencyP99Ms", C.c_double),
         ("queueWaitP50Ms", C.c_double), ("queueWaitP99Ms", C.c_double), ("maxResidentSeen", C.c_int),
-        ("rejectedForBudget", C.c_int), ("peakDeviceBytes", C.c_double),
+        ("rejectedForBudget", C.c_int), ("peakDeviceBytes", C.c_double), ("deferredForBudget", C.c_int),
     ]
 
 
@@ -162,6 +162,8 @@ SIGNATURES = {
     "arsim_scene_view": (I, [VP, VP]),
     "arsim_scene_free": (I, [VP]),
     "arsim_serve_run": (I, [VP, VP]),
+    "arsim_serve_request": (I, [VP, I, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    "arsim_engine_state_checksum": (I, [VP, I, C.POINTER(C.c_uint64)]),
     "arsim_template_suffix": (C.c_char_p, [F]),
     "arsim_template_load": (I, [C.c_char_p, C.c_char_p, F, F, PVP]),
     "arsim_template_info": (I, [VP, C.POINTER(I), VP, VP, C.POINTER(F), C.POINTER(F)]),

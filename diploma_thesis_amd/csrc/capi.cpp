@@ -56,7 +56,7 @@ int sort_common(K **keys, unsigned **vals, K **altKeys, unsigned **altVals, int 
 
 extern "C" {
 
-int arsim_abi_version(void) { return 1; }
+int arsim_abi_version(void) { return 2; }
 
 const char *arsim_last_error(void) { return arsim_host::g_lastError.c_str(); }
 

@@ -238,6 +238,33 @@ const char *const *Engine::stage_names(int method, unsigned flags, bool bucket, 
 
 size_t Engine::scene_bytes_estimate() const { return devBytes; }
 
+int Engine::state_checksum(int s, uint64_t *out) {
+    if (s < 0 || s >= S || !out) return fail(ARSIM_ERR_INVALID_ARG, "state_checksum: bad scene");
+    ARSIM_HIP(hipSetDevice(device));
+    ARSIM_HIP(hipStreamSynchronize(st));
+    const size_t po = pStart[s], n = pCount[s], bo = bStart[s], b = bCount[s];
+    std::vector<float4> buf(std::max(n, b));
+    uint64_t h = 0xcbf29ce484222325ull;
+    auto mix = [&](const void *src, size_t count) -> int {
+        ARSIM_HIP(hipMemcpy(buf.data(), src, count * 16, hipMemcpyDeviceToHost));
+        const unsigned char *c = reinterpret_cast<const unsigned char *>(buf.data());
+        for (size_t k = 0; k < count * 16; ++k) {
+            h ^= c[k];
+            h *= 0x100000001b3ull;
+        }
+        return ARSIM_OK;
+    };
+    int rc = mix(pos + po, n);
+    if (!rc) rc = mix(vel + po, n);
+    if (!rc && b) rc = mix(rbPos + bo, b);
+    if (!rc && b) rc = mix(rbVel + bo, b);
+    if (!rc && b) rc = mix(rbAng + bo, b);
+    if (!rc && b) rc = mix(qcum + bo, b);
+    if (rc) return rc;
+    *out = h;
+    return ARSIM_OK;
+}
+
 int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning, int dev, hipStream_t stream) {
     if (src.empty()) return fail(ARSIM_ERR_INVALID_ARG, "engine: no scenes");
     if (src.size() > 65535) return fail(ARSIM_ERR_INVALID_ARG, "engine: at most 65535 scenes");
@@ -907,6 +934,11 @@ int arsim_engine_download(arsim_engine *e, int s, float *pos, float *vel, float 
         if (rbCurrentInertia) ARSIM_HIP(hipMemcpy(rbCurrentInertia, E.Iw + bo * 9, b * 36, hipMemcpyDeviceToHost));
     }
     return ARSIM_OK;
+}
+
+int arsim_engine_state_checksum(arsim_engine *e, int s, uint64_t *out) {
+    if (!e) return fail(ARSIM_ERR_INVALID_ARG, "null engine");
+    return e->e.state_checksum(s, out);
 }
 
 int arsim_engine_set_static(arsim_engine *e, const float *pos4, const float *norm4, int n) {

@@ -106,6 +106,9 @@ struct Engine {
     PartBufs part_bufs() const;
     LongFill long_fill() const { return LongFill{longCnt, longList, longCap}; }
     size_t scene_bytes_estimate() const;
+    // FNV-1a (64-bit) over scene s's state in the reference layout: particle pos and vel, then the
+    // body positions, velocities, angular velocities and cumulative quaternions (synchronises)
+    int state_checksum(int s, uint64_t *out);
 
   private:
     template <class T> int alloc(T **p, size_t count);

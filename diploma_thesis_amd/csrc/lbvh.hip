@@ -6,7 +6,7 @@ namespace arsim {
 
 namespace {
 
-constexpr int kStackDepth = 32;   // per-thread LDS stack entries
+constexpr int kStackDepth = 64;   // per-thread LDS stack entries (the reference's int stack[64], RB_BVHKernels.cu:129)
 constexpr int kWaveStackDepth = 64;   // per-wave LDS stack entries of the packet traversal
 
 __device__ __forceinline__ int xcd_block(int b, int nb) {
@@ -369,6 +369,7 @@ __global__ void k_dem_lbvh(const int *__restrict__ sIdx, Lbvh t, int n, DevPhys 
     int node = t.sceneRoot ? t.sceneRoot[s] : (n >= 2 ? n : -1);
     f3 F = mk3(0.f, 0.f, 0.f);
     int cnt = 0, sp = 0;
+    bool ovf = false;   // stack overflow: the traversal is incomplete and nothing is written
     while (node >= n) {
         int L, R;
         bool oL, oR;
@@ -425,6 +426,7 @@ __global__ void k_dem_lbvh(const int *__restrict__ sIdx, Lbvh t, int n, DevPhys 
             if (tL && tR) {
                 if (sp == kStackDepth) {
                     atomicOr(overflow, 1);
+                    ovf = true;
                     break;
                 }
                 sStack[sp * bd + tid] = R;
@@ -432,6 +434,7 @@ __global__ void k_dem_lbvh(const int *__restrict__ sIdx, Lbvh t, int n, DevPhys 
             }
         }
     }
+    if (ovf) return;
     if (EMIT) {
         o.counts[orig] = cnt;
         return;
@@ -476,6 +479,7 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
     const int myRoot = sceneRoot ? sceneRoot[s] : (n >= 2 ? n : -1);
     f3 F = mk3(0.f, 0.f, 0.f);
     int cnt = 0;
+    bool ovf = false;   // this lane's traversal hit the stack bound: nothing is written for it
     unsigned long long pending = __ballot(valid && myRoot >= n);
     while (pending) {
         const int root = __builtin_amdgcn_readfirstlane(__shfl(myRoot, __builtin_ctzll(pending)));
@@ -527,7 +531,10 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
                 node = tL ? L : R;
                 if (tL && tR) {
                     if (sp == kWaveStackDepth) {
-                        if (act) atomicOr(overflow, 1);
+                        if (act) {
+                            atomicOr(overflow, 1);
+                            ovf = true;
+                        }
                         break;
                     }
                     stack[sp] = R;
@@ -536,7 +543,7 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
             }
         }
     }
-    if (!valid) return;
+    if (!valid || ovf) return;
     if (EMIT) {
         o.counts[orig] = cnt;
         return;

@@ -67,7 +67,45 @@ size_t scene_bytes(int n, int b, int method, int cellBits) {
     return (size_t)n * per + (size_t)b * body + cells;
 }
 
+bool config_ok(const ArSimServeConfig *cfg) {
+    return cfg && cfg->numRequests > 0 && cfg->arrivalRate > 0 && cfg->minParticles > 0 &&
+           cfg->maxParticles >= cfg->minParticles && cfg->minSteps > 0 && cfg->maxSteps >= cfg->minSteps;
+}
+
+// The request stream of cfg: Poisson arrivals, log-uniform particle counts, uniform step counts,
+// one scene seed per request.  Deterministic in cfg->seed.
+std::vector<Req> draw_requests(const ArSimServeConfig *cfg) {
+    const double scale = cfg->timeScale > 0 ? cfg->timeScale : 1.0;
+    Rng rng(cfg->seed);
+    std::vector<Req> reqs(cfg->numRequests);
+    double t = 0;
+    const double lmin = std::log((double)cfg->minParticles), lmax = std::log((double)cfg->maxParticles);
+    for (int i = 0; i < cfg->numRequests; ++i) {
+        double u = rng.uni();
+        t += -std::log(1.0 - u) / cfg->arrivalRate;
+        reqs[i].arrival = t / scale;
+        reqs[i].n = (int)std::lround(std::exp(lmin + (lmax - lmin) * rng.uni()));
+        reqs[i].steps = cfg->minSteps + (int)(rng.uni() * (cfg->maxSteps - cfg->minSteps + 1));
+        if (reqs[i].steps > cfg->maxSteps) reqs[i].steps = cfg->maxSteps;
+        reqs[i].seed = rng.next();
+        reqs[i].bytes = scene_bytes(reqs[i].n, (int)(0.75 * reqs[i].n / 64), cfg->method, 18);
+    }
+    return reqs;
+}
+
 }  // namespace
+
+extern "C" int arsim_serve_request(const ArSimServeConfig *cfg, int i, int *numParticles, int *numSteps,
+                                   uint64_t *sceneSeed, double *arrivalSeconds) {
+    if (!config_ok(cfg) || i < 0 || i >= cfg->numRequests)
+        return fail(ARSIM_ERR_INVALID_ARG, "serve_request: bad configuration or index");
+    const std::vector<Req> reqs = draw_requests(cfg);
+    if (numParticles) *numParticles = reqs[i].n;
+    if (numSteps) *numSteps = reqs[i].steps;
+    if (sceneSeed) *sceneSeed = reqs[i].seed;
+    if (arrivalSeconds) *arrivalSeconds = reqs[i].arrival;
+    return ARSIM_OK;
+}
 
 // The config-0 generator scaled to n particles (used by serving and by the batched bench).
 extern "C" int arsim_generate_mix_scene(int n, uint64_t seed, int method, ArSimSceneOwned **out) {
@@ -97,30 +135,20 @@ extern "C" int arsim_generate_mix_scene(int n, uint64_t seed, int method, ArSimS
 
 extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *rep) {
     using clk = std::chrono::steady_clock;
-    if (!cfg || !rep || cfg->numRequests <= 0 || cfg->arrivalRate <= 0 || cfg->minParticles <= 0 ||
-        cfg->maxParticles < cfg->minParticles || cfg->minSteps <= 0 || cfg->maxSteps < cfg->minSteps)
-        return fail(ARSIM_ERR_INVALID_ARG, "serve: bad configuration");
+    if (!config_ok(cfg) || !rep) return fail(ARSIM_ERR_INVALID_ARG, "serve: bad configuration");
+    // the batched loop admits while a slot and the budget allow: with no slot or no budget it could
+    // never admit anything
+    if (!cfg->oneAtATime && (cfg->maxResident <= 0 || !(cfg->hbmBudgetBytes > 0)))
+        return fail(ARSIM_ERR_INVALID_ARG, "serve: batched serving needs maxResident > 0 and hbmBudgetBytes > 0");
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(ARSIM_ERR_NO_DEVICE, "no HIP device");
     std::memset(rep, 0, sizeof(*rep));
-    const double scale = cfg->timeScale > 0 ? cfg->timeScale : 1.0;
     const int method = cfg->method;
+    if (cfg->resultChecksums)
+        for (int i = 0; i < cfg->numRequests; ++i) cfg->resultChecksums[i] = 0;
 
     // ---- request stream
-    Rng rng(cfg->seed);
-    std::vector<Req> reqs(cfg->numRequests);
-    double t = 0;
-    const double lmin = std::log((double)cfg->minParticles), lmax = std::log((double)cfg->maxParticles);
-    for (int i = 0; i < cfg->numRequests; ++i) {
-        double u = rng.uni();
-        t += -std::log(1.0 - u) / cfg->arrivalRate;
-        reqs[i].arrival = t / scale;
-        reqs[i].n = (int)std::lround(std::exp(lmin + (lmax - lmin) * rng.uni()));
-        reqs[i].steps = cfg->minSteps + (int)(rng.uni() * (cfg->maxSteps - cfg->minSteps + 1));
-        if (reqs[i].steps > cfg->maxSteps) reqs[i].steps = cfg->maxSteps;
-        reqs[i].seed = rng.next();
-        reqs[i].bytes = scene_bytes(reqs[i].n, (int)(0.75 * reqs[i].n / 64), method, 18);
-    }
+    std::vector<Req> reqs = draw_requests(cfg);
 
     // ---- clients: request inputs are prepared ahead of their arrival by producer threads (a request
     // arrives with its scene), at most kLookahead requests ahead of the server.  Eight threads keep
@@ -167,7 +195,7 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
 
     std::vector<double> stepLat, complLat, waitLat;
     double updates = 0;
-    int completed = 0, maxResident = 0, rejected = 0;
+    int completed = 0, maxResident = 0, rejected = 0, deferred = 0;
     double peakBytes = 0;
     const auto t0 = clk::now();
     auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
@@ -204,6 +232,8 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
             hipStreamSynchronize(eng.st);
             double s1 = now();
             r.finished = s1;
+            if (cfg->resultChecksums && (rc = eng.state_checksum(0, &cfg->resultChecksums[&r - &reqs[0]])) != 0)
+                return rc;
             for (int k = 0; k < r.steps; ++k) stepLat.push_back((s1 - s0) / r.steps * 1e3);
             complLat.push_back((r.finished - r.arrival) * 1e3);
             waitLat.push_back((r.admitted - r.arrival) * 1e3);
@@ -224,14 +254,19 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
         int cur = 0;
         arsim::Engine *eng = nullptr;
         size_t residentBytes = 0;
-        std::vector<char> countedReject(cfg->numRequests, 0);
-        while (completed < cfg->numRequests) {
+        std::vector<char> countedDefer(cfg->numRequests, 0);
+        while (completed + rejected < cfg->numRequests) {
             bool changed = false;
             // retire
             std::vector<int> keep;
-            for (int id : resident) {
+            for (size_t k = 0; k < resident.size(); ++k) {
+                const int id = resident[k];
                 Req &r = reqs[id];
                 if (r.done >= r.steps) {
+                    if (cfg->resultChecksums) {
+                        const int rc = eng->state_checksum((int)k, &cfg->resultChecksums[id]);
+                        if (rc) return rc;
+                    }
                     complLat.push_back((r.finished - r.arrival) * 1e3);
                     residentBytes -= r.bytes;
                     ++completed;
@@ -246,8 +281,16 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
             while (!pending.empty() && reqs[pending.front()].arrival <= tn &&
                    (int)(keep.size() + admitted.size()) < cfg->maxResident) {
                 Req &r = reqs[pending.front()];
+                if ((double)r.bytes > cfg->hbmBudgetBytes) {
+                    // larger than the whole budget: it can never be admitted, so it is refused
+                    ArSimSceneOwned *sc = take(pending.front());
+                    if (sc) arsim_scene_free(sc);
+                    ++rejected;
+                    pending.pop_front();
+                    continue;
+                }
                 if ((double)(residentBytes + r.bytes) > cfg->hbmBudgetBytes) {
-                    if (!countedReject[pending.front()]) { ++rejected; countedReject[pending.front()] = 1; }
+                    if (!countedDefer[pending.front()]) { ++deferred; countedDefer[pending.front()] = 1; }
                     break;
                 }
                 r.scene = take(pending.front());
@@ -323,6 +366,7 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
     rep->queueWaitP99Ms = pct(waitLat, 0.99);
     rep->maxResidentSeen = maxResident;
     rep->rejectedForBudget = rejected;
+    rep->deferredForBudget = deferred;
     rep->peakDeviceBytes = peakBytes;
     return ARSIM_OK;
 }

@@ -76,6 +76,13 @@ class Engine:
                                                   P(out["rb_iw"])), "engine_download")
         return out
 
+    def checksum(self, scene=0):
+        """FNV-1a over the scene's state (arsim_engine_state_checksum): pos, vel, body pos, vel,
+        angular velocity, orientation."""
+        h = C.c_uint64(0)
+        capi.check(capi.lib.arsim_engine_state_checksum(self.h, int(scene), C.byref(h)), "engine_state_checksum")
+        return h.value
+
     def snapshot(self, scene=0):
         """A HostScene that rebuilds scene `scene` in its current state (arsim_engine_snapshot)."""
         from .scenes import _from_owned

@@ -259,8 +259,10 @@ int arsim_lbvh_refit(const int *sortedIndices, const float *pos, const float *ve
 /* F17  collideBVHSoARigidBodyWrapper (rigidBodyKernelWrappers.h:49-68, RB_BVHKernels.cu:100-244).
  * Stack traversal from the root per sorted leaf; same contact predicate, force, wall term and
  * outputs as arsim_dem_grid.  Neighbour velocities are read from a sorted copy (sortedVel), so
- * no thread reads a velocity another thread is writing.  The per-thread stack holds 32 entries in
- * LDS; a traversal that would exceed it sets the device int *overflowFlag to 1. */
+ * no thread reads a velocity another thread is writing.  The per-thread stack holds 64 entries in
+ * LDS, as the reference's int stack[64]; a traversal that would exceed it sets the device int
+ * *overflowFlag to 1 and writes nothing for that particle (its pForce / pTorque / vel keep their
+ * previous contents), so a truncated sum is never returned as a result. */
 int arsim_dem_lbvh(const int *rbIndices, float *pForce, float *pTorque, float *vel,
                    const float *sortedVel, const float *relativePos, const int *sortedIndices,
                    const int *leftIndices, const int *rightIndices, const float *CMs,
@@ -513,6 +515,10 @@ typedef struct ArSimServeConfig {
     int    oneAtATime;                /* 1: serve FIFO, one scene at a time, plain path */
     double timeScale;                 /* >1 compresses arrival times (for short runs) */
     int    method;
+    /* Optional (NULL = off), numRequests entries: request i's result, the FNV-1a checksum of its
+     * final state as arsim_engine_state_checksum computes it (0 for a request that was not served).
+     * Costs a device-to-host copy of each finished scene. */
+    uint64_t *resultChecksums;
 } ArSimServeConfig;
 
 typedef struct ArSimServeReport {
@@ -524,11 +530,26 @@ typedef struct ArSimServeReport {
     double completionLatencyP50Ms, completionLatencyP99Ms;
     double queueWaitP50Ms, queueWaitP99Ms;
     int    maxResidentSeen;
-    int    rejectedForBudget;
+    int    rejectedForBudget;         /* requests larger than the whole HBM budget: refused, never served */
     double peakDeviceBytes;
+    int    deferredForBudget;         /* requests that waited at least once for budget to free up */
 } ArSimServeReport;
 
+/* Serves the stream.  Batched mode (oneAtATime = 0) needs maxResident > 0 and hbmBudgetBytes > 0
+ * (else ARSIM_ERR_INVALID_ARG).  A request whose memory estimate exceeds the budget on its own is
+ * refused and counted in rejectedForBudget; the run still returns ARSIM_OK once every other
+ * request has completed, so completed + rejectedForBudget == numRequests. */
 int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *report);
+/* Request i of the stream cfg describes (the same deterministic draw arsim_serve_run makes): its
+ * particle count, step count, scene seed (arsim_generate_mix_scene(numParticles, sceneSeed, method))
+ * and arrival time in seconds (after timeScale).  Any output pointer may be NULL. */
+int arsim_serve_request(const ArSimServeConfig *cfg, int i, int *numParticles, int *numSteps, uint64_t *sceneSeed,
+                        double *arrivalSeconds);
+/* FNV-1a (64-bit) over scene s's state bytes: particle pos and vel (float4 each), then the body
+ * positions, velocities, angular velocities and cumulative quaternions.  Synchronises the engine.
+ * No counterpart in the reference; it lets a served request's result be compared with the same
+ * scene stepped alone. */
+int arsim_engine_state_checksum(arsim_engine *e, int scene, uint64_t *out);
 
 #ifdef __cplusplus
 }

@@ -25,7 +25,7 @@ def test_every_declared_symbol_is_exported_and_bound():
     for n in names:
         assert hasattr(lib, n), n
         assert n in capi.SIGNATURES, f"{n} has no ctypes signature"
-    assert capi.lib.arsim_abi_version() == 1
+    assert capi.lib.arsim_abi_version() == 2
 
 
 def test_simparams_layout_matches_reference():
@@ -109,3 +109,28 @@ def test_mix_scene_sizes():
         sc = scenes.mix_scene(n, 1)
         assert sc.n == n
         assert abs((sc.rb >= 0).mean() - 0.75) < 0.05
+
+
+def test_serve_config_validation_and_request_draw():
+    # batched serving without a slot or a budget could never admit a request: refused up front
+    base = dict(seed=1, numRequests=3, arrivalRate=10.0, minParticles=2048, maxParticles=4096, minSteps=1,
+                maxSteps=2, hbmBudgetBytes=1e9, maxResident=4, oneAtATime=0, timeScale=1.0, method=0)
+    rep = capi.ArSimServeReport()
+    for bad in (dict(maxResident=0), dict(maxResident=-1), dict(hbmBudgetBytes=0.0), dict(numRequests=0),
+                dict(maxParticles=1000)):
+        cfg = capi.ArSimServeConfig(**{**base, **bad})
+        assert capi.lib.arsim_serve_run(C.byref(cfg), C.byref(rep)) == capi.ARSIM_ERR_INVALID_ARG, bad
+    # the request draw is deterministic, in range, and its arrivals increase
+    cfg = capi.ArSimServeConfig(**{**base, "numRequests": 50, "maxParticles": 262144, "maxSteps": 2000})
+    n, st, seed, t = C.c_int(), C.c_int(), C.c_uint64(), C.c_double()
+    prev, draws = -1.0, []
+    for i in range(50):
+        assert capi.lib.arsim_serve_request(C.byref(cfg), i, C.byref(n), C.byref(st), C.byref(seed), C.byref(t)) == 0
+        assert 2048 <= n.value <= 262144 and 1 <= st.value <= 2000 and t.value > prev
+        prev = t.value
+        draws.append((n.value, st.value, seed.value))
+    assert capi.lib.arsim_serve_request(C.byref(cfg), 7, C.byref(n), C.byref(st), C.byref(seed), None) == 0
+    assert (n.value, st.value, seed.value) == draws[7]
+    assert len({d[2] for d in draws}) == 50
+    assert capi.lib.arsim_serve_request(C.byref(cfg), 50, None, None, None, None) == capi.ARSIM_ERR_INVALID_ARG
+
