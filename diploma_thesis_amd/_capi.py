@@ -130,6 +130,7 @@ SIGNATURES = {
     "arsim_sort_workspace_bytes": (SZ, [I, I]),
     "arsim_sort_pairs_u32": (I, [PVP, PVP, PVP, PVP, I, I, I, VP, SZ, VP]),
     "arsim_sort_pairs_u64": (I, [PVP, PVP, PVP, PVP, I, I, I, VP, SZ, VP]),
+    "arsim_sort_pairs_u32_noalloc": (I, [PVP, PVP, PVP, PVP, I, I, I, VP]),
     "arsim_reorder_find_cell_start": (I, [VP, VP, VP, VP, VP, VP, VP, VP, VP, I, U, VP]),
     "arsim_dem_grid": (I, [VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, I, VP, VP, VP]),
     "arsim_segmented_reduce_apply": (I, [VP, VP, VP, VP, VP, VP, VP, VP, VP, I, VP, VP, VP]),

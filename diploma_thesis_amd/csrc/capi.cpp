@@ -201,6 +201,23 @@ int arsim_sort_pairs_u64(uint64_t **keys, unsigned int **values, uint64_t **altK
                                  workspaceBytes, st);
 }
 
+int arsim_sort_pairs_u32_noalloc(unsigned int **keys, unsigned int **values, unsigned int **altKeys,
+                                 unsigned int **altValues, int numItems, int beginBit, int endBit,
+                                 arsim_stream_t st) {
+    if (numItems > ARSIM_SORT_NOALLOC_MAX_ITEMS)
+        return fail(ARSIM_ERR_WORKSPACE, "sort_pairs_u32_noalloc: " + std::to_string(numItems) +
+                                             " items exceed the static scratch's limit of " +
+                                             std::to_string(ARSIM_SORT_NOALLOC_MAX_ITEMS));
+    if (numItems <= 0)
+        return sort_common<unsigned>(keys, values, altKeys, altValues, numItems, beginBit, endBit, nullptr, 0, st);
+    void *ws = nullptr;
+    size_t wsBytes = 0;
+    hipError_t e = sort_static_scratch(&ws, &wsBytes);
+    if (e != hipSuccess)
+        return fail(ARSIM_ERR_HIP, std::string("sort_pairs_u32_noalloc: static scratch lookup: ") + hipGetErrorString(e));
+    return sort_common<unsigned>(keys, values, altKeys, altValues, numItems, beginBit, endBit, ws, wsBytes, st);
+}
+
 /* ---------------------------------------------------------------- F7 */
 int arsim_reorder_find_cell_start(const int *rbIndices, unsigned int *cellStart, unsigned int *cellEnd,
                                   float *sortedPos, float *sortedVel, const unsigned int *hash,

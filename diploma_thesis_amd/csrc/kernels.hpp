@@ -34,6 +34,9 @@ struct SortPlan {
     size_t histWords, statusWords;
 };
 SortPlan sort_plan(void *ws, int n, int keyBytes, int beginBit, int endBit, const SortTuning &tu);
+// Device address and size of the sort's static scratch array on the current device, looked up once
+// per device.  It serves as the workspace of u32 sorts of up to ARSIM_SORT_NOALLOC_MAX_ITEMS items.
+hipError_t sort_static_scratch(void **ptr, size_t *bytes);
 
 // ---------------------------------------------------------------- buffers
 struct BodyBufs {

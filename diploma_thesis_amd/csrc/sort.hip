@@ -15,6 +15,9 @@
 // Look-back words pack a 2-bit flag (1 = aggregate, 2 = inclusive prefix) over a 30-bit count, so a
 // relaxed agent-scope atomic load sees flag and value together.  Pass p uses one of two look-back
 // arrays and clears the other for pass p + 1.
+#include <mutex>
+
+#include "../../include/arsim.h"
 #include "kernels.hpp"
 
 namespace arsim {
@@ -247,15 +250,19 @@ void launch_passes(K **keys, unsigned **vals, K **alt, unsigned **altVals, int n
 
 int sort_max_passes(int keyBytes) { return (keyBytes * 8 + 5) / 6; }
 
-size_t sort_workspace_bytes(int n, int keyBytes) {
-    // sized for the smallest tile (IPT 4) and the narrowest digit (6 bits) any tuning may use
-    const int minTile = kBlock * 4;
-    size_t tiles = (size_t)((n + minTile - 1) / minTile);
+namespace {
+// sized for the smallest tile (IPT 4) and the narrowest digit (6 bits) any tuning may use
+constexpr size_t workspace_bytes(long long n, int keyBytes) {
+    const long long minTile = kBlock * 4;
+    long long tiles = (n + minTile - 1) / minTile;
     if (tiles < 1) tiles = 1;
-    size_t passes = (size_t)sort_max_passes(keyBytes);
-    size_t words = passes * 256 + 64 + 2 * tiles * 256;
-    return words * sizeof(unsigned) + 256;
+    const long long passes = (keyBytes * 8 + 5) / 6;
+    const long long words = passes * 256 + 64 + 2 * tiles * 256;
+    return (size_t)words * sizeof(unsigned) + 256;
 }
+}  // namespace
+
+size_t sort_workspace_bytes(int n, int keyBytes) { return workspace_bytes(n, keyBytes); }
 
 SortPlan sort_plan(void *ws, int n, int keyBytes, int beginBit, int endBit, const SortTuning &tu) {
     SortPlan p{};
@@ -316,6 +323,42 @@ template int sort_pairs<unsigned>(unsigned **, unsigned **, unsigned **, unsigne
                                   size_t, const SortTuning &, hipStream_t, bool);
 template int sort_pairs<uint64_t>(uint64_t **, unsigned **, uint64_t **, unsigned **, int, int, int, bool, void *,
                                   size_t, const SortTuning &, hipStream_t, bool);
+
+// ---------------------------------------------------------------- sort without a caller workspace
+// The workspace of a u32-key sort of up to ARSIM_SORT_NOALLOC_MAX_ITEMS items lives in this
+// file-scope device array (8.0 MiB; every device the library runs on holds its own copy until the
+// process ends).  Host code must never pass the array's name to a kernel: on the host it is only a
+// stand-in for the symbol.  The device address comes from hipGetSymbolAddress, taken once per device
+// with that device current, and is kept per device.  Every sort on one device shares the array, so
+// sorts that overlap on one device (two streams, two threads) race; the sort's first kernel clears
+// the histograms, tile counters and look-back words it relies on, so each call starts clean.
+static_assert(ARSIM_SORT_NOALLOC_MAX_ITEMS < (1 << 30), "item limit must fit the sort's index range");
+constexpr size_t kStaticSortScratchBytes = workspace_bytes(ARSIM_SORT_NOALLOC_MAX_ITEMS, 4);
+static __device__ __align__(256) unsigned char scratch[kStaticSortScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void *g_scratchPtr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t sort_static_scratch(void **ptr, size_t *bytes) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratchPtr[dev] = p;
+    }
+    *ptr = g_scratchPtr[dev];
+    *bytes = kStaticSortScratchBytes;
+    return hipSuccess;
+}
 
 bool sort_tuning_supported(int ipt, int rbits) {
     return (rbits == 8 && (ipt == 4 || ipt == 8 || ipt == 12 || ipt == 16)) ||

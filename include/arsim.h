@@ -185,6 +185,18 @@ int arsim_sort_pairs_u64(uint64_t **keys, unsigned int **values,
                          int numItems, int beginBit, int endBit,
                          void *workspace, size_t workspaceBytes, arsim_stream_t stream);
 
+/* No counterpart in the reference.  arsim_sort_pairs_u32 without a workspace, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, results and pointer exchange as arsim_sort_pairs_u32; the
+ * workspace is a static device array of the library (8.0 MiB per device, held until the process
+ * ends), so the call allocates nothing.  numItems above ARSIM_SORT_NOALLOC_MAX_ITEMS is refused
+ * with ARSIM_ERR_WORKSPACE.  All calls on one device share that array: calls that overlap on one
+ * device (different streams or threads) race and must be serialised by the caller. */
+#define ARSIM_SORT_NOALLOC_MAX_ITEMS (1 << 22)
+int arsim_sort_pairs_u32_noalloc(unsigned int **keys, unsigned int **values,
+                                 unsigned int **altKeys, unsigned int **altValues,
+                                 int numItems, int beginBit, int endBit, arsim_stream_t stream);
+
 /* F7  reorderDataAndFindCellStart (ParticleAuxiliaryFunctions.h:25-35, particleKernels.cu:193-258,
  * particleKernelWrappers.cu:213-254).  cellStart is set to 0xffffffff for all numCells, then
  * cellStart[h] / cellEnd[h] bracket the sorted run of hash h; sortedPos/Vel[i] = pos/vel[index[i]],

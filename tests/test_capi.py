@@ -58,6 +58,9 @@ def test_stage_functions_reject_bad_arguments_without_a_device():
     assert L.arsim_segmented_reduce_apply(null, null, null, null, null, null, null, null, null, 3, C.byref(p),
                                           C.byref(cfg), null) == capi.ARSIM_ERR_INVALID_ARG
     assert L.arsim_lbvh_build(null, 3, 5, null, null, null, null, null, null, null, null) == capi.ARSIM_ERR_INVALID_ARG
+    # the allocation-free sort refuses more items than its static scratch holds before touching a device
+    assert L.arsim_sort_pairs_u32_noalloc(null, null, null, null, (1 << 22) + 1, 0, 32,
+                                          null) == capi.ARSIM_ERR_WORKSPACE
     k = C.c_void_p(16)
     assert L.arsim_sort_pairs_u32(C.byref(k), C.byref(k), C.byref(k), C.byref(k), 10, 0, 33, fake, 1 << 20,
                                   null) == capi.ARSIM_ERR_INVALID_ARG
