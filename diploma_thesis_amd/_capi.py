@@ -84,6 +84,14 @@ class ArSimGenSpec(C.Structure):
     ]
 
 
+class ArSimPileSpec(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("numEllipsoids", C.c_int), ("targetParticles", C.c_int), ("edge", C.c_float),
+        ("radius", C.c_float), ("latticeGap", C.c_float), ("surfaceGap", C.c_float),
+        ("gridLog2", C.c_int), ("method", C.c_int),
+    ]
+
+
 class ArSimServeConfig(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64), ("numRequests", C.c_int), ("arrivalRate", C.c_double),
@@ -160,6 +168,7 @@ SIGNATURES = {
     "arsim_engine_set_dt": (I, [VP, F]),
     "arsim_generate_scene": (I, [VP, PVP]),
     "arsim_generate_mix_scene": (I, [I, C.c_uint64, I, PVP]),
+    "arsim_generate_packed_pile": (I, [VP, PVP]),
     "arsim_scene_view": (I, [VP, VP]),
     "arsim_scene_free": (I, [VP]),
     "arsim_serve_run": (I, [VP, VP]),

@@ -55,10 +55,9 @@ BodyShape cube_shape(int edge, double r) {
     return s;
 }
 
-// Voxelised ellipsoid: lattice points (spacing 2r) inside (x/a)² + (y/b)² + (z/c)² <= 1.
-BodyShape ellipsoid_shape(double a, double b, double c, double r) {
+// Voxelised ellipsoid: lattice points of spacing sp inside (x/a)² + (y/b)² + (z/c)² <= 1.
+BodyShape ellipsoid_shape(double a, double b, double c, double r, double sp) {
     BodyShape s;
-    double sp = 2.0 * r;
     int na = (int)std::floor(a / sp), nb = (int)std::floor(b / sp), nc = (int)std::floor(c / sp);
     double cx = 0, cy = 0, cz = 0;
     for (int i = -na; i <= na; ++i)
@@ -143,7 +142,7 @@ extern "C" int arsim_generate_scene(const ArSimGenSpec *spec, ArSimSceneOwned **
     for (int b = 0; b < spec->numEllipsoids; ++b) {
         double lo = spec->ellipsoidMinSemi, hi = spec->ellipsoidMaxSemi;
         double a = lo + (hi - lo) * rng.uni(), bb = lo + (hi - lo) * rng.uni(), c = lo + (hi - lo) * rng.uni();
-        shapes.push_back(ellipsoid_shape(a, bb, c, r));
+        shapes.push_back(ellipsoid_shape(a, bb, c, r, 2.0 * r));
     }
     const int B = (int)shapes.size();
 
@@ -341,6 +340,241 @@ extern "C" int arsim_generate_scene(const ArSimGenSpec *spec, ArSimSceneOwned **
     c.wallMask = ARSIM_WALLS_OPEN_TOP;
     c.method = spec->method;
     c.flags = 0;
+    *out = s;
+    return ARSIM_OK;
+}
+
+namespace {
+
+// Points on a uniform grid of cells of edge h over [0, n*h)^3 (coordinates clamped into it), with
+// linked lists per cell: "is any point closer than d to (x, y, z)?" for d <= h.
+struct PointGrid {
+    double h;
+    int n[3];
+    std::vector<int> head, next;
+    std::vector<double> P;
+    PointGrid(double cell, const double ext[3]) : h(cell) {
+        for (int a = 0; a < 3; ++a) n[a] = std::max(1, (int)std::ceil(ext[a] / cell) + 1);
+        head.assign((size_t)n[0] * n[1] * n[2], -1);
+    }
+    int c(double v, int a) const {
+        int k = (int)std::floor(v / h);
+        return k < 0 ? 0 : (k >= n[a] ? n[a] - 1 : k);
+    }
+    void add(double x, double y, double z) {
+        size_t cell = ((size_t)c(z, 2) * n[1] + c(y, 1)) * n[0] + c(x, 0);
+        next.push_back(head[cell]);
+        head[cell] = (int)(P.size() / 3);
+        P.insert(P.end(), {x, y, z});
+    }
+    bool near(double x, double y, double z, double d) const {
+        const int cx = c(x, 0), cy = c(y, 1), cz = c(z, 2);
+        const double d2 = d * d;
+        for (int k = std::max(cz - 1, 0); k <= std::min(cz + 1, n[2] - 1); ++k)
+            for (int j = std::max(cy - 1, 0); j <= std::min(cy + 1, n[1] - 1); ++j)
+                for (int i = std::max(cx - 1, 0); i <= std::min(cx + 1, n[0] - 1); ++i)
+                    for (int q = head[((size_t)k * n[1] + j) * n[0] + i]; q >= 0; q = next[q]) {
+                        const double dx = P[3 * (size_t)q] - x, dy = P[3 * (size_t)q + 1] - y, dz = P[3 * (size_t)q + 2] - z;
+                        if (dx * dx + dy * dy + dz * dz < d2) return true;
+                    }
+        return false;
+    }
+};
+
+}  // namespace
+
+// Dense settled pile: ellipsoids voxelised at spacing r, placed at random positions and orientations
+// (largest first) so that no two bodies' particles are closer than the lattice distance, then the
+// space around them filled from the floor up with grains on a face-centred cubic lattice.
+extern "C" int arsim_generate_packed_pile(const ArSimPileSpec *spec, ArSimSceneOwned **out) {
+    if (!spec || !out) return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "null argument");
+    if (spec->numEllipsoids < 0 || spec->targetParticles < 0 || !(spec->edge > 0.f) || !(spec->radius > 0.f) ||
+        !(spec->latticeGap > 0.f && spec->latticeGap <= 1.f) || !(spec->surfaceGap > 0.f && spec->surfaceGap <= 1.f) || spec->gridLog2 < 1 || spec->gridLog2 > 12)
+        return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "packed pile: bad counts, sizes or grid");
+    const double r = spec->radius, edge = spec->edge;
+    const double d0 = 2.0 * r * spec->latticeGap;   // nearest-neighbour distance of the lattice
+    const double dS = 2.0 * r * spec->surfaceGap;   // closest a grain may sit to a body particle
+    Rng rng(spec->seed);
+
+    // ---- shapes: semi-axes 2r * 4^(u^2), u ~ U[0,1] per axis, on [2r, 8r] with a heavy tail in size
+    std::vector<BodyShape> shapes;
+    shapes.reserve(spec->numEllipsoids);
+    double bodyParticles = 0, maxCirc = 0;
+    for (int b = 0; b < spec->numEllipsoids; ++b) {
+        double ax[3];
+        for (int a = 0; a < 3; ++a) {
+            const double u = rng.uni();
+            ax[a] = 2.0 * r * std::pow(4.0, u * u);
+        }
+        shapes.push_back(ellipsoid_shape(ax[0], ax[1], ax[2], r, r));
+        bodyParticles += (double)(shapes.back().off.size() / 3);
+        maxCirc = std::max(maxCirc, shapes.back().circum);
+    }
+    if (bodyParticles > spec->targetParticles)
+        return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "packed pile: the bodies alone exceed the particle target");
+    if (edge < 2.0 * maxCirc + 2.0 * r)
+        return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "packed pile: container narrower than the largest body");
+    const double siteVolume = d0 * d0 * d0 / std::sqrt(2.0);
+    const double grainsWanted = spec->targetParticles - bodyParticles;
+    // pile height holding the bodies' voxels and the grains' lattice sites, with 10 % slack
+    double H = 1.1 * (bodyParticles * r * r * r + grainsWanted * siteVolume) / (edge * edge);
+    H = std::max(H, 2.0 * maxCirc + 2.0 * r);
+
+    // ---- random sequential placement of the bodies, largest first
+    std::vector<int> order(shapes.size());
+    for (size_t b = 0; b < shapes.size(); ++b) order[b] = (int)b;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return shapes[a].off.size() > shapes[b].off.size(); });
+    const double ext[3] = {edge, 4.0 * H + 8.0 * maxCirc, edge};
+    PointGrid bodyGrid(2.0 * r, ext);
+    const int B = (int)shapes.size();
+    std::vector<double> centre(3 * (size_t)B), quat(4 * (size_t)B);
+    std::vector<double> world;
+    double yTop = H;
+    for (int k = 0; k < B; ++k) {
+        const int b = order[k];
+        const BodyShape &sh = shapes[b];
+        const size_t np = sh.off.size() / 3;
+        bool placed = false;
+        for (int attempt = 0; !placed; ++attempt) {
+            if (attempt > 0 && attempt % 400 == 0) yTop *= 1.05;   // crowded: let the pile grow
+            if (yTop > ext[1] - 2.0 * sh.circum)
+                return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "packed pile: bodies do not fit the container");
+            const double c[3] = {sh.circum + (edge - 2.0 * sh.circum) * rng.uni(),
+                                 sh.circum + std::max(0.0, yTop - 2.0 * sh.circum) * rng.uni(),
+                                 sh.circum + (edge - 2.0 * sh.circum) * rng.uni()};
+            const double u1 = rng.uni(), u2 = rng.uni(), u3 = rng.uni();
+            const double q[4] = {std::sqrt(1 - u1) * std::sin(6.283185307179586 * u2),
+                                 std::sqrt(1 - u1) * std::cos(6.283185307179586 * u2),
+                                 std::sqrt(u1) * std::sin(6.283185307179586 * u3),
+                                 std::sqrt(u1) * std::cos(6.283185307179586 * u3)};
+            double R[9];
+            quat_to_mat(q, R);
+            world.resize(3 * np);
+            bool ok = true;
+            for (size_t p = 0; p < np && ok; ++p) {
+                // the same FP32 body-frame offsets the scene stores
+                const double x = (float)sh.off[3 * p], y = (float)sh.off[3 * p + 1], z = (float)sh.off[3 * p + 2];
+                world[3 * p] = c[0] + R[0] * x + R[3] * y + R[6] * z;
+                world[3 * p + 1] = c[1] + R[1] * x + R[4] * y + R[7] * z;
+                world[3 * p + 2] = c[2] + R[2] * x + R[5] * y + R[8] * z;
+                ok = !bodyGrid.near(world[3 * p], world[3 * p + 1], world[3 * p + 2], d0);
+            }
+            if (!ok) continue;
+            for (size_t p = 0; p < np; ++p) bodyGrid.add(world[3 * p], world[3 * p + 1], world[3 * p + 2]);
+            for (int a = 0; a < 3; ++a) centre[3 * (size_t)b + a] = c[a];
+            for (int a = 0; a < 4; ++a) quat[4 * (size_t)b + a] = q[a];
+            placed = true;
+        }
+    }
+
+    // ---- grains: FCC half-layers from the floor up, sites closer than d0 to a body particle skipped.
+    // Half-layer L at height r + L a/2 holds the points (r + m a/2, r + n a/2) with m + n + L even,
+    // a = d0 sqrt(2) the cubic cell: every site has 12 neighbours at distance d0.
+    const double a = d0 * std::sqrt(2.0);
+    const int steps = (int)std::floor((edge - 2.0 * r) / (0.5 * a)) + 1;
+    const long long want = (long long)std::llround(grainsWanted);
+    std::vector<double> G;
+    G.reserve(3 * (size_t)std::max(want, 0LL));
+    for (int layer = 0; (long long)(G.size() / 3) < want; ++layer) {
+        const double y = r + layer * 0.5 * a;
+        if (y > ext[1] - r) return arsim_host::fail(ARSIM_ERR_INVALID_ARG, "packed pile: grains do not fit");
+        for (int n = 0; n < steps && (long long)(G.size() / 3) < want; ++n)
+            for (int m = 0; m < steps && (long long)(G.size() / 3) < want; ++m) {
+                if ((m + n + layer) & 1) continue;
+                const double x = r + m * 0.5 * a, z = r + n * 0.5 * a;
+                if (bodyGrid.near(x, y, z, dS)) continue;
+                G.insert(G.end(), {x, y, z});
+            }
+    }
+
+    // ---- output: bodies in Morton order of their centres, then grains in Morton order
+    ArSimSceneOwned *s = new (std::nothrow) ArSimSceneOwned();
+    if (!s) return arsim_host::fail(ARSIM_ERR_OUT_OF_MEMORY, "host allocation failed");
+    double top = 0;
+    for (size_t p = 1; p < bodyGrid.P.size(); p += 3) top = std::max(top, bodyGrid.P[p]);
+    for (size_t p = 1; p < G.size(); p += 3) top = std::max(top, G[p]);
+    const double box[3] = {edge, std::max(edge, top + 2.0 * r), edge};
+    std::vector<int> bo(B);
+    std::vector<uint64_t> code(B);
+    for (int b = 0; b < B; ++b) {
+        bo[b] = b;
+        code[b] = morton_code(centre[3 * (size_t)b], centre[3 * (size_t)b + 1], centre[3 * (size_t)b + 2], box);
+    }
+    std::stable_sort(bo.begin(), bo.end(), [&](int x, int y) { return code[x] < code[y]; });
+    for (int k = 0; k < B; ++k) {
+        const int b = bo[k];
+        const BodyShape &sh = shapes[b];
+        const double *q = &quat[4 * (size_t)b], *c = &centre[3 * (size_t)b];
+        double R[9];
+        quat_to_mat(q, R);
+        const size_t np = sh.off.size() / 3;
+        double I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (size_t p = 0; p < np; ++p) {
+            const float fx = (float)sh.off[3 * p], fy = (float)sh.off[3 * p + 1], fz = (float)sh.off[3 * p + 2];
+            const double x = fx, y = fy, z = fz;
+            I[0] += y * y + z * z; I[3] -= x * y; I[6] -= x * z;
+            I[1] -= x * y; I[4] += x * x + z * z; I[7] -= y * z;
+            I[2] -= x * z; I[5] -= y * z; I[8] += x * x + y * y;
+            const double w[3] = {c[0] + R[0] * x + R[3] * y + R[6] * z, c[1] + R[1] * x + R[4] * y + R[7] * z,
+                                 c[2] + R[2] * x + R[5] * y + R[8] * z};
+            s->pos.insert(s->pos.end(), {(float)w[0], (float)w[1], (float)w[2], 0.f});
+            s->vel.insert(s->vel.end(), {0.f, 0.f, 0.f, 0.f});
+            s->bodyFrame.insert(s->bodyFrame.end(), {fx, fy, fz, 0.f});
+            s->rb.push_back(k);
+        }
+        double Iinv[9];
+        inv3(I, Iinv);
+        for (int m = 0; m < 9; ++m) s->rbInertia.push_back((float)Iinv[m]);
+        s->rbPos.insert(s->rbPos.end(), {(float)c[0], (float)c[1], (float)c[2], 0.f});
+        s->rbVel.insert(s->rbVel.end(), {0.f, 0.f, 0.f, 0.f});
+        s->rbAng.insert(s->rbAng.end(), {0.f, 0.f, 0.f, 0.f});
+        s->rbQuat.insert(s->rbQuat.end(), {(float)q[0], (float)q[1], (float)q[2], (float)q[3]});
+        s->rbMass.push_back((float)np);
+        s->ppo.push_back((int)np);
+    }
+    const size_t ng = G.size() / 3;
+    std::vector<uint64_t> gcode(ng);
+    std::vector<int> go(ng);
+    for (size_t g = 0; g < ng; ++g) {
+        go[g] = (int)g;
+        gcode[g] = morton_code(G[3 * g], G[3 * g + 1], G[3 * g + 2], box);
+    }
+    std::stable_sort(go.begin(), go.end(), [&](int x, int y) { return gcode[x] < gcode[y]; });
+    for (size_t k = 0; k < ng; ++k) {
+        const size_t g = (size_t)go[k];
+        s->pos.insert(s->pos.end(), {(float)G[3 * g], (float)G[3 * g + 1], (float)G[3 * g + 2], 0.f});
+        s->vel.insert(s->vel.end(), {0.f, 0.f, 0.f, 0.f});
+        s->bodyFrame.insert(s->bodyFrame.end(), {0.f, 0.f, 0.f, 0.f});
+        s->rb.push_back(-1);
+    }
+    s->N = (int)s->rb.size();
+    s->B = B;
+    ArSimParams &p = s->params;
+    std::memset(&p, 0, sizeof(p));
+    p.gravity = {0.f, -9.81f, 0.f};
+    p.globalDamping = 1.0f;
+    p.particleRadius = spec->radius;
+    const unsigned g = 1u << spec->gridLog2;
+    p.gridSize = {g, g, g};
+    p.numCells = g * g * g;
+    p.worldOrigin = {0.f, 0.f, 0.f};
+    p.cellSize = {2.0f * spec->radius, 2.0f * spec->radius, 2.0f * spec->radius};
+    p.numBodies = (unsigned)B;
+    p.spring = 5e4f;
+    p.damping = 50.f;
+    p.shear = 10.f;
+    p.attraction = 0.0f;
+    p.boundaryDamping = -0.5f;
+    p.Wdamping = 1.0f;
+    p.Vdamping = 1.0f;
+    ArSimStepConfig &cf = s->cfg;
+    std::memset(&cf, 0, sizeof(cf));
+    cf.dt = 1e-4f;
+    for (int m = 0; m < 3; ++m) { cf.boxMin[m] = 0.f; cf.boxMax[m] = (float)box[m]; }
+    cf.wallMask = ARSIM_WALLS_OPEN_TOP;
+    cf.method = spec->method;
+    cf.flags = 0;
     *out = s;
     return ARSIM_OK;
 }

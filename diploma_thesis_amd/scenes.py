@@ -142,19 +142,23 @@ def config1(method=capi.METHOD_GRID, seed=0):
                           method=method, body_sigma=1.0, grain_sigma=1.0))
 
 
-def config2(method=capi.METHOD_GRID, seed=0, scale=1.0):
-    """Dense pile: 16,384 voxelised ellipsoids (semi-axes U[2r, 8r]) plus grains to ~4.19 M particles,
-    dropped from heights in [0, 200], v = 0.  scale < 1 shrinks counts and box for tests."""
-    nb = max(1, int(16384 * scale))
-    box = 256.0 * scale ** (1.0 / 3.0)
-    spec = _spec(seed=seed, ellipsoids=nb, semi=(1.0, 4.0), box=(box, 200.0 * max(scale, 0.05) ** (1 / 3), box),
-                 grid_log2=(8, 8, 8), method=method, body_sigma=0.0, grain_sigma=0.0, drop=200.0)
-    probe = generate(spec)
-    target = int(4194304 * scale)
-    grains = max(0, target - probe.n)
-    spec.box[1] = max(spec.box[1], 200.0)
-    spec.numGrains = grains
-    return generate(spec)
+def config2(method=capi.METHOD_GRID, seed=0, scale=1.0, gap=0.98, surface=0.9):
+    """Dense settled pile: 16,384 ellipsoids (semi-axes on [2r, 8r], voxelised at spacing r) and
+    grains on a compressed FCC lattice, 4,194,304 particles in a container 160 wide, grid 256^3.
+    scale < 1 shrinks the counts and the container's floor area (same pile height) for tests."""
+    sp = capi.ArSimPileSpec()
+    sp.seed = seed
+    sp.numEllipsoids = max(1, int(round(16384 * scale)))
+    sp.targetParticles = int(round(4194304 * scale))
+    sp.edge = 160.0 * math.sqrt(scale)
+    sp.radius = 0.5
+    sp.latticeGap = gap
+    sp.surfaceGap = surface
+    sp.gridLog2 = 8
+    sp.method = method
+    owned = C.c_void_p()
+    capi.check(capi.lib.arsim_generate_packed_pile(C.byref(sp), C.byref(owned)), "generate_packed_pile")
+    return _from_owned(owned)
 
 
 def batch_mix(num_scenes=512, lo=1024, hi=65536, seed=0, method=capi.METHOD_GRID):

@@ -463,6 +463,26 @@ int arsim_generate_scene(const ArSimGenSpec *spec, ArSimSceneOwned **out);
  * free grains, box edge 64 * (n / 16384)^(1/3), r = 0.5, k = 5e4, c = 50, shear = 10, dt = 1e-4.
  * Used for the batched and served mixes. */
 int arsim_generate_mix_scene(int n, uint64_t seed, int method, ArSimSceneOwned **out);
+/* Dense settled pile (the 4 M-particle workload).  numEllipsoids ellipsoids with semi-axes
+ * 2r * 4^(u^2), u ~ U[0,1] per axis (on [2r, 8r]), voxelised at spacing r (33 to 2,145 particles,
+ * heavy-tailed), are placed at random positions and orientations, largest first, so that no two
+ * bodies' particles are closer than 2r * latticeGap.  The space around them is then filled from the
+ * floor up with grains on a face-centred cubic lattice of nearest-neighbour distance 2r * latticeGap
+ * (12 neighbours in contact for latticeGap < 1), skipping sites closer than 2r * surfaceGap to a body
+ * particle, until the scene holds targetParticles.  Container [0, edge] in x and z with an open top, v = 0, grid
+ * 2^gridLog2 cells of 2r per axis, k = 5e4, c = 50, shear 10, dt = 1e-4, g = -9.81. */
+typedef struct ArSimPileSpec {
+    uint64_t seed;
+    int   numEllipsoids;
+    int   targetParticles;
+    float edge;
+    float radius;
+    float latticeGap;
+    float surfaceGap;
+    int   gridLog2;
+    int   method;
+} ArSimPileSpec;
+int arsim_generate_packed_pile(const ArSimPileSpec *spec, ArSimSceneOwned **out);
 int arsim_scene_view(const ArSimSceneOwned *s, ArSimSceneHost *view);
 int arsim_scene_free(ArSimSceneOwned *s);
 

@@ -1,0 +1,80 @@
+"""The benchmark configurations themselves, stepped on the GPU and compared bit for bit with the
+oracle: the 2^20-particle scene with the 30- and 63-bit LBVH, the 512-scene batch as one LBVH forest,
+and the full 4,194,304-particle dense pile.  The engine's synchronize reports an LBVH traversal-stack
+overflow as an error, so a passing LBVH test also shows that no traversal overflowed."""
+import numpy as np
+import pytest
+
+import oracle
+from diploma_thesis_amd import Engine, capi, scenes
+from test_gpu_engine import assert_state_equal
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("method", [capi.METHOD_LBVH30, capi.METHOD_LBVH63])
+def test_benchmark_scene_lbvh_two_steps_bit_exact(method):
+    """8,192 cubes + 524,288 grains in [0,512]^3: 2^20 leaves whose 30-bit codes hold many duplicates
+    (the index tie-break decides those subtrees), two graph-replayed steps."""
+    print(f"engine: 2^20-particle benchmark scene, LBVH method={method}, 2 steps", flush=True)
+    sc = scenes.config1(method)
+    assert sc.n == 1 << 20
+    with Engine(sc, use_graph=True) as e:
+        e.step(2)
+        e.sync()                      # raises on a traversal-stack overflow
+        got = e.download()
+    st = oracle.State(sc, np.float32)
+    st.step(2)
+    assert_state_equal(got, st, f"benchmark scene, method {method}")
+
+
+def test_benchmark_batch_lbvh_forest_one_step():
+    """The 512-scene batch (8.85 M particles) as one LBVH forest: 64-bit keys with the scene id above
+    bit 30, one tree per scene.  Every scene's state after a step equals the oracle's step of that
+    scene alone."""
+    print("engine: 512-scene benchmark batch, LBVH30 forest, 1 step", flush=True)
+    mix = scenes.batch_mix(512, seed=0, method=capi.METHOD_LBVH30)
+    assert sum(s.n for s in mix) > 8_000_000
+    with Engine(mix, use_graph=True) as e:
+        e.step(1)
+        e.sync()
+        got = [e.download(k) for k in range(len(mix))]
+    for k, sc in enumerate(mix):
+        st = oracle.State(sc, np.float32)
+        st.step(1)
+        assert_state_equal(got[k], st, f"forest scene {k}")
+
+
+def dense_pile_stats(sc):
+    """Body sizes and contacts per particle of a pile (contact = the DEM predicate: closer than 2r and
+    not two particles of one body)."""
+    from scipy.spatial import cKDTree
+    pos = sc.pos[:, :3].astype(np.float64)
+    pairs = cKDTree(pos).query_pairs(2.0 * sc.params.particleRadius, output_type="ndarray")
+    a, b = sc.rb[pairs[:, 0]], sc.rb[pairs[:, 1]]
+    dem = pairs[(a != b) | (a < 0)]
+    per = np.bincount(dem.ravel(), minlength=sc.n)
+    sizes = np.bincount(sc.rb[sc.rb >= 0])
+    return {"sizes": sizes, "contacts": per.mean(), "grain_contacts": per[sc.rb < 0].mean(),
+            "pairs_within_2r": 2.0 * len(pairs) / sc.n}
+
+
+@pytest.mark.parametrize("method", [capi.METHOD_GRID, capi.METHOD_LBVH30])
+def test_dense_pile_full_size_one_step(method):
+    """16,384 ellipsoids voxelised at spacing r (33 to ~2,000 particles each, heavy-tailed) in a
+    packed bed of grains, 4,194,304 particles: one step equals the oracle's."""
+    print(f"engine: dense pile, 4,194,304 particles, method={method}, 1 step", flush=True)
+    sc = scenes.config2(method)
+    assert sc.n == 4194304 and sc.b == 16384
+    st = dense_pile_stats(sc)
+    print(f"  body particles {st['sizes'].min()}..{st['sizes'].max()} (median {np.median(st['sizes']):.0f}), "
+          f"contacts/particle {st['contacts']:.2f} (grains {st['grain_contacts']:.2f}), "
+          f"pairs within 2r/particle {st['pairs_within_2r']:.1f}", flush=True)
+    assert st["sizes"].min() >= 30 and st["sizes"].max() > 1000 and st["grain_contacts"] > 6.0
+    with Engine(sc, use_graph=True) as e:
+        e.step(1)
+        e.sync()
+        got = e.download()
+    ref = oracle.State(sc, np.float32)
+    ref.step(1)
+    assert_state_equal(got, ref, f"dense pile, method {method}")

@@ -67,8 +67,8 @@ def test_reference_compat_flags_step(method):
 @pytest.mark.parametrize("method", [capi.METHOD_GRID, capi.METHOD_LBVH63])
 def test_ellipsoid_pile_steps(method):
     print(f"engine: ellipsoid pile method={method}", flush=True)
-    sc = scenes.config2(method, seed=1, scale=0.004)
-    assert sc.b > 20 and sc.ppo.min() < sc.ppo.max()
+    sc = scenes.config2(method, seed=1, scale=0.01)
+    assert sc.b > 100 and sc.ppo.min() >= 30 and sc.ppo.max() > 8 * sc.ppo.min()
     I = sc.rb_inertia.reshape(-1, 3, 3)
     assert np.any(np.abs(I[:, 0, 0] - I[:, 1, 1]) > 1e-6)       # anisotropic inverse inertia
     sc.rb_ang[:, :3] = np.random.default_rng(0).normal(scale=2.0, size=(sc.b, 3)).astype(np.float32)
@@ -262,3 +262,4 @@ def test_benchmark_batch_one_step_bit_exact():
         st = oracle.State(sc, np.float32)
         st.step(1)
         assert_state_equal(got[k], st, f"batch scene {k}")
+
