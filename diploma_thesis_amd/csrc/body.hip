@@ -1,6 +1,7 @@
 // body.hip -- rigid-body integration, particle pose / grain integration with key generation,
 // the segmented per-body reduction, and small utility kernels.
 #include "kernels.hpp"
+#include "reduce_apply.hpp"
 
 namespace arsim {
 
@@ -324,37 +325,15 @@ __global__ __launch_bounds__(256) void k_reduce_apply(const float4 *__restrict__
     const int lane = threadIdx.x & 63;
     if (body >= B) return;
     const int st = b.pstart[body], cnt = b.pcount[body];
-    const int j0 = ((ph.flags & kDropFirst) && body > 0) ? 1 : 0;
     f3 aF = mk3(0.f, 0.f, 0.f), aT = mk3(0.f, 0.f, 0.f);
-    for (int j = j0 + lane; j < cnt; j += 64) {
+    for (int j = reduce_first(body, ph) + lane; j < cnt; j += 64) {
         size_t o = (size_t)st + j;
         f3 F = xyz(pF[o]);
         aF = add3(aF, F);
         f3 T = pT ? xyz(pT[o]) : cross3(xyz(rel[o]), F);
         aT = add3(aT, T);
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        aF.x = aF.x + __shfl_xor(aF.x, off, 64);
-        aF.y = aF.y + __shfl_xor(aF.y, off, 64);
-        aF.z = aF.z + __shfl_xor(aF.z, off, 64);
-        aT.x = aT.x + __shfl_xor(aT.x, off, 64);
-        aT.y = aT.y + __shfl_xor(aT.y, off, 64);
-        aT.z = aT.z + __shfl_xor(aT.z, off, 64);
-    }
-    if (lane != 0) return;
-    const float sc = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
-    const float m = b.mass[body];
-    f3 dv = mk3((aF.x * sc) / m, (aF.y * sc) / m, (aF.z * sc) / m);
-    float Iw[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Iw[k] = b.Iw[(size_t)body * 9 + k];
-    f3 dw = mat_vec(Iw, mk3(aT.x * sc, aT.y * sc, aT.z * sc));
-    float4 V = b.vel[body], W = b.ang[body];
-    V.x += dv.x; V.y += dv.y; V.z += dv.z;
-    W.x += dw.x; W.y += dw.y; W.z += dw.z;
-    b.vel[body] = V;
-    b.ang[body] = W;
+    wave_reduce_apply(aF, aT, lane, body, b, ph);
 }
 
 __global__ void k_fill_u32(unsigned *p, unsigned v, size_t n) {
@@ -436,6 +415,14 @@ __global__ void k_copy_offset(int *dst, const int *src, int n, int delta) {
     dst[i] = v >= 0 ? v + delta : v;
 }
 
+// dst[r.x + k] = r.y + k for k < r.z, one range r per workgroup (grid-stride over the ranges).
+__global__ void k_iota_ranges(const int4 *__restrict__ ranges, int numRanges, int *__restrict__ dst) {
+    for (int r = blockIdx.x; r < numRanges; r += gridDim.x) {
+        const int4 q = ranges[r];
+        for (int k = threadIdx.x; k < q.z; k += blockDim.x) dst[q.x + k] = q.y + k;
+    }
+}
+
 __global__ void k_fill_u16(uint16_t *p, uint16_t v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -497,6 +484,12 @@ __global__ void k_carry_bodies(const CarryRange *__restrict__ tab, int R, int ca
 void launch_copy_offset(int *dst, const int *src, int n, int delta, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_copy_offset, dim3(blocks_for(n, 256)), dim3(256), 0, st, dst, src, n, delta);
+}
+
+void launch_iota_ranges(const int4 *ranges, int numRanges, int *dst, hipStream_t st) {
+    if (numRanges <= 0) return;
+    hipLaunchKernelGGL(k_iota_ranges, dim3((unsigned)(numRanges < 4096 ? numRanges : 4096)), dim3(256), 0, st, ranges, numRanges,
+                       dst);
 }
 
 void launch_fill_u16(uint16_t *p, uint16_t value, size_t count, hipStream_t st) {

@@ -211,7 +211,7 @@ PartBufs Engine::part_bufs() const {
     return p;
 }
 
-const char *const *Engine::stage_names(int method, unsigned flags, bool bucket, int *count) {
+const char *const *Engine::stage_names(int method, unsigned flags, bool bucket, bool fusedDem, int *count) {
     // with the bucket sort the pose stage also ranks each tile by the top digit (the sort's first pass)
     // and the bucket stage finishes the sort with the reorder
     static const char *const sis[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "sis_find",
@@ -228,7 +228,13 @@ const char *const *Engine::stage_names(int method, unsigned flags, bool bucket, 
                                         "reduce_apply"};
     static const char *const lbvh[] = {"body_integrate", "pose_key", "sort", "lbvh_build", "lbvh_refit", "dem_lbvh",
                                        "reduce_apply"};
+    static const char *const gridF[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "dem_reduce"};
+    static const char *const gridBF[] = {"body_integrate", "pose_key", "bucket_sort_reorder", "dem_reduce"};
     if (method == ARSIM_METHOD_GRID) {
+        if (fusedDem) {
+            *count = bucket ? 4 : 5;
+            return bucket ? gridBF : gridF;
+        }
         *count = bucket ? 5 : 6;
         return bucket ? gridB : grid;
     }
@@ -336,6 +342,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         return fail(ARSIM_ERR_UNSUPPORTED, "engine: the sequential-impulse model runs on the grid method");
     geoms.resize(S);
     pStart.resize(S); pCount.resize(S); bStart.resize(S); bCount.resize(S);
+    std::vector<int> newBodyParticles(S, 0);   // assigned to bodyParticles once the sources are read
     long long nTot = 0, bTot = 0;
     int maxCellBits = 0;
     long long totalCells = 0;
@@ -368,13 +375,13 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         keyBits = std::max(1, ceil_log2(totalCells));
         keyBytes = 4;
         numCells = (size_t)totalCells;
-        // one entry per cell (the DEM then reads no keys) up to 2^27 cells = 512 MiB of table;
-        // beyond that, segments of 2^segBits cells keep the table at that size
-        segBits = keyBits > 27 ? keyBits - 27 : 0;
+        // one entry per cell (the DEM then reads no keys) up to 2^30 cells = 4 GiB of table, rewritten
+        // every step; beyond that, segments of 2^segBits cells keep the table at that size
+        segBits = keyBits > kExactTableBits ? keyBits - kExactTableBits : 0;
         numSegs = (long long)((numCells - 1) >> segBits) + 1;
         bucketSort = segBits == 0 && bucket_sort_fits(nTot, keyBits, numCells);
         if ((flags & ARSIM_CONTACT_SIS) && segBits != 0)
-            return fail(ARSIM_ERR_UNSUPPORTED, "engine: the sequential-impulse model needs at most 2^27 cells");
+            return fail(ARSIM_ERR_UNSUPPORTED, "engine: the sequential-impulse model needs at most 2^30 cells");
         long long base = 0;
         for (int s = 0; s < S; ++s) {
             geoms[s].cellBase = (unsigned)base;
@@ -406,6 +413,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     sortWs = sortWsBuf;
     A(overflowD, 4); A(flagD, 4);
     if (method == ARSIM_METHOD_GRID) {
+        A(grainIdx, N); A(grainRanges, (size_t)S);
         A(segStart, (size_t)numSegs + 1);
         A(tileInfo, (size_t)256 * pose_tile_count(N));
         A(sortedPos, N); A(sortedVel, N);
@@ -443,6 +451,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         for (int s = 0; s < bulk; ++s) {
             const int k = src[s].fromScene;
             tab[s] = CarryRange{pStart[s], bulkFrom->pStart[k], pCount[s], bStart[s], bulkFrom->bStart[k], bCount[s], s};
+            newBodyParticles[s] = bulkFrom->bodyParticles[k];
         }
         ARSIM_HIP(hipMemcpyAsync(carryTab, tab.data(), sizeof(CarryRange) * bulk, hipMemcpyHostToDevice, st));
         const int carriedN = pStart[bulk - 1] + pCount[bulk - 1], carriedB = bStart[bulk - 1] + bCount[bulk - 1];
@@ -485,15 +494,24 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
                         if (j - (int)po >= (int)n || h->rbIndices[j - po] != (int)k)
                             return fail(ARSIM_ERR_INVALID_ARG, "engine: body particles must be contiguous, in body order");
                 }
+                // grains follow the bodies (the reference layout)
+                for (int j = acc; j < (int)(po + n); ++j)
+                    if (h->rbIndices[j - po] != -1)
+                        return fail(ARSIM_ERR_INVALID_ARG, "engine: grains must follow the body particles");
+                newBodyParticles[s] = acc - (int)po;
                 ARSIM_HIP(hipMemcpyAsync(qinc + bo, ident.data(), b * 16, hipMemcpyHostToDevice, st));
                 ARSIM_HIP(hipMemcpyAsync(pstartD + bo, ps.data(), b * 4, hipMemcpyHostToDevice, st));
                 ARSIM_HIP(hipMemcpyAsync(pcountD + bo, pc.data(), b * 4, hipMemcpyHostToDevice, st));
                 ARSIM_HIP(hipStreamSynchronize(st));
+            } else {
+                for (size_t j = 0; j < n; ++j)
+                    if (h->rbIndices[j] != -1) return fail(ARSIM_ERR_INVALID_ARG, "engine: rbIndex without a body");
             }
         } else {
             const Engine *o = src[s].from;
             const int k = src[s].fromScene;
             const size_t opo = o->pStart[k], obo = o->bStart[k];
+            newBodyParticles[s] = o->bodyParticles[k];
             ARSIM_HIP(hipMemcpyAsync(pos + po, o->pos + opo, n * 16, hipMemcpyDeviceToDevice, st));
             ARSIM_HIP(hipMemcpyAsync(vel + po, o->vel + opo, n * 16, hipMemcpyDeviceToDevice, st));
             ARSIM_HIP(hipMemcpyAsync(bodyFrame + po, o->bodyFrame + opo, n * 16, hipMemcpyDeviceToDevice, st));
@@ -514,6 +532,20 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
             }
         }
         launch_fill_u16(scene + po, (uint16_t)s, n, st);
+    }
+    bodyParticles = newBodyParticles;
+    numGrains = 0;
+    if (method == ARSIM_METHOD_GRID) {
+        // the list of grain indices the fused contact kernel walks: each scene's grains follow its bodies
+        std::vector<int4> gr(S);
+        for (int s = 0; s < S; ++s) {
+            const int cnt = pCount[s] - bodyParticles[s];
+            gr[s] = make_int4(numGrains, pStart[s] + bodyParticles[s], cnt, 0);
+            numGrains += cnt;
+        }
+        ARSIM_HIP(hipMemcpyAsync(grainRanges, gr.data(), sizeof(int4) * S, hipMemcpyHostToDevice, st));
+        launch_iota_ranges(grainRanges, S, grainIdx, st);
+        ARSIM_HIP(hipStreamSynchronize(st));   // gr is a host temporary
     }
     ARSIM_HIP(hipMemcpyAsync(geomD, geoms.data(), sizeof(SceneGeom) * S, hipMemcpyHostToDevice, st));
     ARSIM_HIP(hipMemcpyAsync(sceneStartD, hStart.data(), 4 * (size_t)S, hipMemcpyHostToDevice, st));
@@ -655,15 +687,22 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         unsigned *kA = nullptr, *vA = nullptr;
         // the DEM below writes every grain's velocity, so the pose kernel's copy is not stored
         if (int rc = enqueue_grid_broadphase(bb, pb, sv, stt, true, &kA, &vA, rec)) return rc;
-        DemOut o{};
-        o.pForce = pForce; o.vel = vel; o.relPos = relPos;
-        launch_dem_grid_seg(sortedPos, sortedVel, kA, vA, segStart, segBits, N, ph, sv, S > 1, o, tu.demBlock, st);
-        if (numStatic)
-            launch_dem_static(sortedPos, sortedVel, vA, N, stPos, stNorm, nullptr, stTable, nullptr, ph, geoms[0],
-                              pForce, nullptr, relPos, false, st);
-        rec();
-        launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
-        rec();
+        if (fused_dem()) {
+            const FusedDemIn in{gatherRec, relPos, grainIdx, numGrains, vel};
+            launch_dem_fused(sortedPos, sortedVel, vA, segStart, in, bb, B, ph, sv, S > 1, st);
+            rec();
+        } else {
+            DemOut o{};
+            o.pForce = pForce; o.vel = vel; o.relPos = relPos;
+            launch_dem_grid_seg(sortedPos, sortedVel, kA, vA, segStart, segBits, N, ph, sv, S > 1, o, tu.demBlock,
+                                st);
+            if (numStatic)
+                launch_dem_static(sortedPos, sortedVel, vA, N, stPos, stNorm, nullptr, stTable, nullptr, ph, geoms[0],
+                                  pForce, nullptr, relPos, false, st);
+            rec();
+            launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
+            rec();
+        }
         prevSorted = kA;
         lastKeys = kA;
         lastIdx = vA;
@@ -1021,7 +1060,7 @@ int arsim_engine_profile_step(arsim_engine *e, float *ms, const char **names, in
     if (!e) return fail(ARSIM_ERR_INVALID_ARG, "null engine");
     Engine &E = e->e;
     int count = 0;
-    const char *const *nm = Engine::stage_names(E.method, E.flags, E.bucketSort, &count);
+    const char *const *nm = Engine::stage_names(E.method, E.flags, E.bucketSort, E.fused_dem(), &count);
     std::vector<hipEvent_t> ev(count + 1);
     for (auto &x : ev) ARSIM_HIP(hipEventCreate(&x));
     int ns = 0;
@@ -1127,7 +1166,7 @@ int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best,
                 if (!arsim::sort_tuning_supported(ip, rb)) continue;
                 for (auto &s : snaps)
                     if (s.dst) hipMemcpyAsync((void *)s.src, s.dst, s.bytes, hipMemcpyDeviceToDevice, E.st);
-                E.tu = ArSimTuning{bl, ip, rb, orig.useGraph};
+                E.tu = ArSimTuning{bl, ip, rb, orig.useGraph, orig.fuseReduce};
                 E.reset_grid_tables();
                 if ((rc = E.step(2)) != 0) break;
                 hipEventRecord(a, E.st);

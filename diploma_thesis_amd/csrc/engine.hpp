@@ -17,6 +17,9 @@ struct SceneSource {
     int fromScene = -1;
 };
 
+// Widest grid key (all scenes' cells) with one cell-table entry per cell.
+constexpr int kExactTableBits = 30;
+
 struct Engine {
     int device = 0;
     hipStream_t st = nullptr;
@@ -27,9 +30,11 @@ struct Engine {
     ArSimParams p0{};
     ArSimStepConfig c0{};
     DevPhys ph{};
-    ArSimTuning tu{128, 16, 8, 1};
+    ArSimTuning tu{128, 16, 8, 1, 0};
     std::vector<SceneGeom> geoms;
     std::vector<int> pStart, pCount, bStart, bCount;
+    std::vector<int> bodyParticles;         // per scene: its body particles (they precede its grains)
+    int numGrains = 0;
     int cellBits = 0, sceneBits = 0, keyBits = 0, keyBytes = 4;
     size_t numCells = 0;
 
@@ -47,6 +52,8 @@ struct Engine {
     float *Ib = nullptr, *Iw = nullptr, *mass = nullptr;
     float *model = nullptr;       // per-body column-major mat4 (render state), written each step
     int *pstartD = nullptr, *pcountD = nullptr;
+    int *grainIdx = nullptr;                // grid: every grain's particle index, in order
+    int4 *grainRanges = nullptr;            // grid: per scene (first list slot, first grain, grain count)
     SceneGeom *geomD = nullptr;
     int *sceneStartD = nullptr, *sceneCountD = nullptr;
     void *keysA = nullptr, *keysB = nullptr;
@@ -99,7 +106,10 @@ struct Engine {
     int set_static(const float *pos4, const float *norm4, int n);
     void free_static();
     int reset_grid_tables();
-    static const char *const *stage_names(int method, unsigned flags, bool bucket, int *count);
+    static const char *const *stage_names(int method, unsigned flags, bool bucket, bool fusedDem, int *count);
+    // the grid DEM and the per-body reduction run as one kernel (on request; exact cell table, no static set)
+    bool fused_dem() const { return tu.fuseReduce && method == ARSIM_METHOD_GRID && !(flags & ARSIM_CONTACT_SIS) &&
+                                    segBits == 0 && numStatic == 0; }
     SceneView scene_view() const;
     BodyBufs body_bufs() const;
     StateBufs state_bufs() const;

@@ -1,6 +1,8 @@
 // grid.hip -- uniform-grid broadphase: cell bounds + reorder, and the 27-cell DEM kernel.
 #include "csr_fill.hpp"
 #include "kernels.hpp"
+#include "reduce_apply.hpp"
+
 
 namespace arsim {
 
@@ -195,13 +197,17 @@ __device__ __forceinline__ void scan_cells(const unsigned *__restrict__ keys, co
     if (cur != 0xffffffffu) F = add3(F, csum);
 }
 
-// Contacts of one sorted particle with the sorted run [j, e) that holds up to three consecutive
-// cells, the second starting at b1 and the third at b2 (exact cell offsets: no key is read).
-// A cell's partial is added to F when a non-empty cell ends, as in the 27-cell loop.
+// Contacts of one particle with the sorted run [j, e) that holds up to three consecutive cells, the
+// second starting at b1 and the third at b2 (exact cell offsets: no key is read).  A cell's partial
+// is added to F when a non-empty cell ends, as in the 27-cell loop.
 // Candidates are loaded two at a time (the second index clamped into the run), so that a wave has
 // two loads in flight per lane: the loop is bound by load latency, not by the force arithmetic.
 // Three or more per round trip cost more occupancy (registers) than they hide.
-template <bool EMIT>
+// The particle itself is excluded by its sorted index i (SELF_BY_ORIG false), or, when the caller
+// walks particles in their original order, by its original index: only a candidate at distance 0
+// from a grain can be the grain itself, so only then is the candidate's original index read (a body
+// particle's own copy is excluded by the same-body rule anyway).
+template <bool EMIT, bool SELF_BY_ORIG = false>
 __device__ __forceinline__ void scan_run(const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
                                          const unsigned *__restrict__ sIdx, unsigned j, unsigned b1, unsigned b2,
                                          unsigned e, int i, int ra, f3 pa, f3 va, const DevPhys &ph, f3 &F, int &cnt,
@@ -213,7 +219,15 @@ __device__ __forceinline__ void scan_run(const float4 *__restrict__ sPos, const 
             F = add3(F, csum);
             csum = mk3(0.f, 0.f, 0.f);
         }
-        if (!candidate(Q, pa, ra, (int)jj == i, ph)) return;
+        if (SELF_BY_ORIG) {
+            const f3 rel = sub3(mk3(Q.x, Q.y, Q.z), pa);
+            const float d2 = dot3(rel, rel);
+            bool ok = !(d2 > ph.cd2Reject) & pair_allowed(ra, __float_as_int(Q.w));
+            if (ok && ra < 0 && d2 == 0.0f) ok = sIdx[jj] != orig;
+            if (!ok) return;
+        } else {
+            if (!candidate(Q, pa, ra, (int)jj == i, ph)) return;
+        }
         if (EMIT) {
             if (in_contact(pa, xyz(Q), ph)) {
                 if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = (int)sIdx[jj];
@@ -231,6 +245,49 @@ __device__ __forceinline__ void scan_run(const float4 *__restrict__ sPos, const 
         if (j + 1u < e) visit(j + 1u, Q1);
     }
     if (e > j0) F = add3(F, csum);
+}
+
+// Σ of the pair forces on one particle over its 27 neighbour cells with the exact cell table
+// (segStart = cellStart, numCells + 1 entries): for each of the 9 (dz, dy) rows in z, y order the
+// three x-cells are one contiguous run [segStart[h0], segStart[h0 + 3]) unless x wraps, visited in
+// the order x-1, x, x+1.  The walls are not included.
+template <bool EMIT, bool SELF_BY_ORIG>
+__device__ __forceinline__ f3 dem_rows_exact(const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
+                                             const unsigned *__restrict__ sIdx, const unsigned *__restrict__ segStart,
+                                             const SceneGeom &g, int i, int ra, f3 pa, f3 va, const DevPhys &ph,
+                                             int &cnt, unsigned orig, const DemOut &o) {
+    int gx, gy, gz;
+    grid_pos(pa, g, gx, gy, gz);
+    const unsigned x0 = (unsigned)(gx - 1) & g.gridMask[0], x1 = (unsigned)gx & g.gridMask[0],
+                   x2 = (unsigned)(gx + 1) & g.gridMask[0];
+    const bool contiguous = (x0 + 1u == x1) && (x1 + 1u == x2);
+    f3 F = mk3(0.f, 0.f, 0.f);
+    auto row_of = [&](int r) {
+        const unsigned uz = (unsigned)(gz + (r / 3) - 1) & g.gridMask[2];
+        const unsigned uy = (unsigned)(gy + (r % 3) - 1) & g.gridMask[1];
+        return g.cellBase + uz * g.gxy + uy * g.gx;
+    };
+    for (int r = 0; r < 9; ++r) {
+        const unsigned row = row_of(r);
+        if (contiguous) {
+            // the row's four offsets in one 16-byte load (4-byte aligned): the kernel issues fewer
+            // vector memory instructions, which bound it more than bytes do
+            uint4 bnd;
+            __builtin_memcpy(&bnd, segStart + row + x0, sizeof(bnd));
+            scan_run<EMIT, SELF_BY_ORIG>(sPos, sVel, sIdx, bnd.x, bnd.y, bnd.z, bnd.w, i, ra, pa, va, ph, F, cnt,
+                                         orig, o);
+        } else {
+            const unsigned xs[3] = {x0, x1, x2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const unsigned h = row + xs[k];
+                const unsigned e = segStart[h + 1];
+                scan_run<EMIT, SELF_BY_ORIG>(sPos, sVel, sIdx, segStart[h], e, e, e, i, ra, pa, va, ph, F, cnt,
+                                             orig, o);
+            }
+        }
+    }
+    return F;
 }
 
 // Engine DEM over the segment index: for each of the 9 (dz, dy) rows the three x-cells are
@@ -323,6 +380,58 @@ __global__ void k_dem_grid_seg(const float4 *__restrict__ sPos, const float4 *__
     }
 }
 
+// Grid DEM fused with the per-body reduction and apply (engine, exact cell table, no static set):
+// one launch in place of k_dem_grid_seg + k_reduce_apply, with no per-particle force round trip
+// through memory.  Waves [0, B) take one body each: lane l walks the body's particles l, l+64, ... in
+// index order, computes each particle's force exactly as k_dem_grid_seg does (same cells, same order,
+// same per-cell partials, walls last) and accumulates force and torque relPos × F in the order of
+// k_reduce_apply; the butterfly and the apply follow.  The remaining waves take 64 grains each (grain
+// indices in original order) and update their velocities.  A wave's lanes are then particles of one
+// kind, close in space (one body, or grains in index order), so their candidate loops have similar
+// lengths, where a wave of consecutive sorted particles mixes body particles and grains.
+__global__ __launch_bounds__(256) void k_dem_fused(const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
+                                                   const unsigned *__restrict__ sIdx,
+                                                   const unsigned *__restrict__ segStart, FusedDemIn in, BodyBufs b,
+                                                   int B, DevPhys ph, SceneView sv, int sceneInVelW) {
+    const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int wave = lb * 4 + (int)(threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63);
+    const DemOut none{};
+    int cnt = 0;
+    if (wave < B) {
+        const int body = wave;
+        const int st = b.pstart[body], count = b.pcount[body];
+        f3 aF = mk3(0.f, 0.f, 0.f), aT = mk3(0.f, 0.f, 0.f);
+        for (int j = reduce_first(body, ph) + lane; j < count; j += 64) {
+            const size_t o = (size_t)st + j;
+            const float4 P = in.rec[2 * o], V = in.rec[2 * o + 1];
+            const int ra = __float_as_int(P.w);
+            SceneGeom g;
+            load_geom(sv, sceneInVelW ? __float_as_int(V.w) : 0, g);
+            const f3 pa = xyz(P), va = xyz(V);
+            f3 F = dem_rows_exact<false, true>(sPos, sVel, sIdx, segStart, g, -1, ra, pa, va, ph, cnt, (unsigned)o,
+                                               none);
+            F = add3(F, wall_force(pa, va, ph, g));
+            aF = add3(aF, F);
+            aT = add3(aT, cross3(xyz(in.relPos[o]), F));
+        }
+        wave_reduce_apply(aF, aT, lane, body, b, ph);
+        return;
+    }
+    const long long k = (long long)(wave - B) * 64 + lane;
+    if (k >= in.numGrains) return;
+    const unsigned o = (unsigned)in.grainIdx[k];
+    const float4 P = in.rec[2 * (size_t)o], V = in.rec[2 * (size_t)o + 1];
+    const int ra = __float_as_int(P.w);
+    SceneGeom g;
+    load_geom(sv, sceneInVelW ? __float_as_int(V.w) : 0, g);
+    const f3 pa = xyz(P), va = xyz(V);
+    f3 F = dem_rows_exact<false, true>(sPos, sVel, sIdx, segStart, g, -1, ra, pa, va, ph, cnt, o, none);
+    if (!(ph.flags & kGrainClamp)) F = add3(F, wall_force(pa, va, ph, g));
+    const float fs = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
+    in.vel[o] = make_float4(va.x + F.x * fs, va.y + F.y * fs, va.z + F.z * fs, 0.0f);
+}
+
 }  // namespace
 
 void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, const float4 *pos, const float4 *vel,
@@ -394,6 +503,16 @@ void launch_dem_grid(const float4 *sortedPos, const float4 *sortedVel, const uns
     else
         hipLaunchKernelGGL(k_dem_grid<false>, grid, dim3(block), 0, st, sortedPos, sortedVel, sortedIdx, cellStart,
                            cellEnd, n, ph, sv, sceneInVelW ? 1 : 0, o);
+}
+
+void launch_dem_fused(const float4 *sortedPos, const float4 *sortedVel, const unsigned *sortedIdx,
+                      const unsigned *segStart, const FusedDemIn &in, const BodyBufs &b, int numBodies,
+                      const DevPhys &ph, const SceneView &sv, bool sceneInVelW, hipStream_t st) {
+    const long long waves = (long long)numBodies + (in.numGrains + 63) / 64;
+    if (waves <= 0) return;
+    const long long blocks = (waves + 3) / 4;
+    hipLaunchKernelGGL(k_dem_fused, dim3((unsigned)blocks), dim3(256), 0, st, sortedPos, sortedVel, sortedIdx, segStart,
+                       in, b, numBodies, ph, sv, sceneInVelW ? 1 : 0);
 }
 
 }  // namespace arsim

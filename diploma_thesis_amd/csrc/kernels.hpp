@@ -95,6 +95,8 @@ void launch_check_finite(const float4 *a, int n, int *flag, hipStream_t st);
 // dst[i] = src[i] + delta for src[i] >= 0, else src[i] (rebases body indices when scenes move).
 void launch_copy_offset(int *dst, const int *src, int n, int delta, hipStream_t st);
 void launch_fill_u16(uint16_t *p, uint16_t value, size_t count, hipStream_t st);
+// dst[r.x + k] = r.y + k for every range r = (first destination, first value, count).
+void launch_iota_ranges(const int4 *ranges, int numRanges, int *dst, hipStream_t st);
 
 // Scene carry-over between two engines' buffers (a rebuild for a new scene set): destination
 // scenes [dstP, dstP + n) / [dstB, dstB + b) come from [srcP, ...) / [srcB, ...), ranges in
@@ -167,6 +169,19 @@ void launch_fill_long(unsigned *table, unsigned long long tableSize, const LongF
 void launch_dem_grid_seg(const float4 *sortedPos, const float4 *sortedVel, const unsigned *sortedKeys,
                          const unsigned *sortedIdx, const unsigned *segStart, int segBits, int n, const DevPhys &ph,
                          const SceneView &sv, bool sceneInVelW, const DemOut &o, int block, hipStream_t st);
+// Grid DEM fused with the per-body reduction and apply (exact cell table, no static collider set):
+// bodies take one wave each, grains 64 per wave.  rec is the pose kernel's gather record in the
+// original order (pos.xyz + rbIndex bits, vel.xyz + scene id); grainIdx lists every grain's index.
+struct FusedDemIn {
+    const float4 *rec;
+    const float4 *relPos;
+    const int *grainIdx;
+    int numGrains;
+    float4 *vel;                  // grains: velocity after the contact impulse
+};
+void launch_dem_fused(const float4 *sortedPos, const float4 *sortedVel, const unsigned *sortedIdx,
+                      const unsigned *segStart, const FusedDemIn &in, const BodyBufs &b, int numBodies,
+                      const DevPhys &ph, const SceneView &sv, bool sceneInVelW, hipStream_t st);
 
 // ---------------------------------------------------------------- LBVH path (lbvh.hip)
 struct Lbvh {

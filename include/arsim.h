@@ -402,6 +402,10 @@ typedef struct ArSimTuning {
     int sortItemsPerThread;           /* onesweep items per thread (4..16) */
     int sortRadixBits;                /* onesweep digit width (4..8) */
     int useGraph;                     /* capture the step in a hipGraph and replay it */
+    int fuseReduce;                   /* grid method: one kernel for the contact forces and the
+                                         per-body reduction (bodies one wave each, grains 64 per
+                                         wave, original order); same results, measured slower than
+                                         the sorted-order contact kernel plus the reduction */
 } ArSimTuning;
 
 typedef struct arsim_engine arsim_engine;

@@ -263,3 +263,24 @@ def test_benchmark_batch_one_step_bit_exact():
         st.step(1)
         assert_state_equal(got[k], st, f"batch scene {k}")
 
+
+def test_segmented_cell_table_past_the_exact_limit():
+    """Two scenes on 1024^3 grids (2^31 cells in all, past the 2^30 cells of an exact table): the table
+    holds segments of two cells, the DEM scans the sorted keys of each row, and every scene's state
+    after two graph-replayed steps equals the oracle's step of that scene alone."""
+    print("engine: segmented cell table, 2 x 1024^3 cells", flush=True)
+    mix = [scenes.mix_scene(n, 300 + n, capi.METHOD_GRID) for n in (5000, 7000)]
+    for sc in mix:
+        p = sc.params
+        p.gridSize.x = p.gridSize.y = p.gridSize.z = 1024
+        p.numCells = 1024 ** 3
+    with Engine(mix, use_graph=True) as e:
+        e.step(2)
+        e.sync()
+        got = [e.download(k) for k in range(len(mix))]
+        stages = [name for name, _ in e.profile()]
+    assert "dem_grid" in stages and "reduce_apply" in stages, stages   # the unfused kernels
+    for k, sc in enumerate(mix):
+        st = oracle.State(sc, np.float32)
+        st.step(2)
+        assert_state_equal(got[k], st, f"segmented table, scene {k}")

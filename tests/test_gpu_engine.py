@@ -186,3 +186,21 @@ def test_rebuild_carry_over_matches_alone(method):
             want = e.download()
         for key in ("pos", "vel", "rb_pos", "rb_vel", "rb_ang", "rb_quat", "rb_iw"):
             assert np.array_equal(got[k][key], want[key]), (k, key)
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_fused_contact_reduce_kernel_bit_exact(batched):
+    """The opt-in fused contact + reduction kernel (bodies one wave each in index order, grains 64 per
+    wave) gives the oracle's state bit for bit, for one scene and for a batch, over graph replays."""
+    print(f"engine: fused contact + reduction kernel, batched={batched}", flush=True)
+    mix = [scenes.mix_scene(n, 400 + n, capi.METHOD_GRID) for n in ((5000, 12000, 3000) if batched else (20000,))]
+    with Engine(mix, use_graph=True, fuse_reduce=True) as e:
+        e.step(3)
+        e.sync()
+        got = [e.download(k) for k in range(len(mix))]
+        stages = [name for name, _ in e.profile()]
+    assert stages[-1] == "dem_reduce", stages
+    for k, sc in enumerate(mix):
+        st = oracle.State(sc, np.float32)
+        st.step(3)
+        assert_state_equal(got[k], st, f"fused kernel, scene {k}")
