@@ -21,22 +21,23 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# algorithmic bytes per particle (or per body) of each stage, compulsory traffic in the reference
-# layout with neighbour re-reads assumed cached
-# Algorithmic HBM bytes per launch = per_particle * N + per_cell * C (compulsory traffic of the reference
-# layout, neighbour re-reads assumed cached; C = grid cells)
+# Algorithmic HBM bytes per launch of each stage: compulsory traffic in the reference layout with
+# neighbour re-reads assumed cached, as (per particle, per grid cell, per body particle, per body).
+# A fused kernel counts the sum of its stages.
 ALGO_BYTES = {
-    "dem_grid": (120.0, 8.0),           # sorted pos+vel 32, index 4, rbIndex 4, relativePos 16, pForce/pTorque RMW 64;
-                                        # cellStart + cellEnd 8 per cell
-    "dem_lbvh": (120.0, 0.0),
-    "reorder_cellstart": (72.0, 4.0),   # hash 4, index 4, gathered pos+vel 32, sorted pos+vel 32; cellStart clear 4 per
-                                        # cell (the 8 B per occupied cell are left out)
-    "pose_key": (68.0 + 24.0, 0.0),     # body integrate + pose 68, hash 24
-    "sort": (4.0 + 16.0 * 3, 0.0),      # onesweep, u32 key + u32 value, 3 digit passes over the 24-bit grid key
-    "reduce_apply": (32.0, 0.0),
+    "dem_grid": (120.0, 8.0, 0.0, 0.0),     # sorted pos+vel 32, index 4, rbIndex 4, relativePos 16, pForce/pTorque
+                                            # RMW 64; cellStart + cellEnd 8 per cell
+    "dem_lbvh": (120.0, 0.0, 0.0, 0.0),
+    "reorder_cellstart": (72.0, 4.0, 0.0, 0.0),   # hash 4, index 4, gathered pos+vel 32, sorted pos+vel 32; cellStart
+                                                  # clear 4 per cell (the 8 B per occupied cell are left out)
+    "pose_key": (68.0 + 24.0, 0.0, 0.0, 0.0),     # body integrate + pose 68, hash 24
+    "sort": (4.0 + 16.0 * 3, 0.0, 0.0, 0.0),      # onesweep, u32 key + u32 value, 3 digit passes over the 24-bit key
+    "reduce_apply": (0.0, 0.0, 32.0, 240.0),      # pForce + relativePos of each body particle; body state
     # bucket sort: the top digit ranked per tile inside pose_key, the two remaining passes fused with the
     # reorder and the cell table
-    "bucket_sort_reorder": (72.0 + 2 * 16.0, 4.0),
+    "bucket_sort_reorder": (72.0 + 2 * 16.0, 4.0, 0.0, 0.0),
+    # the grid DEM fused with the per-body reduction and apply
+    "dem_reduce": (120.0, 8.0, 32.0, 240.0),
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -51,13 +52,20 @@ def parse():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-scenes", type=int, default=512)
     ap.add_argument("--batched-steps", type=int, default=100)
-    ap.add_argument("--cpu-sample-steps", type=int, default=0, help="0: choose ~15 s of CPU work")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=500.0,
+                    help="Poisson arrival rate of the serving leg (scenes/s); 10x the specified 50/s by default, so "
+                         "that the leg measures serving capacity rather than the arrival rate")
     ap.add_argument("--quick", action="store_true",
                     help="timed steps only (no latency / stage loops, no CPU baseline, no batched run): for profilers")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 256 MiB device copy kernel (3 launches) to calibrate HBM byte counters, then exit")
     ap.add_argument("--batched-only", action="store_true",
                     help="only step the 512-scene batch (default tuning, no autotune): for profilers")
+    ap.add_argument("--traversal-stats", action="store_true",
+                    help="LBVH packet traversal statistics of the 2^20-particle scene (nodes visited per wave, leaf "
+                         "tests per particle, stack depth) from the counting build libarsim_traversal_stats.so")
     ap.add_argument("--scene", choices=["small", "pile", "sis"], default=None,
                     help="time another plain scene instead of the headline one: 'small' = 256 cubes (16,384 "
                          "particles, 64^3 grid), 'pile' = the dense ellipsoid pile (~4.2 M particles), 'sis' = the "
@@ -78,14 +86,7 @@ def other_scene(args):
         sc = scenes.config0(method) if args.scene == "small" else scenes.config2(method)
     cpu = None
     if floor is not None and not args.no_cpu_baseline:
-        import oracle
-        st = oracle.State(sc, np.float32)
-        st.set_static(*floor)
-        st.step(1)
-        t0 = time.perf_counter()
-        st.step(5)
-        cpu = {"value": sc.n * 5 / (time.perf_counter() - t0), "unit": "particle-updates/s", "cores": 1,
-               "kind": "port", "sample": "5 steps after 1 untimed step, oracle FP32 serial C++ (g++ -O2)"}
+        cpu = cpu_leg(sc, 5, 10.0, "the stacked-cubes scene", static=floor)
     with Engine(sc, use_graph=True) as e:
         if floor is not None:
             e.set_static(*floor)
@@ -104,6 +105,32 @@ def other_scene(args):
                       "steps": args.steps, "updates_per_s": sc.n * args.steps / sec,
                       "ms_per_step": sec * 1e3 / args.steps, "stage_ms": stage_ms, "finite": finite,
                       "cpu_baseline": cpu}))
+
+
+def traversal_stats(args):
+    """Counters of the LBVH packet traversal over `args.steps` steps of the 2^20-particle scene."""
+    os.environ["ARSIM_LIB"] = os.path.join(ROOT, "diploma_thesis_amd", "libarsim_traversal_stats.so")
+    from diploma_thesis_amd import Engine, capi, scenes
+    fn = capi.lib.arsim_debug_traversal_stats
+    fn.restype, fn.argtypes = C.c_int, [C.c_void_p, C.c_int]
+    method = {"grid": capi.METHOD_LBVH30, "lbvh30": capi.METHOD_LBVH30, "lbvh63": capi.METHOD_LBVH63}[args.method]
+    sc = scenes.config1(method)
+    out = np.zeros(8, np.uint64)
+    with Engine(sc, use_graph=False) as e:
+        e.step(1)
+        e.sync()
+        capi.check(fn(None, 1), "reset")
+        e.step(args.steps)
+        e.sync()
+        capi.check(fn(capi.ptr(out), 1), "stats")
+    waves, visits, lanes, leaf_tests, interest, max_sp, done, contacts = (int(x) for x in out)
+    print(json.dumps({"traversal_stats": True, "method": "lbvh30" if method == capi.METHOD_LBVH30 else "lbvh63",
+                      "particles": sc.n, "steps": args.steps, "waves": waves,
+                      "nodes_visited_per_wave": visits / max(waves, 1),
+                      "leaf_tests_per_particle": leaf_tests / max(lanes, 1),
+                      "contacts_per_particle": contacts / max(lanes, 1),
+                      "packet_efficiency": interest / max(visits * 64, 1),
+                      "max_stack_depth": max_sp, "completed_lanes": done, "valid_lanes": lanes}))
 
 
 def batched_only(args):
@@ -147,6 +174,53 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_info():
+    """Core count, the cores this process may use and the CPU model of the host the baseline runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)), "cpu_model": model}
+
+
+class one_core:
+    """Pins this process to one of its usable cores for the serial CPU baseline, then restores the set."""
+
+    def __enter__(self):
+        self.prev = os.sched_getaffinity(0)
+        self.core = min(self.prev)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.prev)
+
+
+def cpu_leg(sc, min_steps, target_s, what, static=None):
+    """The oracle's serial FP32 step on scene sc, pinned to one core: one untimed step, then about
+    target_s seconds of steps (at least min_steps)."""
+    import oracle
+    st = oracle.State(sc, np.float32)
+    if static is not None:
+        st.set_static(*static)
+    with one_core() as pin:
+        t0 = time.perf_counter()
+        st.step(1)
+        one = time.perf_counter() - t0
+        k = max(min_steps, min(200, int(target_s / max(one, 1e-3))))
+        t0 = time.perf_counter()
+        st.step(k)
+        sec = time.perf_counter() - t0
+    return {"value": sc.n * k / sec, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{k} steps of {what} after 1 untimed step, oracle FP32 serial C++ (g++ -O2), {sec:.1f} s",
+            "pinned_core": pin.core, **host_info()}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -158,6 +232,9 @@ def main():
         return
     if args.batched_only:
         batched_only(args)
+        return
+    if args.traversal_stats:
+        traversal_stats(args)
         return
     if args.scene:
         other_scene(args)
@@ -212,9 +289,11 @@ def main():
     dom = max((k for k in stage_ms if not k.startswith("sort")), key=lambda k: stage_ms[k])
     cells = int(sc.params.numCells)
 
+    body_particles = int((sc.rb >= 0).sum())
+
     def algo_bytes(stage):
-        per_p, per_c = ALGO_BYTES.get(stage, (0.0, 0.0))
-        return per_p * n + per_c * cells
+        per_p, per_c, per_bp, per_b = ALGO_BYTES.get(stage, (0.0, 0.0, 0.0, 0.0))
+        return per_p * n + per_c * cells + per_bp * body_particles + per_b * sc.b
 
     achieved = algo_bytes(dom) / (stage_ms[dom] * 1e-3) / 1e9
     traffic = None
@@ -233,19 +312,8 @@ def main():
     # ---- CPU baseline: the oracle's serial FP32 restatement of the same step on the same scene
     cpu = None
     if not args.no_cpu_baseline:
-        import oracle
-        st = oracle.State(sc, np.float32)
-        t0 = time.perf_counter()
-        st.step(1)
-        one = time.perf_counter() - t0
-        k = args.cpu_sample_steps or max(1, min(20, int(15.0 / max(one, 1e-3))))
-        t0 = time.perf_counter()
-        st.step(k)
-        sec = time.perf_counter() - t0
-        cpu = {"value": n * k / sec, "unit": "particle-updates/s", "cores": 1, "kind": "port",
-               "sample": f"{k} steps of the 2^20-particle scene after 1 untimed step, oracle FP32 serial C++ "
-                         f"(g++ -O2), {sec:.1f} s"}
-        log(f"cpu baseline: {cpu['value']:.3e} particle-updates/s over {k} steps")
+        cpu = cpu_leg(sc, 1, 15.0, "the 2^20-particle scene")
+        log(f"cpu baseline: {cpu['value']:.3e} particle-updates/s ({cpu['sample']})")
 
     # ---- batched form vs the plain path looping over scenes
     batched = None
@@ -292,6 +360,25 @@ def main():
                    "tuning": {"dem_block": best[0], "sort_items_per_thread": best[1], "sort_radix_bits": best[2]}}
         log(f"batched: {bv:.3e} vs plain loop {pv:.3e} particle-updates/s ({bv / pv:.1f}x)")
 
+    # ---- served form: continuous batching vs one scene at a time on the same request stream
+    serving = None
+    if not args.no_serving:
+        from diploma_thesis_amd import serve_bench
+        log(f"serving: {args.serve_requests} requests at {args.serve_rate}/s, batched and one at a time")
+        sv = serve_bench.compare(args.serve_requests, args.serve_rate)
+        b, o = sv["batched"], sv["one_at_a_time"]
+        serving = {"workload": sv["workload"], "throughput_ratio": sv["throughput_ratio"],
+                   "batched_updates_per_s": b["updatesPerSecond"], "one_at_a_time_updates_per_s": o["updatesPerSecond"],
+                   "step_latency_ms_p50": b["stepLatencyP50Ms"], "step_latency_ms_p99": b["stepLatencyP99Ms"],
+                   "completion_latency_ms_p50": b["completionLatencyP50Ms"],
+                   "completion_latency_ms_p99": b["completionLatencyP99Ms"],
+                   "one_at_a_time_completion_latency_ms_p50": o["completionLatencyP50Ms"],
+                   "one_at_a_time_completion_latency_ms_p99": o["completionLatencyP99Ms"],
+                   "queue_wait_ms_p50": b["queueWaitP50Ms"], "max_resident": b["maxResidentSeen"],
+                   "completed": b["completed"], "rejected_for_budget": b["rejectedForBudget"]}
+        log(f"serving: {b['updatesPerSecond']:.3e} vs {o['updatesPerSecond']:.3e} particle-updates/s "
+            f"({sv['throughput_ratio']:.2f}x)")
+
     line = {
         "metric": "particle-updates/s batched vs plain on 1 GPU; p50/p99 per-scene step latency",
         "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -306,6 +393,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line))
 

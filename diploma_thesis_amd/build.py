@@ -69,5 +69,17 @@ def build(verbose=False, out=LIB, build_dir=BUILD, defines=()):
     return out
 
 
+# Counting build of the LBVH packet traversal (nodes visited, leaf tests, stack depth), loaded through
+# ARSIM_LIB by `bench.py --traversal-stats`; never the product library.
+STATS_LIB = os.path.join(HERE, "libarsim_traversal_stats.so")
+
+
+def build_traversal_stats(verbose=False):
+    return build(verbose=verbose, out=STATS_LIB, build_dir=os.path.join(HERE, "_build_stats"),
+                 defines=("ARSIM_TRAVERSAL_STATS",))
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    if "--traversal-stats" in sys.argv:
+        print(build_traversal_stats(verbose=True))

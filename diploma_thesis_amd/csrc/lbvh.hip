@@ -16,6 +16,14 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + k;
 }
 
+#ifdef ARSIM_TRAVERSAL_STATS
+// Counting build only: totals of the packet traversal, accumulated per wave.
+//   [0] waves  [1] internal nodes visited (wave-level)  [2] valid lanes  [3] lane-leaf tests
+//   [4] lanes whose box overlaps an entered node, summed over entered nodes  [5] largest stack depth
+//   [6] lanes that completed  [7] contacts found
+__device__ unsigned long long g_travStats[8];
+#endif
+
 __device__ __forceinline__ int clz_key(unsigned v) { return __clz((int)v); }
 __device__ __forceinline__ int clz_key(uint64_t v) { return __clzll((long long)v); }
 
@@ -480,6 +488,10 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
     f3 F = mk3(0.f, 0.f, 0.f);
     int cnt = 0;
     bool ovf = false;   // this lane's traversal hit the stack bound: nothing is written for it
+#ifdef ARSIM_TRAVERSAL_STATS
+    unsigned long long stVisits = 0, stLeafTests = 0, stInterest = 0, stContacts = 0;
+    int stMaxSp = 0;
+#endif
     unsigned long long pending = __ballot(valid && myRoot >= n);
     while (pending) {
         const int root = __builtin_amdgcn_readfirstlane(__shfl(myRoot, __builtin_ctzll(pending)));
@@ -493,6 +505,14 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
             const int R = __builtin_amdgcn_readfirstlane(__float_as_int(a1.w));
             const bool oL = act & (L != i) & overlap(qMin, qMax, a0, a1);
             const bool oR = act & (R != i) & overlap(qMin, qMax, b0, b1);
+#ifdef ARSIM_TRAVERSAL_STATS
+            ++stVisits;
+            stLeafTests += (L < n && oL ? 1 : 0) + (R < n && oR ? 1 : 0);
+            if (L < n && oL && in_contact(pa, xyz(cms[L]), ph)) ++stContacts;
+            if (R < n && oR && in_contact(pa, xyz(cms[R]), ph)) ++stContacts;
+            if (L >= n) stInterest += (unsigned long long)__popcll(__ballot(oL));
+            if (R >= n) stInterest += (unsigned long long)__popcll(__ballot(oR));
+#endif
             if (L < n && oL) {
                 const float4 CL = cms[L];
                 if (candidate(CL, pa, ra, false, ph)) {
@@ -539,10 +559,35 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
                     }
                     stack[sp] = R;
                     ++sp;
+#ifdef ARSIM_TRAVERSAL_STATS
+                    stMaxSp = sp > stMaxSp ? sp : stMaxSp;
+#endif
                 }
             }
         }
     }
+#ifdef ARSIM_TRAVERSAL_STATS
+    {
+        // stLeafTests / stContacts are per lane; the others are wave-uniform
+        unsigned long long lt = stLeafTests, ct = stContacts;
+        for (int off = 32; off >= 1; off >>= 1) {
+            lt += __shfl_xor(lt, off, 64);
+            ct += __shfl_xor(ct, off, 64);
+        }
+        const unsigned long long lanes = (unsigned long long)__popcll(__ballot(valid));
+        const unsigned long long done = (unsigned long long)__popcll(__ballot(valid && !ovf));
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&g_travStats[0], 1ull);
+            atomicAdd(&g_travStats[1], stVisits);
+            atomicAdd(&g_travStats[2], lanes);
+            atomicAdd(&g_travStats[3], lt);
+            atomicAdd(&g_travStats[4], stInterest);
+            atomicMax(&g_travStats[5], (unsigned long long)stMaxSp);
+            atomicAdd(&g_travStats[6], done);
+            atomicAdd(&g_travStats[7], ct);
+        }
+    }
+#endif
     if (!valid || ovf) return;
     if (EMIT) {
         o.counts[orig] = cnt;
@@ -561,6 +606,19 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
 }
 
 }  // namespace
+
+#ifdef ARSIM_TRAVERSAL_STATS
+// Counting build only: copies the packet traversal's totals (8 counters, see g_travStats) and, when
+// reset is non-zero, clears them.
+extern "C" int arsim_debug_traversal_stats(unsigned long long *out, int reset) {
+    if (out && hipMemcpyFromSymbol(out, HIP_SYMBOL(g_travStats), sizeof(g_travStats)) != hipSuccess) return 3;
+    if (reset) {
+        const unsigned long long zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (hipMemcpyToSymbol(HIP_SYMBOL(g_travStats), zero, sizeof(zero)) != hipSuccess) return 3;
+    }
+    return hipDeviceSynchronize() == hipSuccess ? 0 : 3;
+}
+#endif
 
 int lbvh_refit_block() { return kBlk; }
 int lbvh_refit_levels(int n) {

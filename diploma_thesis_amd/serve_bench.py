@@ -26,6 +26,21 @@ def run(args, one_at_a_time):
     return {k: getattr(rep, k) for k, _ in capi.ArSimServeReport._fields_}
 
 
+def compare(requests, rate, time_scale=1.0, seed=4, min_particles=2048, max_particles=262144, min_steps=50,
+            max_steps=2000, budget=200e9, max_resident=4096):
+    """One request stream served with continuous batching and one scene at a time; both reports and the
+    throughput ratio."""
+    args = argparse.Namespace(requests=requests, rate=rate, time_scale=time_scale, seed=seed,
+                              min_particles=min_particles, max_particles=max_particles, min_steps=min_steps,
+                              max_steps=max_steps, budget=budget, max_resident=max_resident)
+    out = {"workload": f"{requests} requests, Poisson {rate}/s (time scale {time_scale}), N log-U[{min_particles}, "
+                       f"{max_particles}], steps U[{min_steps}, {max_steps}], budget {budget:.3g} B"}
+    out["batched"] = run(args, False)
+    out["one_at_a_time"] = run(args, True)
+    out["throughput_ratio"] = out["batched"]["updatesPerSecond"] / out["one_at_a_time"]["updatesPerSecond"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--requests", type=int, default=2000)
