@@ -413,7 +413,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     sortWs = sortWsBuf;
     A(overflowD, 4); A(flagD, 4);
     if (method == ARSIM_METHOD_GRID) {
-        A(grainIdx, N); A(grainRanges, (size_t)S);
+        if (tu.fuseReduce) { A(grainIdx, N); A(grainRanges, (size_t)S); }
         A(segStart, (size_t)numSegs + 1);
         A(tileInfo, (size_t)256 * pose_tile_count(N));
         A(sortedPos, N); A(sortedVel, N);
@@ -535,7 +535,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     }
     bodyParticles = newBodyParticles;
     numGrains = 0;
-    if (method == ARSIM_METHOD_GRID) {
+    if (method == ARSIM_METHOD_GRID && tu.fuseReduce) {
         // the list of grain indices the fused contact kernel walks: each scene's grains follow its bodies
         std::vector<int4> gr(S);
         for (int s = 0; s < S; ++s) {
