@@ -44,6 +44,16 @@ struct DevPhys {
 constexpr unsigned kEmpty = 0xffffffffu;
 constexpr unsigned kRefUnits = 0x01u, kDropFirst = 0x02u, kGrainClamp = 0x04u, kDynamicAabb = 0x08u;
 
+// Maps hardware block b of nb to a logical block so that the blocks one XCD runs (dispatch is
+// round-robin over the 8 XCDs) cover one contiguous slice of the work; neighbouring particles then
+// share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int X = 8;
+    int q = nb / X, r = nb % X;
+    int x = b % X, k = b / X;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // ----------------------------------------------------------------------------- float3 helpers
 struct f3 { float x, y, z; };
 __device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }

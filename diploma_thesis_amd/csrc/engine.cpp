@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1133,9 +1135,49 @@ int arsim_engine_check_finite(arsim_engine *e, int *nonFinite) {
     return f ? fail(ARSIM_ERR_NAN, "non-finite state") : ARSIM_OK;
 }
 
+namespace {
+// Autotune results per shape bucket (method, flags, bucket sort or not, key bits, log2 of the particle
+// and scene counts), kept for the life of the process: an engine of a shape already tuned takes the
+// result without trials.
+struct TuneKey {
+    int method, flags, bucket, keyBits, logN, logS;
+    bool operator<(const TuneKey &o) const {
+        const int a[6] = {method, flags, bucket, keyBits, logN, logS}, b[6] = {o.method, o.flags, o.bucket, o.keyBits,
+                                                                               o.logN, o.logS};
+        for (int k = 0; k < 6; ++k)
+            if (a[k] != b[k]) return a[k] < b[k];
+        return false;
+    }
+};
+std::mutex g_tuneMutex;
+std::map<TuneKey, std::pair<ArSimTuning, float>> g_tuneCache;
+int log2_bucket(long long v) {
+    int l = 0;
+    while ((1ll << l) < v) ++l;
+    return l;
+}
+}  // namespace
+
 int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best, float *bestMs) {
     if (!e || stepsPerTrial <= 0) return fail(ARSIM_ERR_INVALID_ARG, "autotune: bad arguments");
     Engine &E = e->e;
+    const TuneKey key{E.method, (int)E.flags, E.bucketSort ? 1 : 0, E.keyBits, log2_bucket(E.N), log2_bucket(E.S)};
+    {
+        std::lock_guard<std::mutex> lock(g_tuneMutex);
+        auto it = g_tuneCache.find(key);
+        if (it != g_tuneCache.end()) {
+            ArSimTuning t = it->second.first;
+            t.useGraph = E.tu.useGraph;
+            t.fuseReduce = E.tu.fuseReduce;
+            E.tu = t;
+            E.reset_grid_tables();
+            if (best) *best = t;
+            if (bestMs) *bestMs = it->second.second;
+            return ARSIM_OK;
+        }
+    }
+    // with the bucket sort (and no contact-list sort) the sort tuning is unused: only the block size
+    const bool sortUnused = E.method == ARSIM_METHOD_GRID && E.bucketSort && !(E.flags & ARSIM_CONTACT_SIS);
     // the trials advance a scratch copy of the state: snapshot everything a step writes
     struct Snap { void *dst; const void *src; size_t bytes; };
     std::vector<Snap> snaps = {
@@ -1164,6 +1206,7 @@ int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best,
             for (int rb : rbits) {
                 if (rc) break;
                 if (!arsim::sort_tuning_supported(ip, rb)) continue;
+                if (sortUnused && (ip != orig.sortItemsPerThread || rb != orig.sortRadixBits)) continue;
                 for (auto &s : snaps)
                     if (s.dst) hipMemcpyAsync((void *)s.src, s.dst, s.bytes, hipMemcpyDeviceToDevice, E.st);
                 E.tu = ArSimTuning{bl, ip, rb, orig.useGraph, orig.fuseReduce};
@@ -1189,6 +1232,10 @@ int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best,
     E.reset_grid_tables();
     hipStreamSynchronize(E.st);
     if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lock(g_tuneMutex);
+        g_tuneCache[key] = {bestT, bestT_ms};
+    }
     if (best) *best = bestT;
     if (bestMs) *bestMs = bestT_ms;
     return ARSIM_OK;

@@ -3,20 +3,9 @@
 #include "kernels.hpp"
 #include "reduce_apply.hpp"
 
-
 namespace arsim {
 
 namespace {
-
-// Maps hardware block b to a logical block so that the blocks one XCD runs (dispatch is
-// round-robin over 8 XCDs) cover one contiguous slice of the sorted particles; neighbouring
-// particles then share that XCD's L2.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-    const int X = 8;
-    int q = nb / X, r = nb % X;
-    int x = b % X, k = b / X;
-    return x * q + (x < r ? x : r) + k;
-}
 
 // reorderDataAndFindCellStartD (Kernels/particleKernels.cu:193-258).  The previous key is read
 // straight from global memory (it is in the same or the previous cache line).  The sorted copies

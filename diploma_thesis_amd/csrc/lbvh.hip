@@ -9,13 +9,6 @@ namespace {
 constexpr int kStackDepth = 64;   // per-thread LDS stack entries (the reference's int stack[64], RB_BVHKernels.cu:129)
 constexpr int kWaveStackDepth = 64;   // per-wave LDS stack entries of the packet traversal
 
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-    const int X = 8;
-    int q = nb / X, r = nb % X;
-    int x = b % X, k = b / X;
-    return x * q + (x < r ? x : r) + k;
-}
-
 #ifdef ARSIM_TRAVERSAL_STATS
 // Counting build only: totals of the packet traversal, accumulated per wave.
 //   [0] waves  [1] internal nodes visited (wave-level)  [2] valid lanes  [3] lane-leaf tests

@@ -1,6 +1,7 @@
 """Full-step parity of the engine (scene-level update) against the oracle, determinism, hipGraph
 replay, and batch invariance (a scene gives the same bits alone and inside a batch)."""
 import ctypes as C
+import time
 
 import numpy as np
 import pytest
@@ -140,6 +141,19 @@ def test_profile_and_autotune_leave_state_consistent():
         assert ms > 0
         e.step(2)
         e.sync()
+    # a second engine of the same shape takes the cached result without trials, and its steps still
+    # equal the oracle's
+    with Engine(sc) as e2:
+        t0 = time.perf_counter()
+        best2, ms2 = e2.autotune(steps_per_trial=2)
+        assert time.perf_counter() - t0 < 0.5
+        assert best2 == best and ms2 == ms
+        e2.step(2)
+        e2.sync()
+        got = e2.download()
+    st = oracle.State(sc, np.float32)
+    st.step(2)
+    assert_state_equal(got, st, "after a cached autotune")
 
 
 def test_serve_small_stream():
