@@ -251,8 +251,10 @@ def main():
         # block size / items per thread / digit width picked by the engine's autotuner (state is restored)
         best, best_ms = eng.autotune(steps_per_trial=3)
         eng.close()
-        eng = Engine(sc, dem_block=best[0], sort_ipt=best[1], sort_radix_bits=best[2], use_graph=True)
-        tuning = {"dem_block": best[0], "sort_items_per_thread": best[1], "sort_radix_bits": best[2]}
+        eng = Engine(sc, dem_block=best[0], sort_ipt=best[1], sort_radix_bits=best[2], use_graph=True,
+                     bucket_split=best[3])
+        tuning = {"dem_block": best[0], "sort_items_per_thread": best[1], "sort_radix_bits": best[2],
+                  "bucket_split": best[3]}
     log(f"engine ready: {n} particles, {sc.b} bodies, {eng.device_bytes / 2**20:.0f} MiB on device, tuning {tuning}")
     eng.step(args.warmup)
     eng.sync()
@@ -324,7 +326,8 @@ def main():
         be = Engine(mix, use_graph=True)
         best, best_ms = be.autotune(steps_per_trial=3)
         be.close()
-        be = Engine(mix, dem_block=best[0], sort_ipt=best[1], sort_radix_bits=best[2], use_graph=True)
+        be = Engine(mix, dem_block=best[0], sort_ipt=best[1], sort_radix_bits=best[2], use_graph=True,
+                    bucket_split=best[3])
         be.step(3)
         be.sync()
         t0 = time.perf_counter()
@@ -357,7 +360,8 @@ def main():
                    "batched_updates_per_s": bv, "plain_loop_updates_per_s": pv, "speedup": bv / pv,
                    "batched_step_ms_p50": float(np.percentile(blat, 50)),
                    "batched_step_ms_p99": float(np.percentile(blat, 99)),
-                   "tuning": {"dem_block": best[0], "sort_items_per_thread": best[1], "sort_radix_bits": best[2]}}
+                   "tuning": {"dem_block": best[0], "sort_items_per_thread": best[1], "sort_radix_bits": best[2],
+                              "bucket_split": best[3]}}
         log(f"batched: {bv:.3e} vs plain loop {pv:.3e} particle-updates/s ({bv / pv:.1f}x)")
 
     # ---- served form: continuous batching vs one scene at a time on the same request stream

@@ -70,7 +70,7 @@ class ArSimSceneHost(C.Structure):
 
 class ArSimTuning(C.Structure):
     _fields_ = [("demBlock", C.c_int), ("sortItemsPerThread", C.c_int), ("sortRadixBits", C.c_int),
-                ("useGraph", C.c_int), ("fuseReduce", C.c_int)]
+                ("useGraph", C.c_int), ("fuseReduce", C.c_int), ("bucketSplit", C.c_int)]
 
 
 class ArSimGenSpec(C.Structure):

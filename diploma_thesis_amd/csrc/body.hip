@@ -77,9 +77,10 @@ __global__ void k_body_integrate(BodyBufs b, int B, DevPhys ph, ZeroJobs z, int 
 // computeGlobalAttributes (rigidBodyKernels.cu:386-421) for body particles, integrate_functor
 // (particleKernels.cu:50-129) for grains, then the broadphase key of the updated position:
 // calcHashD (particleKernels.cu:170-189) or generateMortonCodes (BVHkernels.cu:670-716).
-template <int MODE>
+template <int MODE, bool SCENE_UNIFORM = false>
 __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
-                                                 KeyOut ko, int doBodies, int doGrains, unsigned *shist);
+                                                 KeyOut ko, int doBodies, int doGrains, unsigned *shist,
+                                                 int scene = 0);
 
 // Adds 1 to an LDS histogram bin for every active lane.  Spatially ordered keys share digits across
 // a wave, so lanes are grouped by digit (8 ballots find the lanes with the same 8-bit digit) and
@@ -118,15 +119,18 @@ __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N,
     }
 }
 
-// Returns the key (0 without one); a grid key is stored only when ko.keys is set.
-template <int MODE>
+// Returns the key (0 without one); a grid key is stored only when ko.keys is set.  SCENE_UNIFORM: the
+// caller's particles all belong to scene `scene`, whose geometry is then read through scalar loads
+// from the scene table (which holds every scene, also a single one).
+template <int MODE, bool SCENE_UNIFORM>
 __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
-                                                 KeyOut ko, int doBodies, int doGrains, unsigned *shist) {
+                                                 KeyOut ko, int doBodies, int doGrains, unsigned *shist, int scene) {
     if (i >= N) return 0;
     const int rbi = p.rb ? p.rb[i] : -1;
-    const int s = p.scene ? (int)p.scene[i] : 0;
+    const int s = SCENE_UNIFORM ? scene : (p.scene ? (int)p.scene[i] : 0);
     SceneGeom g;
-    load_geom(sv, s, g);
+    if (SCENE_UNIFORM) g = sv.table[scene];
+    else load_geom(sv, s, g);
     float4 P, Vfin;
     bool haveV = false;
     if (rbi >= 0) {
@@ -218,12 +222,13 @@ __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, 
     return key;
 }
 
-// Pose and grid key of one tile of THREADS * 4 consecutive particles, the tile then ranked by the key's
-// top digit in LDS and written in digit order (stable: particle order within a digit), with the
-// tile's per-digit count and first slot in tileInfo[d * numTiles + tile] = count << 16 | first.  This
-// stands in for the first (top-digit) pass of the grid sort: the bucket kernel gathers each digit's
-// keys from every tile's run, in tile order.  Wave w owns particles [tile + 256 w, tile + 256 w + 256),
-// four per lane, so (wave, item, lane) order is particle order.
+// Pose and grid key of one tile of up to THREADS * 4 consecutive particles of one scene, the tile then
+// ranked by the key's bucket (its scene's cell range cut into 2^shift-cell ranges) in LDS and written
+// in bucket order (stable: particle order within a bucket), with the tile's per-bucket count and first
+// slot in info[infoBase + j * tiles + k] = count << 16 | first.  This stands in for the first pass of
+// the grid sort: the bucket kernel gathers each bucket's keys from its scene's tiles' runs, in tile
+// order.  Wave w owns particles [tile + 256 w, tile + 256 w + 256), four per lane, so (wave, item,
+// lane) order is particle order.
 constexpr int kTileItems = 4;   // particles per thread
 
 __device__ __forceinline__ uint64_t match8(unsigned d, bool ok) {
@@ -248,7 +253,12 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
     __shared__ unsigned sScan[WAVES];
     const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
     for (int k = t; k < WAVES * 256; k += THREADS) (&sWaveHist[0][0])[k] = 0u;
-    const long long tileBase = (long long)blockIdx.x * TILE;
+    const int scene = to.tileScene[blockIdx.x];
+    const BucketScene bs = to.scenes[scene];
+    const int kt = (int)blockIdx.x - bs.tileBase;   // the tile's place among its scene's tiles
+    const long long tileBase = (long long)bs.pStart + (long long)kt * TILE;
+    const long long sceneEnd = (long long)bs.pStart + bs.n;
+    const long long tileEnd = tileBase + TILE < sceneEnd ? tileBase + TILE : sceneEnd;
     const long long waveBase = tileBase + (long long)wave * (kTileItems * 64);
     KeyOut ko{};
     ko.grainVelDead = grainVelDead != 0;
@@ -256,7 +266,8 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
         const long long i = waveBase + k * 64 + lane;
-        keys[k] = i < N ? (unsigned)pose_key_one<kKeyGrid>((int)i, p, b, N, ph, sv, ko, 1, 1, nullptr) : 0u;
+        keys[k] = i < tileEnd ? (unsigned)pose_key_one<kKeyGrid, true>((int)i, p, b, N, ph, sv, ko, 1, 1, nullptr, scene)
+                              : 0u;
     }
     __syncthreads();
     // ranks within the wave, in particle order
@@ -264,8 +275,8 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
     unsigned rank[kTileItems];
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
-        const bool valid = waveBase + k * 64 + lane < N;
-        const unsigned d = valid ? keys[k] >> to.topShift : 0u;
+        const bool valid = waveBase + k * 64 + lane < tileEnd;
+        const unsigned d = valid ? (keys[k] - bs.cellBase) >> bs.shift : 0u;
         const uint64_t peers = match8(d, valid);
         const unsigned prior = (unsigned)__popcll(peers & lower);
         const unsigned old = sWaveHist[wave][d];
@@ -298,15 +309,17 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
     const unsigned first = before + x - cnt;
     if (t < 256) {
         sDigitStart[t] = first;
-        to.info[(size_t)t * to.numTiles + blockIdx.x] = (cnt << 16) | first;
-        if (cnt) atomicAdd(&to.hist[t], cnt);
+        if (t < bs.buckets) {
+            to.info[(size_t)bs.infoBase + (size_t)t * bs.tiles + kt] = (cnt << 16) | first;
+            if (cnt) atomicAdd(&to.hist[bs.bucketBase + t], cnt);
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
         const long long i = waveBase + k * 64 + lane;
-        if (i < N) {
-            const unsigned d = keys[k] >> to.topShift;
+        if (i < tileEnd) {
+            const unsigned d = (keys[k] - bs.cellBase) >> bs.shift;
             const unsigned pos = sDigitStart[d] + sWaveHist[wave][d] + rank[k];
             to.keys[tileBase + pos] = keys[k];
             to.vals[tileBase + pos] = (unsigned)i;
@@ -545,13 +558,6 @@ void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, con
             hipLaunchKernelGGL(k_pose_key<kKeyNone>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
             break;
     }
-}
-
-int pose_tile_keys(long long numParticles) { return numParticles <= 65536 ? 1024 : kTileKeys; }
-
-int pose_tile_count(long long numParticles) {
-    const int tk = pose_tile_keys(numParticles);
-    return (int)((numParticles + tk - 1) / tk);
 }
 
 void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,

@@ -1,16 +1,19 @@
-// bucket.hip -- the grid key's sort finished per top-digit bucket, fused with the reorder and the
-// cell table.
+// bucket.hip -- the grid key's sort finished per bucket, fused with the reorder and the cell table.
 //
-// The pose kernel leaves every tile of 4096 particles ranked by the key's top 8 bits (tile-local,
-// stable).  The keys of one top digit d, the bucket, are then the runs of d in every tile taken in
-// tile order, and the cells of that bucket are the table range [d << lowBits, (d + 1) << lowBits).
-// Two workgroups per bucket then
-//   1. sort the bucket by the low bits, stably: a bucket that fits in LDS is sorted there as packed
-//      words (low key << 16 | position in the bucket), so one 32-bit word carries key and order;
-//      a larger bucket is sorted in global memory by the same stable passes (slow, but correct);
-//   2. write the sorted keys / indices and the gathered sorted copies (the reorder), a half each;
+// The pose kernel leaves every tile of particles ranked by bucket (tile-local, stable): a bucket is
+// the range [c0, c0 + 2^shift) of one scene's cells.  The keys of a bucket are then the runs of that
+// bucket in its scene's tiles taken in tile order, and its place in the sorted order is its scene's
+// first particle plus the counts of the scene's earlier buckets (scenes' cell ranges are packed in
+// scene order).  One or two workgroups per bucket then
+//   1. sort the bucket by the key's offset from c0, stably: a bucket that fits in LDS is sorted there
+//      as packed words (offset << 16 | position in the bucket), so one 32-bit word carries key and
+//      order; a larger bucket is sorted in global memory by the same stable passes (slow, but correct);
+//   2. write the sorted keys / indices and the gathered sorted copies (the reorder), a share each;
 //   3. write their part of the cell table: table[c] = first sorted index with key >= c.
 // The result is the same as the stable LSD sort followed by the reorder, for any distribution.
+#include <algorithm>
+#include <cmath>
+
 #include "csr_fill.hpp"
 #include "kernels.hpp"
 
@@ -23,8 +26,7 @@ constexpr int kBkWaves = kBkThreads / 64;
 // Bucket size sorted in LDS: two buffers of packed words plus the 16 KiB cursor table, the wave sums
 // and the tile map stay within 80 KiB, half of a gfx950 CU's 160 KiB, so two workgroups share a CU.
 constexpr int kBkCap = 7776;
-constexpr int kBkSplit = 2;      // workgroups per bucket: each sorts it, then writes its share of the outputs
-constexpr int kBkMaxTiles = 384;  // pose tiles a bucket gathers from (2^20 keys: 256)
+constexpr int kBkMaxTiles = 384;  // tiles of one scene, which a bucket gathers from (2^20 keys: 256)
 constexpr int kBkLdsBytes = 2 * kBkCap * 4 + kBkWaves * 256 * 4 + kBkWaves * 4 + 2 * kBkMaxTiles * 4;
 static_assert(kBkLdsBytes <= 80 * 1024, "two bucket workgroups must fit in one CU's LDS");
 
@@ -154,46 +156,55 @@ __device__ __forceinline__ void block_fill(unsigned *table, unsigned long long a
 
 // eight waves per SIMD (two workgroups per CU): registers are held to that occupancy
 __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bucket_sort(
-    const unsigned *__restrict__ tKeys, const unsigned *__restrict__ tVals, const unsigned *__restrict__ tInfo,
-    int numTiles, int tileKeys, unsigned *__restrict__ kOut, unsigned *__restrict__ vOut, const unsigned *__restrict__ hist,
-    int lowBits, const float4 *__restrict__ rec, float4 *__restrict__ sPos, float4 *__restrict__ sVel,
+    TileKeyOut to, int split, unsigned *__restrict__ kOut, unsigned *__restrict__ vOut,
+    const float4 *__restrict__ rec, float4 *__restrict__ sPos, float4 *__restrict__ sVel,
     unsigned *__restrict__ table, unsigned numCells) {
     __shared__ unsigned sA[kBkCap], sB[kBkCap];
     __shared__ unsigned sCur[kBkWaves][256];
     __shared__ unsigned sWave[kBkWaves];
     __shared__ unsigned sOff[kBkMaxTiles], sSrc[kBkMaxTiles];   // tile map: bucket offset and source slot
+    const unsigned *__restrict__ tKeys = to.keys;
+    const unsigned *__restrict__ tVals = to.vals;
     const int t = threadIdx.x;
-    const int d = blockIdx.x / kBkSplit, part = blockIdx.x % kBkSplit;
-    const unsigned lowMask = (1u << lowBits) - 1u;
-    const unsigned start = block_sum(t < d && t < 256 ? hist[t] : 0u, sWave);
-    const int cnt = d < 256 ? (int)hist[d] : 0;
+    // consecutive buckets (neighbouring cells, and one scene's particles) run on one XCD
+    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int g = blk / split, part = blk % split;
+    const BucketScene bs = to.scenes[to.bucketScene[g]];
+    const int j = g - bs.bucketBase;
+    const int lowBits = bs.shift;
+    const unsigned *hist = to.hist + bs.bucketBase;
+    const unsigned start = bs.pStart + block_sum(t < j ? hist[t] : 0u, sWave);
+    const int cnt = (int)hist[j];
     const bool inLds = cnt <= kBkCap;
     // a bucket too large for LDS is sorted in global memory by its first workgroup alone
     if (!inLds && part != 0) return;
-    const int nparts = inLds ? kBkSplit : 1;
+    const int nparts = inLds ? split : 1;
     const int i0 = (int)((long long)cnt * part / nparts), i1 = (int)((long long)cnt * (part + 1) / nparts);
-    // the bucket's cells [c0, c1): table[c] = start + (number of bucket keys below c)
-    const unsigned long long c0 = (unsigned long long)d << lowBits;
-    const unsigned long long cEnd = (unsigned long long)numCells + 1ull;
-    const unsigned long long c1 = c0 + (1ull << lowBits) < cEnd ? c0 + (1ull << lowBits) : cEnd;
+    // the bucket's cells [c0, c1): table[c] = start + (number of bucket keys below c); the last bucket
+    // also writes the table's closing entry, table[numCells]
+    const unsigned long long c0 = (unsigned long long)bs.cellBase + ((unsigned long long)j << lowBits);
+    const unsigned long long sceneEnd = (unsigned long long)bs.cellBase + bs.cells;
+    const unsigned long long c1 = g == to.numBuckets - 1 ? sceneEnd + 1ull
+                                  : c0 + (1ull << lowBits) < sceneEnd ? c0 + (1ull << lowBits) : sceneEnd;
     if (cnt == 0) {
         const unsigned long long n = c1 - c0;
-        block_fill(table, c0 + n * part / kBkSplit, c0 + n * (part + 1) / kBkSplit, start);
+        block_fill(table, c0 + n * part / split, c0 + n * (part + 1) / split, start);
         return;
     }
-    // The bucket's keys are the runs of digit d in every tile, in tile order: tile k's run starts at
-    // bucket position sOff[k] and at slot sSrc[k] of the tile-ordered arrays.
+    // The bucket's keys are the runs of bucket j in the scene's tiles, in tile order: tile k's run
+    // starts at bucket position sOff[k] and at slot sSrc[k] of the tile-ordered arrays.
+    const int numTiles = bs.tiles;
     {
         unsigned c = 0u, f = 0u;
         if (t < numTiles) {
-            const unsigned w = tInfo[(size_t)d * numTiles + t];
+            const unsigned w = to.info[(size_t)bs.infoBase + (size_t)j * numTiles + t];
             c = w >> 16;
             f = w & 0xffffu;
         }
         const unsigned off = block_excl_scan(c, sWave);
         if (t < numTiles) {
             sOff[t] = off;
-            sSrc[t] = (unsigned)t * (unsigned)tileKeys + f;
+            sSrc[t] = bs.pStart + (unsigned)t * (unsigned)to.tileKeys + f;
         }
         __syncthreads();
     }
@@ -207,7 +218,7 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
         }
         return sSrc[lo] + (i - sOff[lo]);
     };
-    // low(i) = low bits of the i-th sorted key of the bucket
+    // low(i) = offset from c0 of the i-th sorted key of the bucket
     auto fill_table = [&](auto low) {
         // head: cells up to the first key; tail: cells past the last key
         const unsigned kFirst = low(0), kLast = low(cnt - 1);
@@ -225,8 +236,9 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
             wave_fill(table, has, a, b, start + (unsigned)i);
         }
     };
+    const unsigned base = (unsigned)c0;
     if (inLds) {
-        for (int i = t; i < cnt; i += kBkThreads) sA[i] = ((tKeys[slot((unsigned)i)] & lowMask) << 16) | (unsigned)i;
+        for (int i = t; i < cnt; i += kBkThreads) sA[i] = ((tKeys[slot((unsigned)i)] - base) << 16) | (unsigned)i;
         __syncthreads();
         unsigned *src = sA, *dst = sB;
         for (int sh = 0; sh < lowBits; sh += 8) {
@@ -236,7 +248,6 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
             dst = x;
         }
         const unsigned *sorted = src;   // packed words in sorted order
-        const unsigned topBits = (unsigned)d << lowBits;
         // the gathers of the first two rounds of positions are issued before the table is filled, so
         // that their latency overlaps the table's stores; a lane past the part's end loads sorted
         // position 0 (the bucket is not empty) and stores nothing, so the loads need no branch
@@ -248,13 +259,13 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
         const float4 pb = rec[2 * (size_t)vb], qb = rec[2 * (size_t)vb + 1];
         fill_table([&](int i) -> unsigned { return sorted[i] >> 16; });
         if (ha) {
-            kOut[start + ia] = topBits | (wa >> 16);
+            kOut[start + ia] = base + (wa >> 16);
             vOut[start + ia] = va;
             sPos[start + ia] = pa;
             sVel[start + ia] = qa;
         }
         if (hb) {
-            kOut[start + ib] = topBits | (wb >> 16);
+            kOut[start + ib] = base + (wb >> 16);
             vOut[start + ib] = vb;
             sPos[start + ib] = pb;
             sVel[start + ib] = qb;
@@ -262,18 +273,19 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
         for (int i = ib + kBkThreads; i < i1; i += kBkThreads) {
             const unsigned w = sorted[i];
             const unsigned val = tVals[slot(w & 0xffffu)];
-            kOut[start + i] = topBits | (w >> 16);
+            kOut[start + i] = base + (w >> 16);
             vOut[start + i] = val;
             sPos[start + i] = rec[2 * (size_t)val];
             sVel[start + i] = rec[2 * (size_t)val + 1];
         }
     } else {
         // a bucket too large for LDS, sorted in global memory by its first workgroup alone: its runs are
-        // gathered into (kOut, vOut) at [start, start + cnt), and the sorted-copy arrays' range (16 B
-        // per key) serves as the second buffer of the passes until the copies are written
+        // gathered into (kOut, vOut) at [start, start + cnt) as offsets from c0, and the sorted-copy
+        // arrays' range (16 B per key) serves as the second buffer of the passes until the copies are
+        // written; the keys get c0 back at the end
         for (int i = t; i < cnt; i += kBkThreads) {
             const unsigned src = slot((unsigned)i);
-            kOut[start + i] = tKeys[src];
+            kOut[start + i] = tKeys[src] - base;
             vOut[start + i] = tVals[src];
         }
         __syncthreads();
@@ -297,27 +309,98 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
             sVel[start + i] = rec[2 * (size_t)val + 1];
         }
         __syncthreads();
-        fill_table([&](int i) -> unsigned { return kOut[start + i] & lowMask; });
+        fill_table([&](int i) -> unsigned { return kOut[start + i]; });
+        __syncthreads();
+        for (int i = t; i < cnt; i += kBkThreads) kOut[start + i] += base;
     }
 }
 
 }  // namespace
 
-bool bucket_sort_fits(long long n, int keyBits, unsigned long long numCells) {
-    if (keyBits < 9 || keyBits > 24) return false;
-    if (pose_tile_count(n) > kBkMaxTiles) return false;
-    const int lowBits = keyBits - 8;
-    const unsigned long long buckets = (numCells >> lowBits) + 1ull;
-    // a bucket averages n / 256 keys; the LDS path takes up to kBkCap, larger ones go to global memory
-    return (double)n / (double)(buckets < 256ull ? buckets : 256ull) <= 0.75 * kBkCap;
+bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *out) {
+    const size_t S = shapes.size();
+    long long nTot = 0, tiles4k = 0;
+    for (const auto &sh : shapes) {
+        nTot += sh.particles;
+        tiles4k += (sh.particles + kTileKeys - 1) / kTileKeys;
+    }
+    // tiles of 4096 particles, or of 1024 for at most 65,536 particles (too few tiles otherwise) and
+    // where more than a fifth of the larger tiles' slots would lie past scene ends
+    const int tk = nTot <= 65536 || (double)tiles4k * kTileKeys > 1.25 * (double)nTot ? 1024 : kTileKeys;
+    // keys per bucket on average: a 256th of all keys (256 buckets fill the GPU with two workgroups
+    // each), at least 64 and at most 3/4 of what a workgroup sorts in LDS
+    const double target = std::min(0.75 * kBkCap, std::max(64.0, (double)nTot / 256.0));
+    BucketPlan p;
+    p.tileKeys = tk;
+    p.scenes.resize(S);
+    long long pBase = 0, cBase = 0, info = 0;
+    int bBase = 0, tBase = 0;
+    for (size_t s = 0; s < S; ++s) {
+        const BucketSceneShape &sc = shapes[s];
+        const long long n = sc.particles;
+        const long long c = (long long)sc.grid[0] * sc.grid[1] * sc.grid[2];
+        if (c <= 0 || n < 0) return false;
+        const long long tiles = (n + tk - 1) / tk;
+        if (tiles > kBkMaxTiles) return false;
+        double uneven[3];
+        for (int a = 0; a < 3; ++a) {
+            const double r = sc.extent[a] / (double)sc.grid[a];
+            uneven[a] = r > 0.0 ? std::ceil(r) / r : 1.0;
+        }
+        const bool measured = sc.peak.size() == 17;
+        // the widest shift (fewest buckets, at most 256, offsets of at most 16 bits) whose buckets
+        // average at most `target` keys and whose fullest bucket holds at most 4/5 of the LDS capacity
+        // (measured, leaving room for the particles to move) or 3/4 (estimated); else the narrowest
+        // one if its buckets still average within that
+        int shift = -1, narrowest = -1;
+        for (int b = 16; b >= 0; --b) {
+            const long long nb = (c + (1ll << b) - 1) >> b;
+            if (nb > 256) break;
+            const long long span = 1ll << b;
+            const double avg = (double)n * (double)span / (double)c;
+            double peak = avg * uneven[2];
+            if (span < (long long)sc.grid[0] * sc.grid[1]) peak *= uneven[1];
+            if (span < (long long)sc.grid[0]) peak *= uneven[0];
+            const bool fits = measured ? sc.peak[b] <= 0.8 * kBkCap : peak <= 0.75 * kBkCap;
+            if (avg <= 0.75 * kBkCap) narrowest = b;
+            if (avg <= target && fits) {
+                shift = b;
+                break;
+            }
+        }
+        if (shift < 0) shift = narrowest;
+        if (sc.shift >= 0 && sc.shift <= 16 && ((c + (1ll << sc.shift) - 1) >> sc.shift) <= 256) shift = sc.shift;
+        if (shift < 0) return false;
+        BucketScene &b = p.scenes[s];
+        b.pStart = (unsigned)pBase;
+        b.n = (unsigned)n;
+        b.cellBase = (unsigned)cBase;
+        b.cells = (unsigned)c;
+        b.shift = shift;
+        b.buckets = (int)((c + (1ll << shift) - 1) >> shift);
+        b.bucketBase = bBase;
+        b.tileBase = tBase;
+        b.tiles = (int)tiles;
+        b.infoBase = (int)info;
+        for (int k = 0; k < b.tiles; ++k) p.tileScene.push_back((int)s);
+        for (int k = 0; k < b.buckets; ++k) p.bucketScene.push_back((int)s);
+        pBase += n;
+        cBase += c;
+        bBase += b.buckets;
+        tBase += b.tiles;
+        info += (long long)b.buckets * tiles;
+        if (cBase > 0xffffffffll || info > 0x7fffffffll) return false;
+    }
+    p.infoWords = (size_t)info;
+    *out = std::move(p);
+    return true;
 }
 
-void launch_bucket_sort(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, int keyBits, const float4 *rec,
-                        float4 *sortedPos, float4 *sortedVel, unsigned *table, unsigned numCells, hipStream_t st) {
-    const int lowBits = keyBits - 8;
-    const unsigned buckets = (unsigned)((numCells >> lowBits) + 1u);
-    hipLaunchKernelGGL(k_bucket_sort, dim3(buckets * kBkSplit), dim3(kBkThreads), 0, st, to.keys, to.vals, to.info,
-                       to.numTiles, to.tileKeys, kOut, vOut, to.hist, lowBits, rec, sortedPos, sortedVel, table, numCells);
+void launch_bucket_sort(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, const float4 *rec, float4 *sortedPos,
+                        float4 *sortedVel, unsigned *table, unsigned numCells, int split, hipStream_t st) {
+    if (to.numBuckets <= 0) return;
+    hipLaunchKernelGGL(k_bucket_sort, dim3(to.numBuckets * split), dim3(kBkThreads), 0, st, to, split, kOut, vOut, rec,
+                       sortedPos, sortedVel, table, numCells);
 }
 
 }  // namespace arsim

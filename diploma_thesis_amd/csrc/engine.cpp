@@ -273,6 +273,36 @@ int Engine::state_checksum(int s, uint64_t *out) {
     return ARSIM_OK;
 }
 
+namespace {
+
+// The fullest bucket of a scene's grid keys for every bucket width 2^b cells (b = 0..16, UINT_MAX
+// where there would be more than 256 buckets), from host positions (float4 per particle).
+std::vector<unsigned> bucket_peaks(const float *pos4, int n, const SceneGeom &g) {
+    std::vector<unsigned> keys(n);
+    for (int i = 0; i < n; ++i) {
+        unsigned c[3];
+        for (int a = 0; a < 3; ++a) {
+            const double q = std::floor(((double)pos4[4 * (size_t)i + a] - g.origin[a]) / g.cellSize[a]);
+            const long long v = q > -1e18 && q < 1e18 ? (long long)q : 0;   // NaN and huge values: cell 0
+            c[a] = (unsigned)v & g.gridMask[a];
+        }
+        keys[i] = c[2] * g.gxy + c[1] * g.gx + c[0];
+    }
+    const unsigned long long cells =
+        (unsigned long long)(g.gridMask[0] + 1u) * (g.gridMask[1] + 1u) * (g.gridMask[2] + 1u);
+    std::vector<unsigned> peak(17, 0xffffffffu);
+    std::vector<unsigned> hist(256);
+    for (int b = 16; b >= 0; --b) {
+        if (((cells + (1ull << b) - 1) >> b) > 256) break;
+        std::fill(hist.begin(), hist.end(), 0u);
+        for (unsigned k : keys) ++hist[k >> b];
+        peak[b] = *std::max_element(hist.begin(), hist.end());
+    }
+    return peak;
+}
+
+}  // namespace
+
 int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning, int dev, hipStream_t stream) {
     if (src.empty()) return fail(ARSIM_ERR_INVALID_ARG, "engine: no scenes");
     if (src.size() > 65535) return fail(ARSIM_ERR_INVALID_ARG, "engine: at most 65535 scenes");
@@ -293,9 +323,11 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     if (tu.demBlock <= 0 || tu.demBlock > 1024 || (tu.demBlock & 63)) return fail(ARSIM_ERR_INVALID_ARG, "bad demBlock");
     if (!sort_tuning_supported(tu.sortItemsPerThread, tu.sortRadixBits))
         return fail(ARSIM_ERR_INVALID_ARG, "unsupported sort tuning");
+    if (tu.bucketSplit < 0 || tu.bucketSplit > 2) return fail(ARSIM_ERR_INVALID_ARG, "bad bucketSplit");
 
     // ---- layout
     S = (int)src.size();
+    bucketSort = false;
     struct Desc { int n, b; ArSimParams p; ArSimStepConfig c; };
     std::vector<Desc> d(S);
     for (int s = 0; s < S; ++s) {
@@ -381,7 +413,23 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         // every step; beyond that, segments of 2^segBits cells keep the table at that size
         segBits = keyBits > kExactTableBits ? keyBits - kExactTableBits : 0;
         numSegs = (long long)((numCells - 1) >> segBits) + 1;
-        bucketSort = segBits == 0 && bucket_sort_fits(nTot, keyBits, numCells);
+        if (segBits == 0) {
+            // bucket widths from the particles of host scenes, kept for scenes carried over from an
+            // engine that planned them
+            std::vector<BucketSceneShape> shapes(S);
+            for (int s = 0; s < S; ++s) {
+                shapes[s].particles = pCount[s];
+                for (int a = 0; a < 3; ++a) {
+                    shapes[s].grid[a] = geoms[s].gridMask[a] + 1u;
+                    shapes[s].extent[a] = ((double)geoms[s].boxMax[a] - (double)geoms[s].boxMin[a]) /
+                                          (double)geoms[s].cellSize[a];
+                }
+                if (src[s].host) shapes[s].peak = bucket_peaks(src[s].host->pos, pCount[s], geoms[s]);
+                else if (src[s].from && src[s].from->bucketSort)
+                    shapes[s].shift = src[s].from->bkPlan.scenes[src[s].fromScene].shift;
+            }
+            bucketSort = plan_bucket_sort(shapes, &bkPlan);
+        }
         if ((flags & ARSIM_CONTACT_SIS) && segBits != 0)
             return fail(ARSIM_ERR_UNSUPPORTED, "engine: the sequential-impulse model needs at most 2^30 cells");
         long long base = 0;
@@ -417,7 +465,14 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     if (method == ARSIM_METHOD_GRID) {
         if (tu.fuseReduce) { A(grainIdx, N); A(grainRanges, (size_t)S); }
         A(segStart, (size_t)numSegs + 1);
-        A(tileInfo, (size_t)256 * pose_tile_count(N));
+        if (bucketSort) {
+            numTiles = (int)bkPlan.tileScene.size();
+            numBuckets = (int)bkPlan.bucketScene.size();
+            tileKeys = bkPlan.tileKeys;
+            A(tileInfo, std::max<size_t>(bkPlan.infoWords, 1)); A(bkScenesD, (size_t)S);
+            A(tileSceneD, std::max(numTiles, 1)); A(bucketSceneD, std::max(numBuckets, 1));
+            A(bucketHist, std::max(numBuckets, 1));
+        }
         A(sortedPos, N); A(sortedVel, N);
         longCap = fill_long_capacity((unsigned long long)numSegs + 1ull);
         A(longCnt, 2); A(longList, (size_t)longCap);
@@ -557,6 +612,14 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     if (longCnt) launch_fill_u32(longCnt, 0u, 1, st);
     maxSceneBodies = 0;
     for (int s = 0; s < S; ++s) maxSceneBodies = std::max(maxSceneBodies, bCount[s]);
+    if (bucketSort) {
+        ARSIM_HIP(hipMemcpyAsync(bkScenesD, bkPlan.scenes.data(), sizeof(BucketScene) * S, hipMemcpyHostToDevice, st));
+        ARSIM_HIP(hipMemcpyAsync(tileSceneD, bkPlan.tileScene.data(), sizeof(int) * numTiles, hipMemcpyHostToDevice,
+                                 st));
+        ARSIM_HIP(hipMemcpyAsync(bucketSceneD, bkPlan.bucketScene.data(), sizeof(int) * numBuckets,
+                                 hipMemcpyHostToDevice, st));
+        ARSIM_HIP(hipStreamSynchronize(st));   // the plan's vectors may be replaced by the next build
+    }
     if (sceneBodyD) {
         std::vector<int2> sb(S);
         for (int s = 0; s < S; ++s) sb[s] = make_int2(bStart[s], bCount[s]);
@@ -599,21 +662,21 @@ template <class Rec>
 int Engine::enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, const SceneView &sv, const SortTuning &tuIn,
                                     bool grainVelDead, unsigned **kAOut, unsigned **vAOut, Rec &rec) {
     if (bucketSort) {
-        // the pose kernel ranks each tile of particles by the key's top digit (the sort's first pass);
-        // the bucket kernel gathers each digit's runs, finishes the sort there, and writes the sorted
-        // copies and the cell table
-        unsigned *hist = (unsigned *)sortWs;   // the top digit's histogram (256 words)
+        // the pose kernel ranks each tile of particles by bucket (the sort's first pass); the bucket
+        // kernel gathers each bucket's runs, finishes the sort there, and writes the sorted copies and
+        // the cell table
         ZeroJobs z{};
-        z.p[0] = hist;
-        z.count[0] = 256;
+        z.p[0] = bucketHist;
+        z.count[0] = (size_t)numBuckets;
         launch_body_integrate(bb, B, ph, &z, st);
         rec();
-        const TileKeyOut to{(unsigned *)keysB, valsB, tileInfo, hist, pose_tile_count(N), keyBits - 8,
-                            pose_tile_keys(N)};
+        const TileKeyOut to{(unsigned *)keysB, valsB, tileInfo, bucketHist, bkScenesD, tileSceneD, bucketSceneD,
+                            numTiles, numBuckets, tileKeys};
         launch_pose_key_tiles(pb, bb, N, ph, sv, to, grainVelDead, st);
         rec();
-        launch_bucket_sort(to, (unsigned *)keysA, valsA, keyBits, gatherRec, sortedPos, sortedVel, segStart,
-                           (unsigned)numCells, st);
+        const int split = tu.bucketSplit == 1 || tu.bucketSplit == 2 ? tu.bucketSplit : (numBuckets <= 512 ? 2 : 1);
+        launch_bucket_sort(to, (unsigned *)keysA, valsA, gatherRec, sortedPos, sortedVel, segStart, (unsigned)numCells,
+                           split, st);
         rec();
         *kAOut = (unsigned *)keysA;
         *vAOut = valsA;
@@ -1176,7 +1239,8 @@ int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best,
             return ARSIM_OK;
         }
     }
-    // with the bucket sort (and no contact-list sort) the sort tuning is unused: only the block size
+    // with the bucket sort (and no contact-list sort) the onesweep tuning is unused: only the block size
+    // and the workgroups per bucket
     const bool sortUnused = E.method == ARSIM_METHOD_GRID && E.bucketSort && !(E.flags & ARSIM_CONTACT_SIS);
     // the trials advance a scratch copy of the state: snapshot everything a step writes
     struct Snap { void *dst; const void *src; size_t bytes; };
@@ -1201,25 +1265,37 @@ int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best,
     hipEvent_t a, b;
     hipEventCreate(&a);
     hipEventCreate(&b);
+    // one trial: restore the state, two steps to settle, then time stepsPerTrial steps
+    auto trial = [&](const ArSimTuning &t) -> int {
+        for (auto &s : snaps)
+            if (s.dst) hipMemcpyAsync((void *)s.src, s.dst, s.bytes, hipMemcpyDeviceToDevice, E.st);
+        E.tu = t;
+        E.reset_grid_tables();
+        if (int r = E.step(2)) return r;
+        hipEventRecord(a, E.st);
+        if (int r = E.step(stepsPerTrial)) return r;
+        hipEventRecord(b, E.st);
+        hipEventSynchronize(b);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, a, b);
+        ms /= stepsPerTrial;
+        if (ms < bestT_ms) { bestT_ms = ms; bestT = t; }
+        return ARSIM_OK;
+    };
+    // the bucket sort's workgroups per bucket are tried in place of the onesweep tuning
+    const int splits[] = {1, 2};
     for (int bl : blocks)
         for (int ip : ipts)
             for (int rb : rbits) {
                 if (rc) break;
                 if (!arsim::sort_tuning_supported(ip, rb)) continue;
                 if (sortUnused && (ip != orig.sortItemsPerThread || rb != orig.sortRadixBits)) continue;
-                for (auto &s : snaps)
-                    if (s.dst) hipMemcpyAsync((void *)s.src, s.dst, s.bytes, hipMemcpyDeviceToDevice, E.st);
-                E.tu = ArSimTuning{bl, ip, rb, orig.useGraph, orig.fuseReduce};
-                E.reset_grid_tables();
-                if ((rc = E.step(2)) != 0) break;
-                hipEventRecord(a, E.st);
-                if ((rc = E.step(stepsPerTrial)) != 0) break;
-                hipEventRecord(b, E.st);
-                hipEventSynchronize(b);
-                float t = 0.f;
-                hipEventElapsedTime(&t, a, b);
-                t /= stepsPerTrial;
-                if (t < bestT_ms) { bestT_ms = t; bestT = E.tu; }
+                if (!E.bucketSort) {
+                    rc = trial(ArSimTuning{bl, ip, rb, orig.useGraph, orig.fuseReduce, orig.bucketSplit});
+                    continue;
+                }
+                for (int sp : splits)
+                    if (!rc) rc = trial(ArSimTuning{bl, ip, rb, orig.useGraph, orig.fuseReduce, sp});
             }
     hipEventDestroy(a);
     hipEventDestroy(b);

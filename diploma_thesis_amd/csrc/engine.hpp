@@ -30,7 +30,7 @@ struct Engine {
     ArSimParams p0{};
     ArSimStepConfig c0{};
     DevPhys ph{};
-    ArSimTuning tu{128, 16, 8, 1, 0};
+    ArSimTuning tu{128, 16, 8, 1, 0, 0};
     std::vector<SceneGeom> geoms;
     std::vector<int> pStart, pCount, bStart, bCount;
     std::vector<int> bodyParticles;         // per scene: its body particles (they precede its grains)
@@ -60,12 +60,17 @@ struct Engine {
     unsigned char *keyBufA = nullptr, *keyBufB = nullptr, *sortWsBuf = nullptr;
     unsigned *valsA = nullptr, *valsB = nullptr;
     unsigned *segStart = nullptr;           // grid: CSR offsets over segments of 2^segBits cells
-    unsigned *tileInfo = nullptr;           // grid, bucket sort: per-tile digit runs (count << 16 | first)
+    unsigned *tileInfo = nullptr;           // grid, bucket sort: per-tile bucket runs (count << 16 | first)
+    BucketScene *bkScenesD = nullptr;       // grid, bucket sort: the plan (bucket.hip)
+    int *tileSceneD = nullptr, *bucketSceneD = nullptr;
+    unsigned *bucketHist = nullptr;         // keys per bucket (zeroed each step)
+    int numTiles = 0, numBuckets = 0, tileKeys = 0;
+    BucketPlan bkPlan;
     unsigned *longCnt = nullptr;            // grid: run count of the long-run table fill (zeroed each step)
     uint4 *longList = nullptr;
     unsigned longCap = 0;
     int segBits = 3;
-    bool bucketSort = false;                // grid: sort finished per top-digit bucket (bucket.hip)
+    bool bucketSort = false;                // grid: sort finished per bucket of cells (bucket.hip)
     long long numSegs = 0;
     float4 *sortedPos = nullptr, *sortedVel = nullptr;
     void *sortWs = nullptr;

@@ -284,20 +284,35 @@ __device__ __forceinline__ f3 dem_rows_exact(const float4 *__restrict__ sPos, co
 // keys visit them in the order x-1, x, x+1.  EXACT (segments of one cell, segStart = cellStart
 // with numCells + 1 entries): a row is the run [segStart[h0], segStart[h0 + 3]) with its cell
 // boundaries read from the table, and the keys are not read at all.
-template <bool EMIT, bool EXACT>
+// Where the kernel takes a scene's geometry from: the one scene's, passed by value (kGeomSingle);
+// for several scenes (scene id in sortedVel.w), the table entry of the block's scene when all of the
+// block's particles belong to it (kGeomBlock: the block's first and last sorted particle tell, and
+// the geometry is then read through scalar loads), or each lane's own entry (kGeomLane).  A batch runs
+// the kGeomBlock launch and then the kGeomLane launch, each doing the blocks the other skips.
+enum { kGeomSingle = 0, kGeomBlock = 1, kGeomLane = 2 };
+
+template <bool EMIT, bool EXACT, int GEOM>
 __global__ void k_dem_grid_seg(const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
                                const unsigned *__restrict__ keys, const unsigned *__restrict__ sIdx,
                                const unsigned *__restrict__ segStart, int segBits, int n, DevPhys ph, SceneView sv,
-                               int sceneInVelW, DemOut o) {
+                               DemOut o) {
     const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
-    const int i = lb * (int)blockDim.x + (int)threadIdx.x;
+    const int first = lb * (int)blockDim.x;
+    SceneGeom g;
+    if (GEOM == kGeomSingle) {
+        g = sv.single;
+    } else {
+        const int last = min(n, first + (int)blockDim.x) - 1;
+        const int sf = __float_as_int(sVel[first].w), sl = __float_as_int(sVel[last].w);
+        if ((sf == sl) != (GEOM == kGeomBlock)) return;
+        if (GEOM == kGeomBlock) g = sv.table[sf];
+    }
+    const int i = first + (int)threadIdx.x;
     if (i >= n) return;
     const float4 P = sPos[i];
     const float4 Vq = sVel[i];
     const int ra = __float_as_int(P.w);
-    const int s = sceneInVelW ? __float_as_int(Vq.w) : 0;
-    SceneGeom g;
-    load_geom(sv, s, g);
+    if (GEOM == kGeomLane) g = sv.table[__float_as_int(Vq.w)];
     const f3 pa = xyz(P), va = xyz(Vq);
     int gx, gy, gz;
     grid_pos(pa, g, gx, gy, gz);
@@ -453,21 +468,32 @@ void launch_dem_grid_seg(const float4 *sortedPos, const float4 *sortedVel, const
     if (n <= 0) return;
     if (block <= 0) block = 128;
     dim3 grid((n + block - 1) / block);
-    const int w = sceneInVelW ? 1 : 0;
+    auto run = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(block), 0, st, sortedPos, sortedVel, sortedKeys, sortedIdx, segStart,
+                           segBits, n, ph, sv, o);
+    };
+    auto geoms = [&](auto single, auto perBlock, auto perLane) {
+        if (!sceneInVelW) {
+            run(single);
+        } else {
+            run(perBlock);
+            run(perLane);
+        }
+    };
     if (o.pairs) {
         if (segBits == 0)
-            hipLaunchKernelGGL((k_dem_grid_seg<true, true>), grid, dim3(block), 0, st, sortedPos, sortedVel, sortedKeys,
-                               sortedIdx, segStart, segBits, n, ph, sv, w, o);
+            geoms(k_dem_grid_seg<true, true, kGeomSingle>, k_dem_grid_seg<true, true, kGeomBlock>,
+                  k_dem_grid_seg<true, true, kGeomLane>);
         else
-            hipLaunchKernelGGL((k_dem_grid_seg<true, false>), grid, dim3(block), 0, st, sortedPos, sortedVel,
-                               sortedKeys, sortedIdx, segStart, segBits, n, ph, sv, w, o);
+            geoms(k_dem_grid_seg<true, false, kGeomSingle>, k_dem_grid_seg<true, false, kGeomBlock>,
+                  k_dem_grid_seg<true, false, kGeomLane>);
     } else {
         if (segBits == 0)
-            hipLaunchKernelGGL((k_dem_grid_seg<false, true>), grid, dim3(block), 0, st, sortedPos, sortedVel,
-                               sortedKeys, sortedIdx, segStart, segBits, n, ph, sv, w, o);
+            geoms(k_dem_grid_seg<false, true, kGeomSingle>, k_dem_grid_seg<false, true, kGeomBlock>,
+                  k_dem_grid_seg<false, true, kGeomLane>);
         else
-            hipLaunchKernelGGL((k_dem_grid_seg<false, false>), grid, dim3(block), 0, st, sortedPos, sortedVel,
-                               sortedKeys, sortedIdx, segStart, segBits, n, ph, sv, w, o);
+            geoms(k_dem_grid_seg<false, false, kGeomSingle>, k_dem_grid_seg<false, false, kGeomBlock>,
+                  k_dem_grid_seg<false, false, kGeomLane>);
     }
 }
 

@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "arsim_device.hpp"
 #include "csr_fill.hpp"
 
@@ -140,28 +142,59 @@ void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, c
                         const int *rb, const uint16_t *scene, const float4 *rec, float4 *sortedPos,
                         float4 *sortedVel, unsigned *segStart, int segBits, long long numSegs, int n,
                         const LongFill &lf, hipStream_t st);
+// The bucket sort's plan (bucket.hip).  Scene s holds particles [pStart, pStart + n) and cells
+// [cellBase, cellBase + cells); the scenes' cell ranges are packed in scene order, so the keys of
+// scene s sort below those of scene s + 1 and the scene's sorted range is its particle range.  A
+// scene's cells are cut into `buckets` (at most 256) ranges of 2^shift cells (shift <= 16), the
+// scene's bucket j being global bucket bucketBase + j; its particles into `tiles` tiles of tileKeys
+// particles, global tiles tileBase .. tileBase + tiles - 1.  One scene of 2^24 cells is cut into 256
+// buckets of its top 8 key bits.
+struct BucketScene {
+    unsigned pStart, n, cellBase, cells;
+    int shift, buckets, bucketBase, tileBase, tiles, infoBase;
+};
 // Grid keys ranked per tile of particles inside the pose kernel (the first pass of the bucket sort,
-// body.hip): the tile's keys and particle indices in top-digit order (stable), tileInfo[d * numTiles +
-// tile] = count << 16 | first slot of digit d in the tile, and the top digit's histogram accumulated
-// into hist (zeroed beforehand).  topShift = keyBits - 8.  Tiles hold pose_tile_keys(n) particles:
-// 4096 (kTileKeys), or 1024 for scenes of at most 65,536 particles, which would have too few tiles.
+// body.hip): the tile's keys and particle indices in bucket order (stable), info[infoBase + j * tiles
+// + k] = count << 16 | first slot of bucket j in the scene's tile k, and every bucket's count
+// accumulated into hist[bucketBase + j] (zeroed beforehand).  Tiles hold tileKeys particles: 4096
+// (kTileKeys), or 1024 where tiles of 4096 would be too few to fill the GPU or scenes are small.
 constexpr int kTileKeys = 4096;
 struct TileKeyOut {
     unsigned *keys, *vals, *info, *hist;
-    int numTiles, topShift, tileKeys;
+    const BucketScene *scenes;
+    const int *tileScene, *bucketScene;   // scene of each tile and of each bucket
+    int numTiles, numBuckets, tileKeys;
 };
-int pose_tile_keys(long long numParticles);
-int pose_tile_count(long long numParticles);
 void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,
                            const SceneView &sv, const TileKeyOut &to, bool grainVelDead, hipStream_t st);
-// The grid sort finished per top-digit bucket (bucket.hip): each bucket gathers its keys from every
-// tile's run of its digit (tile order keeps the sort stable), two workgroups sort it by the low bits,
-// write the sorted keys / indices to (kOut, vOut), the gathered sorted copies and their range of the
-// exact cell table.  bucket_sort_fits: keyBits in [9, 24], at most 384 tiles, and buckets that mostly
-// fit in LDS for n keys.
-bool bucket_sort_fits(long long n, int keyBits, unsigned long long numCells);
-void launch_bucket_sort(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, int keyBits, const float4 *rec,
-                        float4 *sortedPos, float4 *sortedVel, unsigned *table, unsigned numCells, hipStream_t st);
+// The host side of the plan: tiles of `tileKeys` particles, and per scene the widest shift whose
+// buckets fit in LDS.  A scene is described by its particle count, its grid (cells per axis), and
+// either the fullest bucket measured on its particles for every shift (peak[b], b = 0..16), or the
+// extent of its container in cells per axis (0 when unknown): where the container is not a whole
+// number of grid periods long the hashed cells are unevenly filled, and the plan then expects the
+// fullest bucket to hold ceil(r) / r times the average along each axis a bucket does not span,
+// r = extent / grid size.  A scene that kept its shift from an earlier plan (a rebuild) passes it.
+// False when a scene needs more than 256 buckets or 384 tiles.
+struct BucketSceneShape {
+    long long particles = 0;
+    unsigned grid[3] = {1, 1, 1};
+    double extent[3] = {0.0, 0.0, 0.0};
+    std::vector<unsigned> peak;   // empty, or 17 entries
+    int shift = -1;
+};
+struct BucketPlan {
+    std::vector<BucketScene> scenes;
+    std::vector<int> tileScene, bucketScene;
+    size_t infoWords = 0;
+    int tileKeys = kTileKeys;
+};
+bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *out);
+// The grid sort finished per bucket (bucket.hip): each bucket gathers its keys from its scene's
+// tiles' runs (tile order keeps the sort stable), `split` workgroups (1 or 2) sort it by the low bits,
+// each writing its share of the sorted keys / indices to (kOut, vOut), of the gathered sorted copies
+// and of the bucket's range of the exact cell table (whose last entry, numCells, the last bucket writes).
+void launch_bucket_sort(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, const float4 *rec, float4 *sortedPos,
+                        float4 *sortedVel, unsigned *table, unsigned numCells, int split, hipStream_t st);
 // Capacity of the long-run list of a table of tableSize entries (runs are disjoint and longer than
 // kLongRange), and the launch that fills the listed runs.
 unsigned fill_long_capacity(unsigned long long tableSize);

@@ -199,7 +199,7 @@ extern "C" int arsim_serve_run(const ArSimServeConfig *cfg, ArSimServeReport *re
     double peakBytes = 0;
     const auto t0 = clk::now();
     auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
-    ArSimTuning tune{128, 16, 8, 0, 0};
+    ArSimTuning tune{128, 16, 8, 0, 0, 0};
     double lastReport = 0.0;
     auto progress = [&](int done) {   // a line on stderr every 10 s of wall time
         const double t = now();

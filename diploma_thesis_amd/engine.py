@@ -10,13 +10,13 @@ class Engine:
     """S >= 1 scenes resident on the GPU; step() advances all of them (one launch per stage)."""
 
     def __init__(self, scenes, dem_block=128, sort_ipt=16, sort_radix_bits=8, use_graph=True, device=0,
-                 fuse_reduce=False):
+                 fuse_reduce=False, bucket_split=0):
         if not isinstance(scenes, (list, tuple)):
             scenes = [scenes]
         self.scenes = list(scenes)
         self._views = (capi.ArSimSceneHost * len(scenes))(*[s.view() for s in scenes])
         self.tuning = capi.ArSimTuning(dem_block, sort_ipt, sort_radix_bits, 1 if use_graph else 0,
-                                       1 if fuse_reduce else 0)
+                                       1 if fuse_reduce else 0, bucket_split)
         self.h = C.c_void_p()
         capi.check(capi.lib.arsim_engine_create(self._views, len(scenes), C.byref(self.tuning), device,
                                                 C.byref(self.h)), "engine_create")
@@ -151,7 +151,7 @@ class Engine:
         capi.check(capi.lib.arsim_engine_autotune(self.h, steps_per_trial, C.byref(best), C.byref(ms)),
                    "engine_autotune")
         self.tuning = best
-        return (best.demBlock, best.sortItemsPerThread, best.sortRadixBits), ms.value
+        return (best.demBlock, best.sortItemsPerThread, best.sortRadixBits, best.bucketSplit), ms.value
 
 
 def contact_set(pairs, counts, offset=0):

@@ -406,6 +406,8 @@ typedef struct ArSimTuning {
                                          per-body reduction (bodies one wave each, grains 64 per
                                          wave, original order); same results, measured slower than
                                          the sorted-order contact kernel plus the reduction */
+    int bucketSplit;                  /* grid method, bucket sort: workgroups per bucket (1 or 2;
+                                         0 = two while there are at most 512 buckets, else one) */
 } ArSimTuning;
 
 typedef struct arsim_engine arsim_engine;

@@ -29,7 +29,7 @@ class ParticleSystemLite {
     // The scene is copied to the device; the host arrays behind `scene` may be released afterwards.
     explicit ParticleSystemLite(const ArSimSceneHost &scene, int device = 0, bool useGraph = true)
         : numParticles_(scene.numParticles), numBodies_(scene.numBodies), dt_(scene.cfg.dt) {
-        ArSimTuning tuning = {128, 16, 8, useGraph ? 1 : 0, 0};
+        ArSimTuning tuning = {128, 16, 8, useGraph ? 1 : 0, 0, 0};
         check(arsim_engine_create(&scene, 1, &tuning, device, &engine_), "engine_create");
     }
     ~ParticleSystemLite() {

@@ -142,10 +142,9 @@ def test_engine_cell_table_with_long_empty_runs():
     assert_state_equal(g, st, "sparse grid")
 
 
-def test_bucket_sort_overflowing_bucket():
-    """9,000 grains in one slab of a 64^3 grid: their keys share the top 8 bits, so one bucket of the
-    per-bucket sort holds more keys than LDS does (8,160) and is sorted in global memory."""
-    print("engine: bucket sort, one bucket past the LDS capacity", flush=True)
+def overflowing_scene():
+    """A scene of 64^3 cells with 9,000 grains in one slab (z in [10, 11), y in [1, 15)): even the
+    narrowest buckets (256 of 1,024 cells) leave one bucket with more keys than LDS holds (7,776)."""
     sc = scenes.mix_scene(8000, 95, capi.METHOD_GRID)
     rng = np.random.default_rng(95)
     g = np.where(sc.rb < 0)[0]
@@ -165,27 +164,66 @@ def test_bucket_sort_overflowing_bucket():
     sc2.vel = np.concatenate([sc2.vel, np.zeros_like(add_pos)])
     sc2.rb = np.concatenate([sc2.rb, np.full(extra, -1, np.int32)])
     sc2.body_frame = np.concatenate([sc2.body_frame, np.zeros_like(add_pos)])
-    p = sc2.params
-    keys, _ = oracle.calc_hash(sc2.pos, p)
-    top = keys >> 10
-    assert np.bincount(top).max() > 8160 and sc2.n / 256 < 0.75 * 8160
+    keys, _ = oracle.calc_hash(sc2.pos, sc2.params)
+    assert np.bincount(keys >> 10).max() > 7776 and sc2.n / 256 < 0.75 * 7776
+    return sc2
+
+
+def read_keys_and_table(e, n, total):
+    kp, tp = C.c_void_p(), C.c_void_p()
+    nul = None
+    capi.check(L.arsim_engine_debug_buffers(e.h, C.byref(kp), nul, C.byref(tp), nul, nul, nul, nul, nul, nul, nul))
+    keys = np.zeros(n, np.uint32)
+    table = np.zeros(total + 1, np.uint32)
+    capi.check(L.arsim_memcpy_dtoh(capi.ptr(keys), kp, keys.nbytes, None))
+    capi.check(L.arsim_memcpy_dtoh(capi.ptr(table), tp, table.nbytes, None))
+    return keys, table
+
+
+def test_bucket_sort_overflowing_bucket():
+    """The overflowing bucket is sorted in global memory; keys, cell table and state stay exact."""
+    print("engine: bucket sort, one bucket past the LDS capacity", flush=True)
+    sc2 = overflowing_scene()
+    total = int(sc2.params.numCells)
     with Engine(sc2, use_graph=False) as e:
         e.step(2)
         e.sync()
-        kp, tp = C.c_void_p(), C.c_void_p()
-        nul = None
-        capi.check(L.arsim_engine_debug_buffers(e.h, C.byref(kp), nul, C.byref(tp), nul, nul, nul, nul, nul, nul, nul))
-        total = int(p.numCells)
-        kk = np.zeros(sc2.n, np.uint32)
-        table = np.zeros(total + 1, np.uint32)
-        capi.check(L.arsim_memcpy_dtoh(capi.ptr(kk), kp, kk.nbytes, None))
-        capi.check(L.arsim_memcpy_dtoh(capi.ptr(table), tp, table.nbytes, None))
+        kk, table = read_keys_and_table(e, sc2.n, total)
         g2 = e.download()
     assert np.all(kk[1:] >= kk[:-1])
     assert np.array_equal(table, np.searchsorted(kk, np.arange(total + 1, dtype=np.int64), "left").astype(np.uint32))
     st = oracle.State(sc2, np.float32)
     st.step(2)
     assert_state_equal(g2, st, "overflowing bucket")
+
+
+def test_bucket_sort_scene_buckets_in_a_batch():
+    """Buckets per scene in a batch: a scene of 8^3 cells first, so the later scenes' cell ranges (and
+    their buckets) start at offsets that are not multiples of the bucket width; then the overflowing
+    scene (its full bucket sorted in global memory) and a scene of 4,000 particles.  The batch's keys are
+    sorted, its cell table is exact CSR over all scenes' cells, and every scene equals the oracle alone
+    after graph-replayed steps, for one and two workgroups per bucket."""
+    print("engine: bucket sort, per-scene buckets in a batch", flush=True)
+    small = scenes.mix_scene(1100, 61, capi.METHOD_GRID)
+    small.params.gridSize.x = small.params.gridSize.y = small.params.gridSize.z = 8
+    small.params.numCells = 512
+    mix = [small, overflowing_scene(), scenes.mix_scene(4000, 62, capi.METHOD_GRID)]
+    total = sum(int(s.params.numCells) for s in mix)
+    n = sum(s.n for s in mix)
+    for split in (1, 2):
+        with Engine(mix, use_graph=True, bucket_split=split) as e:
+            e.step(3)
+            e.sync()
+            keys, table = read_keys_and_table(e, n, total)
+            got = [e.download(k) for k in range(len(mix))]
+            stages = [name for name, _ in e.profile()]
+        assert "bucket_sort_reorder" in stages, stages
+        assert np.all(keys[1:] >= keys[:-1]) and int(keys[-1]) < total
+        assert np.array_equal(table, np.searchsorted(keys, np.arange(total + 1, dtype=np.int64), "left").astype(np.uint32))
+        for k, sc in enumerate(mix):
+            st = oracle.State(sc, np.float32)
+            st.step(3)
+            assert_state_equal(got[k], st, f"split {split}, scene {k}")
 
 
 @pytest.mark.parametrize("g", [8, 16, 32, 128])
@@ -250,7 +288,7 @@ def test_benchmark_scene_two_steps_bit_exact():
 def test_benchmark_batch_one_step_bit_exact():
     """The benchmark's batch: 512 scenes log-uniform in [1024, 65536] particles (8.85 M) stepped as one
     graph-replayed batch.  Every scene's state after a step equals the oracle's step of that scene
-    alone (the classic sort and segment reorder at this size, long empty runs of the cell table)."""
+    alone (27-bit keys: the bucket sort with buckets per scene, and long empty runs of the cell table)."""
     print("engine: 512-scene benchmark batch, 1 step", flush=True)
     mix = scenes.batch_mix(512, seed=0, method=capi.METHOD_GRID)
     assert sum(s.n for s in mix) > 8_000_000
