@@ -231,10 +231,11 @@ __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, 
 // lane) order is particle order.
 constexpr int kTileItems = 4;   // particles per thread
 
-__device__ __forceinline__ uint64_t match8(unsigned d, bool ok) {
+template <int BITS>
+__device__ __forceinline__ uint64_t match_bits(unsigned d, bool ok) {
     uint64_t peers = __ballot(ok);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < BITS; ++b) {
         const unsigned t = 0u - ((d >> b) & 1u);
         const uint64_t m = __ballot(t != 0u);
         const unsigned lo = ~((unsigned)m ^ t), hi = ~((unsigned)(m >> 32) ^ t);
@@ -243,16 +244,18 @@ __device__ __forceinline__ uint64_t match8(unsigned d, bool ok) {
     return peers;
 }
 
-template <int THREADS>
+template <int THREADS, int BITS>
 __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
                                                             TileKeyOut to, int grainVelDead) {
     constexpr int WAVES = THREADS / 64;
     constexpr int TILE = THREADS * kTileItems;
-    __shared__ unsigned sWaveHist[WAVES][256];
-    __shared__ unsigned sDigitStart[256];
+    constexpr int RADIX = 1 << BITS;   // buckets a scene may have
+    static_assert(RADIX <= THREADS, "a thread per bucket");
+    __shared__ unsigned sWaveHist[WAVES][RADIX];
+    __shared__ unsigned sDigitStart[RADIX];
     __shared__ unsigned sScan[WAVES];
     const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
-    for (int k = t; k < WAVES * 256; k += THREADS) (&sWaveHist[0][0])[k] = 0u;
+    for (int k = t; k < WAVES * RADIX; k += THREADS) (&sWaveHist[0][0])[k] = 0u;
     const int scene = to.tileScene[blockIdx.x];
     const BucketScene bs = to.scenes[scene];
     const int kt = (int)blockIdx.x - bs.tileBase;   // the tile's place among its scene's tiles
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
     for (int k = 0; k < kTileItems; ++k) {
         const bool valid = waveBase + k * 64 + lane < tileEnd;
         const unsigned d = valid ? (keys[k] - bs.cellBase) >> bs.shift : 0u;
-        const uint64_t peers = match8(d, valid);
+        const uint64_t peers = match_bits<BITS>(d, valid);
         const unsigned prior = (unsigned)__popcll(peers & lower);
         const unsigned old = sWaveHist[wave][d];
         rank[k] = old + prior;
@@ -286,7 +289,7 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
     __syncthreads();
     // per digit: exclusive over waves and the tile's count, then the digits' first slots
     unsigned cnt = 0u;
-    if (t < 256) {
+    if (t < RADIX) {
         unsigned run = 0u;
 #pragma unroll
         for (int w = 0; w < WAVES; ++w) {
@@ -307,7 +310,7 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
     unsigned before = 0u;
     for (int w = 0; w < wave; ++w) before += sScan[w];
     const unsigned first = before + x - cnt;
-    if (t < 256) {
+    if (t < RADIX) {
         sDigitStart[t] = first;
         if (t < bs.buckets) {
             to.info[(size_t)bs.infoBase + (size_t)t * bs.tiles + kt] = (cnt << 16) | first;
@@ -564,12 +567,15 @@ void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticle
                            const SceneView &sv, const TileKeyOut &to, bool grainVelDead, hipStream_t st) {
     if (numParticles <= 0) return;
     const int g = grainVelDead ? 1 : 0;
+    const dim3 grid(to.numTiles);
     if (to.tileKeys == 1024)
-        hipLaunchKernelGGL(k_pose_key_tiles<256>, dim3(to.numTiles), dim3(256), 0, st, p, b, numParticles, ph, sv, to,
-                           g);
+        hipLaunchKernelGGL((k_pose_key_tiles<256, 8>), grid, dim3(256), 0, st, p, b, numParticles, ph, sv, to, g);
+    else if (to.bucketBits <= 8)
+        hipLaunchKernelGGL((k_pose_key_tiles<1024, 8>), grid, dim3(1024), 0, st, p, b, numParticles, ph, sv, to, g);
+    else if (to.bucketBits == 9)
+        hipLaunchKernelGGL((k_pose_key_tiles<1024, 9>), grid, dim3(1024), 0, st, p, b, numParticles, ph, sv, to, g);
     else
-        hipLaunchKernelGGL(k_pose_key_tiles<1024>, dim3(to.numTiles), dim3(1024), 0, st, p, b, numParticles, ph, sv,
-                           to, g);
+        hipLaunchKernelGGL((k_pose_key_tiles<1024, 10>), grid, dim3(1024), 0, st, p, b, numParticles, ph, sv, to, g);
 }
 
 void launch_reduce_apply(const float4 *pForce, const float4 *pTorque, const float4 *relPos, const BodyBufs &b,

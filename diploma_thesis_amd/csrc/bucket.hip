@@ -327,11 +327,15 @@ bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *o
     // tiles of 4096 particles, or of 1024 for at most 65,536 particles (too few tiles otherwise) and
     // where more than a fifth of the larger tiles' slots would lie past scene ends
     const int tk = nTot <= 65536 || (double)tiles4k * kTileKeys > 1.25 * (double)nTot ? 1024 : kTileKeys;
-    // keys per bucket on average: a 256th of all keys (256 buckets fill the GPU with two workgroups
-    // each), at least 64 and at most 3/4 of what a workgroup sorts in LDS
-    const double target = std::min(0.75 * kBkCap, std::max(64.0, (double)nTot / 256.0));
+    // keys per bucket on average: a 512th of all keys (512 buckets fill the GPU with one workgroup
+    // each; 256 buckets with two workgroups each, which sort every bucket twice, and 1,024 buckets
+    // were slower at 2^20 keys), at least 64 and at most 3/4 of what a workgroup sorts in LDS
+    const double target = std::min(0.75 * kBkCap, std::max(64.0, (double)nTot / 512.0));
+    // buckets per scene: the tile rank's digit, up to 10 bits with tiles of 4096 (1024 threads)
+    const int maxBuckets = tk == kTileKeys ? 1024 : 256;
     BucketPlan p;
     p.tileKeys = tk;
+    int maxSceneBuckets = 1;
     p.scenes.resize(S);
     long long pBase = 0, cBase = 0, info = 0;
     int bBase = 0, tBase = 0;
@@ -348,14 +352,14 @@ bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *o
             uneven[a] = r > 0.0 ? std::ceil(r) / r : 1.0;
         }
         const bool measured = sc.peak.size() == 17;
-        // the widest shift (fewest buckets, at most 256, offsets of at most 16 bits) whose buckets
+        // the widest shift (fewest buckets, at most maxBuckets, offsets of at most 16 bits) whose buckets
         // average at most `target` keys and whose fullest bucket holds at most 4/5 of the LDS capacity
         // (measured, leaving room for the particles to move) or 3/4 (estimated); else the narrowest
         // one if its buckets still average within that
         int shift = -1, narrowest = -1;
         for (int b = 16; b >= 0; --b) {
             const long long nb = (c + (1ll << b) - 1) >> b;
-            if (nb > 256) break;
+            if (nb > maxBuckets) break;
             const long long span = 1ll << b;
             const double avg = (double)n * (double)span / (double)c;
             double peak = avg * uneven[2];
@@ -369,7 +373,8 @@ bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *o
             }
         }
         if (shift < 0) shift = narrowest;
-        if (sc.shift >= 0 && sc.shift <= 16 && ((c + (1ll << sc.shift) - 1) >> sc.shift) <= 256) shift = sc.shift;
+        if (sc.shift >= 0 && sc.shift <= 16 && ((c + (1ll << sc.shift) - 1) >> sc.shift) <= maxBuckets)
+            shift = sc.shift;
         if (shift < 0) return false;
         BucketScene &b = p.scenes[s];
         b.pStart = (unsigned)pBase;
@@ -378,6 +383,7 @@ bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *o
         b.cells = (unsigned)c;
         b.shift = shift;
         b.buckets = (int)((c + (1ll << shift) - 1) >> shift);
+        maxSceneBuckets = std::max(maxSceneBuckets, b.buckets);
         b.bucketBase = bBase;
         b.tileBase = tBase;
         b.tiles = (int)tiles;
@@ -392,6 +398,7 @@ bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *o
         if (cBase > 0xffffffffll || info > 0x7fffffffll) return false;
     }
     p.infoWords = (size_t)info;
+    p.bucketBits = maxSceneBuckets <= 256 ? 8 : maxSceneBuckets <= 512 ? 9 : 10;
     *out = std::move(p);
     return true;
 }

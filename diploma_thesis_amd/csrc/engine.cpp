@@ -214,7 +214,7 @@ PartBufs Engine::part_bufs() const {
 }
 
 const char *const *Engine::stage_names(int method, unsigned flags, bool bucket, bool fusedDem, int *count) {
-    // with the bucket sort the pose stage also ranks each tile by the top digit (the sort's first pass)
+    // with the bucket sort the pose stage also ranks each tile by bucket (the sort's first pass)
     // and the bucket stage finishes the sort with the reorder
     static const char *const sis[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "sis_find",
                                       "sis_gather", "sis_solve", "pose"};
@@ -276,7 +276,7 @@ int Engine::state_checksum(int s, uint64_t *out) {
 namespace {
 
 // The fullest bucket of a scene's grid keys for every bucket width 2^b cells (b = 0..16, UINT_MAX
-// where there would be more than 256 buckets), from host positions (float4 per particle).
+// where there would be more than 1024 buckets), from host positions (float4 per particle).
 std::vector<unsigned> bucket_peaks(const float *pos4, int n, const SceneGeom &g) {
     std::vector<unsigned> keys(n);
     for (int i = 0; i < n; ++i) {
@@ -291,9 +291,9 @@ std::vector<unsigned> bucket_peaks(const float *pos4, int n, const SceneGeom &g)
     const unsigned long long cells =
         (unsigned long long)(g.gridMask[0] + 1u) * (g.gridMask[1] + 1u) * (g.gridMask[2] + 1u);
     std::vector<unsigned> peak(17, 0xffffffffu);
-    std::vector<unsigned> hist(256);
+    std::vector<unsigned> hist(1024);
     for (int b = 16; b >= 0; --b) {
-        if (((cells + (1ull << b) - 1) >> b) > 256) break;
+        if (((cells + (1ull << b) - 1) >> b) > 1024) break;
         std::fill(hist.begin(), hist.end(), 0u);
         for (unsigned k : keys) ++hist[k >> b];
         peak[b] = *std::max_element(hist.begin(), hist.end());
@@ -671,10 +671,10 @@ int Engine::enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, cons
         launch_body_integrate(bb, B, ph, &z, st);
         rec();
         const TileKeyOut to{(unsigned *)keysB, valsB, tileInfo, bucketHist, bkScenesD, tileSceneD, bucketSceneD,
-                            numTiles, numBuckets, tileKeys};
+                            numTiles, numBuckets, tileKeys, bkPlan.bucketBits};
         launch_pose_key_tiles(pb, bb, N, ph, sv, to, grainVelDead, st);
         rec();
-        const int split = tu.bucketSplit == 1 || tu.bucketSplit == 2 ? tu.bucketSplit : (numBuckets <= 512 ? 2 : 1);
+        const int split = tu.bucketSplit == 1 || tu.bucketSplit == 2 ? tu.bucketSplit : (numBuckets < 512 ? 2 : 1);
         launch_bucket_sort(to, (unsigned *)keysA, valsA, gatherRec, sortedPos, sortedVel, segStart, (unsigned)numCells,
                            split, st);
         rec();

@@ -145,10 +145,10 @@ void launch_reorder_seg(const unsigned *sortedKeys, const unsigned *sortedIdx, c
 // The bucket sort's plan (bucket.hip).  Scene s holds particles [pStart, pStart + n) and cells
 // [cellBase, cellBase + cells); the scenes' cell ranges are packed in scene order, so the keys of
 // scene s sort below those of scene s + 1 and the scene's sorted range is its particle range.  A
-// scene's cells are cut into `buckets` (at most 256) ranges of 2^shift cells (shift <= 16), the
-// scene's bucket j being global bucket bucketBase + j; its particles into `tiles` tiles of tileKeys
-// particles, global tiles tileBase .. tileBase + tiles - 1.  One scene of 2^24 cells is cut into 256
-// buckets of its top 8 key bits.
+// scene's cells are cut into `buckets` (at most 1,024; 256 with tiles of 1024) ranges of 2^shift cells
+// (shift <= 16), the scene's bucket j being global bucket bucketBase + j; its particles into `tiles`
+// tiles of tileKeys particles, global tiles tileBase .. tileBase + tiles - 1.  One scene of 2^20
+// particles over 2^24 cells is cut into 512 buckets of its top 9 key bits.
 struct BucketScene {
     unsigned pStart, n, cellBase, cells;
     int shift, buckets, bucketBase, tileBase, tiles, infoBase;
@@ -164,6 +164,7 @@ struct TileKeyOut {
     const BucketScene *scenes;
     const int *tileScene, *bucketScene;   // scene of each tile and of each bucket
     int numTiles, numBuckets, tileKeys;
+    int bucketBits;                       // a scene has at most 2^bucketBits buckets (8, 9 or 10)
 };
 void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,
                            const SceneView &sv, const TileKeyOut &to, bool grainVelDead, hipStream_t st);
@@ -174,7 +175,7 @@ void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticle
 // number of grid periods long the hashed cells are unevenly filled, and the plan then expects the
 // fullest bucket to hold ceil(r) / r times the average along each axis a bucket does not span,
 // r = extent / grid size.  A scene that kept its shift from an earlier plan (a rebuild) passes it.
-// False when a scene needs more than 256 buckets or 384 tiles.
+// False when a scene needs more buckets than its tiles can rank or more than 384 tiles.
 struct BucketSceneShape {
     long long particles = 0;
     unsigned grid[3] = {1, 1, 1};
@@ -187,6 +188,7 @@ struct BucketPlan {
     std::vector<int> tileScene, bucketScene;
     size_t infoWords = 0;
     int tileKeys = kTileKeys;
+    int bucketBits = 8;   // the most buckets of one scene, rounded up to a power of two (at least 2^8)
 };
 bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *out);
 // The grid sort finished per bucket (bucket.hip): each bucket gathers its keys from its scene's

@@ -407,7 +407,7 @@ typedef struct ArSimTuning {
                                          wave, original order); same results, measured slower than
                                          the sorted-order contact kernel plus the reduction */
     int bucketSplit;                  /* grid method, bucket sort: workgroups per bucket (1 or 2;
-                                         0 = two while there are at most 512 buckets, else one) */
+                                         0 = two while there are fewer than 512 buckets, else one) */
 } ArSimTuning;
 
 typedef struct arsim_engine arsim_engine;

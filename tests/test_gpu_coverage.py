@@ -197,6 +197,39 @@ def test_bucket_sort_overflowing_bucket():
     assert_state_equal(g2, st, "overflowing bucket")
 
 
+def test_bucket_sort_ten_bit_tile_rank():
+    """A scene of 200,000 particles on a 128^3 grid with its 50,000 grains packed into one z plane: at
+    512 buckets (2^12 cells each) one bucket would hold 12,500 keys, so the plan takes 1,024 buckets of
+    2^11 cells and the pose kernel ranks its tiles by a 10-bit bucket index.  Keys, cell table and state
+    stay exact after graph-replayed steps."""
+    print("engine: bucket sort, 1,024 buckets in one scene", flush=True)
+    sc = scenes.mix_scene(200000, 64, capi.METHOD_GRID)
+    p = sc.params
+    p.gridSize.x = p.gridSize.y = p.gridSize.z = 128
+    p.numCells = 128 ** 3
+    rng = np.random.default_rng(64)
+    g = np.where(sc.rb < 0)[0]
+    assert len(g) >= 45000
+    sc.pos[g, 0] = rng.uniform(1.0, 127.0, len(g)).astype(np.float32)
+    sc.pos[g, 1] = rng.uniform(1.0, 127.0, len(g)).astype(np.float32)
+    sc.pos[g, 2] = rng.uniform(10.05, 10.95, len(g)).astype(np.float32)
+    keys, _ = oracle.calc_hash(sc.pos, p)
+    assert np.bincount(keys >> 12).max() > 7776 and np.bincount(keys >> 11).max() < 7776
+    total = 128 ** 3
+    with Engine(sc, use_graph=True) as e:
+        e.step(2)
+        e.sync()
+        kk, table = read_keys_and_table(e, sc.n, total)
+        got = e.download()
+        stages = [name for name, _ in e.profile()]
+    assert "bucket_sort_reorder" in stages, stages
+    assert np.all(kk[1:] >= kk[:-1])
+    assert np.array_equal(table, np.searchsorted(kk, np.arange(total + 1, dtype=np.int64), "left").astype(np.uint32))
+    st = oracle.State(sc, np.float32)
+    st.step(2)
+    assert_state_equal(got, st, "1,024 buckets")
+
+
 def test_bucket_sort_scene_buckets_in_a_batch():
     """Buckets per scene in a batch: a scene of 8^3 cells first, so the later scenes' cell ranges (and
     their buckets) start at offsets that are not multiples of the bucket width; then the overflowing
