@@ -115,7 +115,7 @@ def traversal_stats(args):
     fn.restype, fn.argtypes = C.c_int, [C.c_void_p, C.c_int]
     method = {"grid": capi.METHOD_LBVH30, "lbvh30": capi.METHOD_LBVH30, "lbvh63": capi.METHOD_LBVH63}[args.method]
     sc = scenes.config1(method)
-    out = np.zeros(8, np.uint64)
+    out = np.zeros(10, np.uint64)
     with Engine(sc, use_graph=False) as e:
         e.step(1)
         e.sync()
@@ -123,13 +123,16 @@ def traversal_stats(args):
         e.step(args.steps)
         e.sync()
         capi.check(fn(capi.ptr(out), 1), "stats")
-    waves, visits, lanes, leaf_tests, interest, max_sp, done, contacts = (int(x) for x in out)
+    waves, visits, lanes, leaf_tests, interest, max_sp, done, contacts, packets, iters = (int(x) for x in out)
+    width = 64 * waves / max(packets, 1)   # lanes per packet
     print(json.dumps({"traversal_stats": True, "method": "lbvh30" if method == capi.METHOD_LBVH30 else "lbvh63",
-                      "particles": sc.n, "steps": args.steps, "waves": waves,
-                      "nodes_visited_per_wave": visits / max(waves, 1),
+                      "particles": sc.n, "steps": args.steps, "waves": waves, "packets": packets,
+                      "lanes_per_packet": width,
+                      "nodes_visited_per_packet": visits / max(packets, 1),
+                      "node_loop_iterations_per_wave": iters / max(waves, 1),
                       "leaf_tests_per_particle": leaf_tests / max(lanes, 1),
                       "contacts_per_particle": contacts / max(lanes, 1),
-                      "packet_efficiency": interest / max(visits * 64, 1),
+                      "packet_efficiency": interest / max(visits * width, 1),
                       "max_stack_depth": max_sp, "completed_lanes": done, "valid_lanes": lanes}))
 
 

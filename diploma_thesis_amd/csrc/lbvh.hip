@@ -10,11 +10,12 @@ constexpr int kStackDepth = 64;   // per-thread LDS stack entries (the reference
 constexpr int kWaveStackDepth = 64;   // per-wave LDS stack entries of the packet traversal
 
 #ifdef ARSIM_TRAVERSAL_STATS
-// Counting build only: totals of the packet traversal, accumulated per wave.
-//   [0] waves  [1] internal nodes visited (wave-level)  [2] valid lanes  [3] lane-leaf tests
+// Counting build only: totals of the packet traversal.
+//   [0] waves  [1] internal nodes visited, summed over packets  [2] valid lanes  [3] lane-leaf tests
 //   [4] lanes whose box overlaps an entered node, summed over entered nodes  [5] largest stack depth
-//   [6] lanes that completed  [7] contacts found
-__device__ unsigned long long g_travStats[8];
+//   [6] lanes that completed  [7] contacts found  [8] packets  [9] iterations of the waves' node loops
+constexpr int kTravStats = 10;
+__device__ unsigned long long g_travStats[kTravStats];
 #endif
 
 __device__ __forceinline__ int clz_key(unsigned v) { return __clz((int)v); }
@@ -578,6 +579,8 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
             atomicMax(&g_travStats[5], (unsigned long long)stMaxSp);
             atomicAdd(&g_travStats[6], done);
             atomicAdd(&g_travStats[7], ct);
+            atomicAdd(&g_travStats[8], 1ull);
+            atomicAdd(&g_travStats[9], stVisits);
         }
     }
 #endif
@@ -598,15 +601,182 @@ __global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__re
     }
 }
 
+// Packets of 16 lanes, four per wave (one scene): each 16 consecutive leaves in Morton order walk the
+// tree together as in k_dem_lbvh_wave, with the packet's own node, LDS stack (kWaveStackDepth entries)
+// and depth.  A smaller packet's lanes are closer together, so it enters fewer nodes that none of its
+// lanes overlaps (at the 2^20-particle scene a 64-lane packet enters 157 nodes, a 16-lane packet 78,
+// and the wave, which runs as long as its slowest packet, 95).  A visit loads the packet's 64-byte node
+// record as one dword per lane and passes the words round the packet by DPP row broadcasts.  Each lane
+// meets exactly the leaves of its own traversal in the same order, as in k_dem_lbvh_wave, so the force
+// sums are the same bits.
+constexpr int kPacketLanes = 16;
+// Word K of each 16-lane row (DPP row_newbcast: every lane of a row receives lane K's value), as a float.
+template <int K>
+__device__ __forceinline__ float row_word(unsigned w) {
+    return __uint_as_float((unsigned)__builtin_amdgcn_update_dpp(0, (int)w, 0x150 + K, 0xf, 0xf, false));
+}
+template <bool EMIT>
+__global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *__restrict__ packed,
+                                   const float4 *__restrict__ cms, const float4 *__restrict__ sVel,
+                                   const float4 *__restrict__ bounds, int n, DevPhys ph, SceneView sv, DemOut o,
+                                   int *overflow) {
+    extern __shared__ int pStack[];
+    const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int i = lb * (int)blockDim.x + (int)threadIdx.x;
+    const bool valid = i < n;
+    const int ii = valid ? i : n - 1;
+    const int lane = (int)(threadIdx.x & 63);
+    const int pl = lane & (kPacketLanes - 1);
+    const unsigned long long pmask = 0xffffull << (lane & ~(kPacketLanes - 1));
+    int *stack = pStack + ((int)threadIdx.x / kPacketLanes) * kWaveStackDepth;
+    const float4 C = cms[ii];
+    const float4 Vq = sVel[ii];
+    const int ra = __float_as_int(C.w);
+    const float4 qMin = bounds[2 * (size_t)ii], qMax = bounds[2 * (size_t)ii + 1];
+    const f3 pa = xyz(C), va = xyz(Vq);
+    const int orig = sIdx[ii];
+    f3 F = mk3(0.f, 0.f, 0.f);
+    int cnt = 0;
+    bool ovf = false;   // this lane's packet hit the stack bound: nothing is written for it
+    // node < 0: the packet's walk has ended (or it has no valid lane)
+    int node = (n >= 2 && (__ballot(valid) & pmask) != 0ull) ? n : -1;
+    int sp = 0;
+    const unsigned *words = reinterpret_cast<const unsigned *>(packed);
+#ifdef ARSIM_TRAVERSAL_STATS
+    const bool packetUsed = node >= 0;
+    unsigned long long stVisits = 0, stLeafTests = 0, stInterest = 0, stContacts = 0, stIters = 0;
+    int stMaxSp = 0;
+#endif
+    while (__ballot(node >= 0) != 0ull) {
+        const bool live = node >= 0;
+#ifdef ARSIM_TRAVERSAL_STATS
+        ++stIters;
+        if (live) ++stVisits;
+#endif
+        const unsigned w = live ? words[16 * (size_t)(node - n) + pl] : 0u;
+        const float4 a0 = make_float4(row_word<0>(w), row_word<1>(w), row_word<2>(w), 0.f);
+        const float4 a1 = make_float4(row_word<4>(w), row_word<5>(w), row_word<6>(w), 0.f);
+        const float4 b0 = make_float4(row_word<8>(w), row_word<9>(w), row_word<10>(w), 0.f);
+        const float4 b1 = make_float4(row_word<12>(w), row_word<13>(w), row_word<14>(w), 0.f);
+        const int L = __float_as_int(row_word<3>(w)), R = __float_as_int(row_word<7>(w));
+        const bool act = live && valid;
+        const bool oL = act & (L != i) & overlap(qMin, qMax, a0, a1);
+        const bool oR = act & (R != i) & overlap(qMin, qMax, b0, b1);
+#ifdef ARSIM_TRAVERSAL_STATS
+        stLeafTests += (L < n && oL ? 1 : 0) + (R < n && oR ? 1 : 0);
+        if (L < n && oL && in_contact(pa, xyz(cms[L]), ph)) ++stContacts;
+        if (R < n && oR && in_contact(pa, xyz(cms[R]), ph)) ++stContacts;
+        if (L >= n) stInterest += (unsigned long long)__popcll(__ballot(oL) & pmask);
+        if (R >= n) stInterest += (unsigned long long)__popcll(__ballot(oR) & pmask);
+#endif
+        if (L < n && oL) {
+            const float4 CL = cms[L];
+            if (candidate(CL, pa, ra, false, ph)) {
+                if (EMIT) {
+                    if (in_contact(pa, xyz(CL), ph)) {
+                        if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[L];
+                        ++cnt;
+                    }
+                } else {
+                    f3 fj;
+                    if (pair_force(pa, xyz(CL), va, xyz(sVel[L]), ph, fj)) F = add3(F, fj);
+                }
+            }
+        }
+        if (R < n && oR) {
+            const float4 CR = cms[R];
+            if (candidate(CR, pa, ra, false, ph)) {
+                if (EMIT) {
+                    if (in_contact(pa, xyz(CR), ph)) {
+                        if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[R];
+                        ++cnt;
+                    }
+                } else {
+                    f3 fj;
+                    if (pair_force(pa, xyz(CR), va, xyz(sVel[R]), ph, fj)) F = add3(F, fj);
+                }
+            }
+        }
+        const bool tL = L >= n && (__ballot(oL) & pmask) != 0ull;
+        const bool tR = R >= n && (__ballot(oR) & pmask) != 0ull;
+#ifdef ARSIM_TRAVERSAL_STATS
+        if (live && tL && tR) stMaxSp = sp + 1 > stMaxSp ? sp + 1 : stMaxSp;
+#endif
+        if (live) {
+            if (!tL && !tR) {
+                if (sp == 0) {
+                    node = -1;
+                } else {
+                    --sp;
+                    node = stack[sp];
+                }
+            } else if (tL && tR) {
+                if (sp == kWaveStackDepth) {
+                    if (valid) atomicOr(overflow, 1);
+                    ovf = true;
+                    node = -1;
+                } else {
+                    if (pl == 0) stack[sp] = R;
+                    ++sp;
+                    node = L;
+                }
+            } else {
+                node = tL ? L : R;
+            }
+        }
+    }
+#ifdef ARSIM_TRAVERSAL_STATS
+    {
+        // per-lane leaf tests and contacts summed over the wave; the packet counts from each packet's
+        // first lane, the wave counts from lane 0 (stInterest and stVisits are packet-uniform)
+        unsigned long long lt = stLeafTests, ct = stContacts;
+        for (int off = 32; off >= 1; off >>= 1) {
+            lt += __shfl_xor(lt, off, 64);
+            ct += __shfl_xor(ct, off, 64);
+        }
+        const unsigned long long lanes = (unsigned long long)__popcll(__ballot(valid));
+        const unsigned long long done = (unsigned long long)__popcll(__ballot(valid && !ovf));
+        if (pl == 0 && packetUsed) {
+            atomicAdd(&g_travStats[1], stVisits);
+            atomicAdd(&g_travStats[4], stInterest);
+            atomicMax(&g_travStats[5], (unsigned long long)stMaxSp);
+            atomicAdd(&g_travStats[8], 1ull);
+        }
+        if (lane == 0) {
+            atomicAdd(&g_travStats[0], 1ull);
+            atomicAdd(&g_travStats[2], lanes);
+            atomicAdd(&g_travStats[3], lt);
+            atomicAdd(&g_travStats[6], done);
+            atomicAdd(&g_travStats[7], ct);
+            atomicAdd(&g_travStats[9], stIters);
+        }
+    }
+#endif
+    if (!valid || ovf) return;
+    if (EMIT) {
+        o.counts[orig] = cnt;
+        return;
+    }
+    const SceneGeom &g = sv.single;
+    if (!((ph.flags & kGrainClamp) && ra < 0)) F = add3(F, wall_force(pa, va, ph, g));
+    if (ra >= 0) {
+        o.pForce[orig] = f4(F, 0.0f);
+        if (o.pTorque) o.pTorque[orig] = f4(cross3(xyz(o.relPos[orig]), F), 0.0f);
+    } else {
+        const float fs = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
+        o.vel[orig] = make_float4(va.x + F.x * fs, va.y + F.y * fs, va.z + F.z * fs, 0.0f);
+    }
+}
+
 }  // namespace
 
 #ifdef ARSIM_TRAVERSAL_STATS
-// Counting build only: copies the packet traversal's totals (8 counters, see g_travStats) and, when
+// Counting build only: copies the packet traversal's totals (10 counters, see g_travStats) and, when
 // reset is non-zero, clears them.
 extern "C" int arsim_debug_traversal_stats(unsigned long long *out, int reset) {
     if (out && hipMemcpyFromSymbol(out, HIP_SYMBOL(g_travStats), sizeof(g_travStats)) != hipSuccess) return 3;
     if (reset) {
-        const unsigned long long zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const unsigned long long zero[kTravStats] = {};
         if (hipMemcpyToSymbol(HIP_SYMBOL(g_travStats), zero, sizeof(zero)) != hipSuccess) return 3;
     }
     return hipDeviceSynchronize() == hipSuccess ? 0 : 3;
@@ -659,6 +829,19 @@ void launch_dem_lbvh(const int *sortedIdx, const Lbvh &t, int n, const DevPhys &
     if (n <= 0) return;
     if (block <= 0) block = 128;
     dim3 grid((n + block - 1) / block);
+    if (t.packed && !t.sceneRoot) {
+        // one scene: packets of 16 lanes
+        block = (block + 63) & ~63;
+        grid = dim3((n + block - 1) / block);
+        const size_t pl = (size_t)kWaveStackDepth * (block / kPacketLanes) * sizeof(int);
+        if (o.pairs)
+            hipLaunchKernelGGL(k_dem_lbvh_packets<true>, grid, dim3(block), pl, st, sortedIdx, t.packed, t.cms,
+                               t.sortedVel, t.bounds, n, ph, sv, o, overflow);
+        else
+            hipLaunchKernelGGL(k_dem_lbvh_packets<false>, grid, dim3(block), pl, st, sortedIdx, t.packed, t.cms,
+                               t.sortedVel, t.bounds, n, ph, sv, o, overflow);
+        return;
+    }
     if (t.packed) {
         block = (block + 63) & ~63;
         grid = dim3((n + block - 1) / block);
