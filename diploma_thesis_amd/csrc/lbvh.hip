@@ -453,173 +453,29 @@ __global__ void k_dem_lbvh(const int *__restrict__ sIdx, Lbvh t, int n, DevPhys 
     }
 }
 
-// Packet form of the same traversal: the 64 lanes of a wave (64 consecutive leaves in Morton
-// order, so spatially close) walk the tree together.  The wave visits a node when any of its lanes'
-// boxes overlaps it; the node id, the stack and the packed node loads are wave-uniform.  A lane
-// handles a leaf only when its own box overlaps the leaf's, and every ancestor's box contains the
-// leaf's box, so each lane meets exactly the leaves of its own traversal, in the same depth-first
-// order (left before right): the per-lane force sums are bit-identical.  Lanes of one wave that
-// start from different scene roots (batched forests) are served one root after another.
-template <bool EMIT>
-__global__ void k_dem_lbvh_wave(const int *__restrict__ sIdx, const float4 *__restrict__ packed,
-                                const float4 *__restrict__ cms, const float4 *__restrict__ sVel,
-                                const float4 *__restrict__ bounds, const int *__restrict__ sceneRoot, int n, DevPhys ph,
-                                SceneView sv, int sceneInVelW, DemOut o, int *overflow) {
-    extern __shared__ int wStack[];
-    const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
-    const int i = lb * (int)blockDim.x + (int)threadIdx.x;
-    const bool valid = i < n;
-    const int ii = valid ? i : n - 1;
-    int *stack = wStack + ((int)threadIdx.x >> 6) * kWaveStackDepth;
-    const float4 C = cms[ii];
-    const float4 Vq = sVel[ii];
-    const int ra = __float_as_int(C.w);
-    const int s = sceneInVelW ? __float_as_int(Vq.w) : 0;
-    const float4 qMin = bounds[2 * (size_t)ii], qMax = bounds[2 * (size_t)ii + 1];
-    const f3 pa = xyz(C), va = xyz(Vq);
-    const int orig = sIdx[ii];
-    const int myRoot = sceneRoot ? sceneRoot[s] : (n >= 2 ? n : -1);
-    f3 F = mk3(0.f, 0.f, 0.f);
-    int cnt = 0;
-    bool ovf = false;   // this lane's traversal hit the stack bound: nothing is written for it
-#ifdef ARSIM_TRAVERSAL_STATS
-    unsigned long long stVisits = 0, stLeafTests = 0, stInterest = 0, stContacts = 0;
-    int stMaxSp = 0;
-#endif
-    unsigned long long pending = __ballot(valid && myRoot >= n);
-    while (pending) {
-        const int root = __builtin_amdgcn_readfirstlane(__shfl(myRoot, __builtin_ctzll(pending)));
-        const bool act = valid && myRoot == root;
-        pending &= ~__ballot(act);
-        int node = root, sp = 0;
-        while (true) {
-            const float4 *q = packed + 4 * (size_t)(node - n);
-            const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3];
-            const int L = __builtin_amdgcn_readfirstlane(__float_as_int(a0.w));
-            const int R = __builtin_amdgcn_readfirstlane(__float_as_int(a1.w));
-            const bool oL = act & (L != i) & overlap(qMin, qMax, a0, a1);
-            const bool oR = act & (R != i) & overlap(qMin, qMax, b0, b1);
-#ifdef ARSIM_TRAVERSAL_STATS
-            ++stVisits;
-            stLeafTests += (L < n && oL ? 1 : 0) + (R < n && oR ? 1 : 0);
-            if (L < n && oL && in_contact(pa, xyz(cms[L]), ph)) ++stContacts;
-            if (R < n && oR && in_contact(pa, xyz(cms[R]), ph)) ++stContacts;
-            if (L >= n) stInterest += (unsigned long long)__popcll(__ballot(oL));
-            if (R >= n) stInterest += (unsigned long long)__popcll(__ballot(oR));
-#endif
-            if (L < n && oL) {
-                const float4 CL = cms[L];
-                if (candidate(CL, pa, ra, false, ph)) {
-                    if (EMIT) {
-                        if (in_contact(pa, xyz(CL), ph)) {
-                            if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[L];
-                            ++cnt;
-                        }
-                    } else {
-                        f3 fj;
-                        if (pair_force(pa, xyz(CL), va, xyz(sVel[L]), ph, fj)) F = add3(F, fj);
-                    }
-                }
-            }
-            if (R < n && oR) {
-                const float4 CR = cms[R];
-                if (candidate(CR, pa, ra, false, ph)) {
-                    if (EMIT) {
-                        if (in_contact(pa, xyz(CR), ph)) {
-                            if (cnt < o.maxPer) o.pairs[(size_t)orig * o.maxPer + cnt] = sIdx[R];
-                            ++cnt;
-                        }
-                    } else {
-                        f3 fj;
-                        if (pair_force(pa, xyz(CR), va, xyz(sVel[R]), ph, fj)) F = add3(F, fj);
-                    }
-                }
-            }
-            const bool tL = L >= n && __ballot(oL) != 0ull;
-            const bool tR = R >= n && __ballot(oR) != 0ull;
-            if (!tL && !tR) {
-                if (sp == 0) break;
-                --sp;
-                node = __builtin_amdgcn_readfirstlane(stack[sp]);
-            } else {
-                node = tL ? L : R;
-                if (tL && tR) {
-                    if (sp == kWaveStackDepth) {
-                        if (act) {
-                            atomicOr(overflow, 1);
-                            ovf = true;
-                        }
-                        break;
-                    }
-                    stack[sp] = R;
-                    ++sp;
-#ifdef ARSIM_TRAVERSAL_STATS
-                    stMaxSp = sp > stMaxSp ? sp : stMaxSp;
-#endif
-                }
-            }
-        }
-    }
-#ifdef ARSIM_TRAVERSAL_STATS
-    {
-        // stLeafTests / stContacts are per lane; the others are wave-uniform
-        unsigned long long lt = stLeafTests, ct = stContacts;
-        for (int off = 32; off >= 1; off >>= 1) {
-            lt += __shfl_xor(lt, off, 64);
-            ct += __shfl_xor(ct, off, 64);
-        }
-        const unsigned long long lanes = (unsigned long long)__popcll(__ballot(valid));
-        const unsigned long long done = (unsigned long long)__popcll(__ballot(valid && !ovf));
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&g_travStats[0], 1ull);
-            atomicAdd(&g_travStats[1], stVisits);
-            atomicAdd(&g_travStats[2], lanes);
-            atomicAdd(&g_travStats[3], lt);
-            atomicAdd(&g_travStats[4], stInterest);
-            atomicMax(&g_travStats[5], (unsigned long long)stMaxSp);
-            atomicAdd(&g_travStats[6], done);
-            atomicAdd(&g_travStats[7], ct);
-            atomicAdd(&g_travStats[8], 1ull);
-            atomicAdd(&g_travStats[9], stVisits);
-        }
-    }
-#endif
-    if (!valid || ovf) return;
-    if (EMIT) {
-        o.counts[orig] = cnt;
-        return;
-    }
-    SceneGeom g;
-    load_geom(sv, s, g);
-    if (!((ph.flags & kGrainClamp) && ra < 0)) F = add3(F, wall_force(pa, va, ph, g));
-    if (ra >= 0) {
-        o.pForce[orig] = f4(F, 0.0f);
-        if (o.pTorque) o.pTorque[orig] = f4(cross3(xyz(o.relPos[orig]), F), 0.0f);
-    } else {
-        const float fs = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
-        o.vel[orig] = make_float4(va.x + F.x * fs, va.y + F.y * fs, va.z + F.z * fs, 0.0f);
-    }
-}
-
-// Packets of 16 lanes, four per wave (one scene): each 16 consecutive leaves in Morton order walk the
-// tree together as in k_dem_lbvh_wave, with the packet's own node, LDS stack (kWaveStackDepth entries)
-// and depth.  A smaller packet's lanes are closer together, so it enters fewer nodes that none of its
-// lanes overlaps (at the 2^20-particle scene a 64-lane packet enters 157 nodes, a 16-lane packet 78,
-// and the wave, which runs as long as its slowest packet, 95).  A visit loads the packet's 64-byte node
-// record as one dword per lane and passes the words round the packet by DPP row broadcasts.  Each lane
-// meets exactly the leaves of its own traversal in the same order, as in k_dem_lbvh_wave, so the force
-// sums are the same bits.
+// Packet form of the same traversal: 16 consecutive leaves in Morton order (four packets per wave)
+// walk the tree together, with the packet's own node, LDS stack (kWaveStackDepth entries) and depth.
+// The packet visits a node when any of its lanes' boxes overlaps it.  A lane handles a leaf only when
+// its own box overlaps the leaf's, and every ancestor's box contains the leaf's box, so each lane meets
+// exactly the leaves of its own traversal, in the same depth-first order (left before right): the
+// per-lane force sums are bit-identical.  A smaller packet's lanes are closer together, so it enters
+// fewer nodes that none of its lanes overlaps (at the 2^20-particle scene a 64-lane packet enters 157
+// nodes, a 16-lane packet 78, and the wave, which runs as long as its slowest packet, 95).  A visit
+// loads the packet's 64-byte node record as one dword per lane and passes the words round the packet
+// by DPP row broadcasts.
 constexpr int kPacketLanes = 16;
 // Word K of each 16-lane row (DPP row_newbcast: every lane of a row receives lane K's value), as a float.
 template <int K>
 __device__ __forceinline__ float row_word(unsigned w) {
     return __uint_as_float((unsigned)__builtin_amdgcn_update_dpp(0, (int)w, 0x150 + K, 0xf, 0xf, false));
 }
-template <bool EMIT>
+// FOREST: scenes' trees under their own roots (scene id in sortedVel.w); a packet whose lanes start at
+// different roots walks them one root after another, the lanes of the current root taking part.
+template <bool EMIT, bool FOREST>
 __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *__restrict__ packed,
                                    const float4 *__restrict__ cms, const float4 *__restrict__ sVel,
-                                   const float4 *__restrict__ bounds, int n, DevPhys ph, SceneView sv, DemOut o,
-                                   int *overflow) {
+                                   const float4 *__restrict__ bounds, const int *__restrict__ sceneRoot, int n,
+                                   DevPhys ph, SceneView sv, DemOut o, int *overflow) {
     extern __shared__ int pStack[];
     const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int i = lb * (int)blockDim.x + (int)threadIdx.x;
@@ -638,16 +494,35 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
     f3 F = mk3(0.f, 0.f, 0.f);
     int cnt = 0;
     bool ovf = false;   // this lane's packet hit the stack bound: nothing is written for it
-    // node < 0: the packet's walk has ended (or it has no valid lane)
-    int node = (n >= 2 && (__ballot(valid) & pmask) != 0ull) ? n : -1;
-    int sp = 0;
+    const int s = FOREST ? __float_as_int(Vq.w) : 0;
+    const int myRoot = FOREST ? sceneRoot[s] : (n >= 2 ? n : -1);
+    // the packet's lanes whose root has not been walked yet; node < 0: no walk under way
+    unsigned long long pending = __ballot(valid && myRoot >= n) & pmask;
+    int root = -1, node = -1, sp = 0;
+    if (!FOREST && pending != 0ull) {
+        root = node = n;
+        pending = 0ull;
+    }
     const unsigned *words = reinterpret_cast<const unsigned *>(packed);
 #ifdef ARSIM_TRAVERSAL_STATS
-    const bool packetUsed = node >= 0;
+    const bool packetUsed = pending != 0ull || node >= 0;
     unsigned long long stVisits = 0, stLeafTests = 0, stInterest = 0, stContacts = 0, stIters = 0;
     int stMaxSp = 0;
 #endif
-    while (__ballot(node >= 0) != 0ull) {
+    while (true) {
+        if (FOREST) {
+            // a packet without a walk starts the next pending root (all lanes take part in the exchange)
+            const bool needRoot = node < 0 && pending != 0ull;
+            const int first = needRoot ? __builtin_ctzll(pending) : lane;
+            const int cand = __shfl(myRoot, first, 64);
+            if (needRoot) {
+                root = cand;
+                node = cand;
+                sp = 0;
+            }
+            pending &= ~(__ballot(needRoot && valid && myRoot == cand) & pmask);
+        }
+        if (__ballot(node >= 0) == 0ull) break;
         const bool live = node >= 0;
 #ifdef ARSIM_TRAVERSAL_STATS
         ++stIters;
@@ -659,7 +534,7 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
         const float4 b0 = make_float4(row_word<8>(w), row_word<9>(w), row_word<10>(w), 0.f);
         const float4 b1 = make_float4(row_word<12>(w), row_word<13>(w), row_word<14>(w), 0.f);
         const int L = __float_as_int(row_word<3>(w)), R = __float_as_int(row_word<7>(w));
-        const bool act = live && valid;
+        const bool act = live && valid && myRoot == root;
         const bool oL = act & (L != i) & overlap(qMin, qMax, a0, a1);
         const bool oR = act & (R != i) & overlap(qMin, qMax, b0, b1);
 #ifdef ARSIM_TRAVERSAL_STATS
@@ -712,8 +587,10 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
                 }
             } else if (tL && tR) {
                 if (sp == kWaveStackDepth) {
-                    if (valid) atomicOr(overflow, 1);
-                    ovf = true;
+                    if (act) {
+                        atomicOr(overflow, 1);
+                        ovf = true;
+                    }
                     node = -1;
                 } else {
                     if (pl == 0) stack[sp] = R;
@@ -757,7 +634,9 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
         o.counts[orig] = cnt;
         return;
     }
-    const SceneGeom &g = sv.single;
+    SceneGeom g;
+    if (FOREST) g = sv.table[s];
+    else g = sv.single;
     if (!((ph.flags & kGrainClamp) && ra < 0)) F = add3(F, wall_force(pa, va, ph, g));
     if (ra >= 0) {
         o.pForce[orig] = f4(F, 0.0f);
@@ -829,30 +708,22 @@ void launch_dem_lbvh(const int *sortedIdx, const Lbvh &t, int n, const DevPhys &
     if (n <= 0) return;
     if (block <= 0) block = 128;
     dim3 grid((n + block - 1) / block);
-    if (t.packed && !t.sceneRoot) {
-        // one scene: packets of 16 lanes
+    if (t.packed) {
+        // packets of 16 lanes, one scene or a forest (whose scene ids ride in sortedVel.w)
         block = (block + 63) & ~63;
         grid = dim3((n + block - 1) / block);
         const size_t pl = (size_t)kWaveStackDepth * (block / kPacketLanes) * sizeof(int);
-        if (o.pairs)
-            hipLaunchKernelGGL(k_dem_lbvh_packets<true>, grid, dim3(block), pl, st, sortedIdx, t.packed, t.cms,
-                               t.sortedVel, t.bounds, n, ph, sv, o, overflow);
-        else
-            hipLaunchKernelGGL(k_dem_lbvh_packets<false>, grid, dim3(block), pl, st, sortedIdx, t.packed, t.cms,
-                               t.sortedVel, t.bounds, n, ph, sv, o, overflow);
-        return;
-    }
-    if (t.packed) {
-        block = (block + 63) & ~63;
-        grid = dim3((n + block - 1) / block);
-        const size_t wl = (size_t)kWaveStackDepth * (block / 64) * sizeof(int);
-        const int w = sceneInVelW ? 1 : 0;
-        if (o.pairs)
-            hipLaunchKernelGGL(k_dem_lbvh_wave<true>, grid, dim3(block), wl, st, sortedIdx, t.packed, t.cms,
-                               t.sortedVel, t.bounds, t.sceneRoot, n, ph, sv, w, o, overflow);
-        else
-            hipLaunchKernelGGL(k_dem_lbvh_wave<false>, grid, dim3(block), wl, st, sortedIdx, t.packed, t.cms,
-                               t.sortedVel, t.bounds, t.sceneRoot, n, ph, sv, w, o, overflow);
+        auto run = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, grid, dim3(block), pl, st, sortedIdx, t.packed, t.cms, t.sortedVel, t.bounds,
+                               t.sceneRoot, n, ph, sv, o, overflow);
+        };
+        if (t.sceneRoot) {
+            if (o.pairs) run(k_dem_lbvh_packets<true, true>);
+            else run(k_dem_lbvh_packets<false, true>);
+        } else {
+            if (o.pairs) run(k_dem_lbvh_packets<true, false>);
+            else run(k_dem_lbvh_packets<false, false>);
+        }
         return;
     }
     size_t lds = (size_t)kStackDepth * block * sizeof(int);
