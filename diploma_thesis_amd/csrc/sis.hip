@@ -16,7 +16,7 @@ namespace {
 
 constexpr int kSisBlock = 1024;   // one workgroup per scene for levels / buckets / sweeps
 constexpr int kSisWaves = kSisBlock / 64;
-constexpr int kSisLdsBodies = 2048;   // 64 KiB of (vel, ang) per scene in LDS
+constexpr int kSisLdsBodies = 2048;   // 136 KiB of (vel, ang) and inverse inertia per scene in LDS
 
 __device__ __forceinline__ float dot4(float4 a, float4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
 
@@ -471,19 +471,22 @@ __device__ __forceinline__ f3 gnormalize(f3 v) {
 }
 __device__ __forceinline__ f3 divs3(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
 
-// Where the sweeps keep the bodies' linear / angular velocities: global memory, or a copy of the
-// scene's bodies in LDS (index b - b0).
+// Where the sweeps keep the bodies' linear / angular velocities and read their inverse world-frame
+// inertia (fixed over the sweeps): global memory, or a copy of the scene's bodies in LDS (index b - b0).
 struct VelGlobal {
     float4 *vel, *ang;
+    const float *Iw;
     __device__ float4 v(int b) const { return vel[b]; }
     __device__ float4 w(int b) const { return ang[b]; }
     __device__ void set(int b, f3 lv, f3 av) const {
         vel[b] = f4(lv, 0.0f);
         ang[b] = f4(av, 0.0f);
     }
+    __device__ void inertia(int b, float I[9]) const { ld_mat(Iw, b, I); }
 };
 struct VelLds {
     float4 *s;   // (vel, ang) pairs
+    const float *I;   // 9 per body
     int b0;
     __device__ float4 v(int b) const { return s[2 * (b - b0)]; }
     __device__ float4 w(int b) const { return s[2 * (b - b0) + 1]; }
@@ -491,6 +494,7 @@ struct VelLds {
         s[2 * (b - b0)] = f4(lv, 0.0f);
         s[2 * (b - b0) + 1] = f4(av, 0.0f);
     }
+    __device__ void inertia(int b, float Im[9]) const { ld_mat(I, b - b0, Im); }
 };
 
 // Sweep records in level order.  k_sis_prep writes, for position q of order[] (contact c =
@@ -499,11 +503,11 @@ struct VelLds {
 // of kSisPlanes float4 (plane k at rec + k * cap):
 //   0: (n, bias)   1: (p1, mc)   2: static (t, kt), body (p2, 0)   3: static (t2, kt2)
 //   4: (b1 bits, b2 bits, m1, static: fb / body: m2)
-//   5..9: I1 (9 floats) then I2 (9 floats), packed
-//   10: the accumulated impulses (normal, friction, friction 2, 0)
+//   5: the accumulated impulses (normal, friction, friction 2, 0)
 // mc already has the |mc| < 1e-5 -> 1 substitution applied.  A sweep then reads its level's records
-// with coalesced loads (a level's positions are consecutive) instead of gathering by contact.
-constexpr int kSisPlanes = 11;
+// with coalesced loads (a level's positions are consecutive) instead of gathering by contact; the
+// bodies' inverse inertia, the same over the sweeps, comes from the bodies (in LDS when they fit).
+constexpr int kSisPlanes = 6;
 __global__ void k_sis_prep(SisContacts ct, SisBodies bd, SisSolvePhys ph, const int *__restrict__ numContacts,
                            const int *__restrict__ order, int cap, float4 *__restrict__ rec) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -547,12 +551,7 @@ __global__ void k_sis_prep(SisContacts ct, SisBodies bd, SisSolvePhys ph, const 
         rec[2 * P + q] = f4(p2, 0.0f);
         rec[4 * P + q] = make_float4(__int_as_float(bb.x), __int_as_float(bb.y), m1, m2);
     }
-    rec[5 * P + q] = make_float4(I1[0], I1[1], I1[2], I1[3]);
-    rec[6 * P + q] = make_float4(I1[4], I1[5], I1[6], I1[7]);
-    rec[7 * P + q] = make_float4(I1[8], I2[0], I2[1], I2[2]);
-    rec[8 * P + q] = make_float4(I2[3], I2[4], I2[5], I2[6]);
-    rec[9 * P + q] = make_float4(I2[7], I2[8], 0.0f, 0.0f);
-    rec[10 * P + q] = ct.impulse[c];
+    rec[5 * P + q] = ct.impulse[c];
 }
 
 // One contact of one sweep, record q: SequentialImpulseSolver (Particles/updateFunctions.cpp:1580-1874).
@@ -565,12 +564,12 @@ __global__ void k_sis_prep(SisContacts ct, SisBodies bd, SisSolvePhys ph, const 
 template <class VA>
 __device__ void sis_contact(size_t q, float4 *__restrict__ rec, size_t P, const VA &va) {
     const float4 r0 = rec[q], r1 = rec[P + q], r2 = rec[2 * P + q], r4 = rec[4 * P + q];
-    const float4 i5 = rec[5 * P + q], i6 = rec[6 * P + q], i7 = rec[7 * P + q];
-    float4 acc = rec[10 * P + q];
+    float4 acc = rec[5 * P + q];
     const f3 n = xyz(r0);
     const float bias = r0.w, mc = r1.w;
     const int b1 = __float_as_int(r4.x), b2 = __float_as_int(r4.y);
-    const float I1[9] = {i5.x, i5.y, i5.z, i5.w, i6.x, i6.y, i6.z, i6.w, i7.x};
+    float I1[9];
+    va.inertia(b1, I1);
     const float upper = 100.0f;
     if (b2 < 0) {
         const float4 r3 = rec[3 * P + q];
@@ -602,8 +601,8 @@ __device__ void sis_contact(size_t q, float4 *__restrict__ rec, size_t P, const 
         w = add3(w, mat_vec(I1, cross3(p, J)));
         va.set(b1, v, w);
     } else {
-        const float4 i8 = rec[8 * P + q], i9 = rec[9 * P + q];
-        const float I2[9] = {i7.y, i7.z, i7.w, i8.x, i8.y, i8.z, i8.w, i9.x, i9.y};
+        float I2[9];
+        va.inertia(b2, I2);
         const f3 p1 = xyz(r1), p2 = xyz(r2);
         const float m1 = r4.z, m2 = r4.w;
         f3 v1 = xyz(va.v(b1)), w1 = xyz(va.w(b1)), v2 = xyz(va.v(b2)), w2 = xyz(va.w(b2));
@@ -622,12 +621,12 @@ __device__ void sis_contact(size_t q, float4 *__restrict__ rec, size_t P, const 
         w2 = sub3(w2, mat_vec(I2, cross3(p2, J)));
         va.set(b2, v2, w2);
     }
-    rec[10 * P + q] = acc;
+    rec[5 * P + q] = acc;
 }
 
 // The sweeps of one scene (one workgroup per scene): level by level, a barrier between levels.
-// LDS: the scene's bodies' velocities live in LDS for the whole kernel (copied in and out), so a
-// level's round trip is an LDS access instead of a global one.
+// LDS: the scene's bodies' velocities and inverse inertia live in LDS for the whole kernel (copied in,
+// velocities copied out), so a level's round trip is an LDS access instead of a global one.
 template <bool LDS>
 __global__ __launch_bounds__(kSisBlock) void k_sis_solve(float4 *__restrict__ rec, int cap, float4 *__restrict__ impulse,
                                                          SisBodies bd, const int2 *__restrict__ sceneRange,
@@ -643,15 +642,17 @@ __global__ __launch_bounds__(kSisBlock) void k_sis_solve(float4 *__restrict__ re
     const int *off = offAll + rg.x + s;
     const size_t P = (size_t)cap;
     const int2 br = sceneBodies ? sceneBodies[s] : make_int2(0, numBodies);
+    float *sI = reinterpret_cast<float *>(sVel + 2 * (size_t)br.y);   // after the (vel, ang) pairs
     if (LDS) {
         for (int k = threadIdx.x; k < br.y; k += kSisBlock) {
             sVel[2 * k] = bd.vel[br.x + k];
             sVel[2 * k + 1] = bd.ang[br.x + k];
         }
+        for (int k = threadIdx.x; k < 9 * br.y; k += kSisBlock) sI[k] = bd.Iw[9 * (size_t)br.x + k];
         __syncthreads();
     }
-    const VelLds vl{sVel, br.x};
-    const VelGlobal vg{bd.vel, bd.ang};
+    const VelLds vl{sVel, sI, br.x};
+    const VelGlobal vg{bd.vel, bd.ang, bd.Iw};
     for (int it = 0; it < iterations; ++it)
         for (int L = 0; L < levels; ++L) {
             const int a = off[L], e = off[L + 1];
@@ -666,7 +667,7 @@ __global__ __launch_bounds__(kSisBlock) void k_sis_solve(float4 *__restrict__ re
             bd.vel[br.x + k] = sVel[2 * k];
             bd.ang[br.x + k] = sVel[2 * k + 1];
         }
-    for (int k = rg.x + (int)threadIdx.x; k < rg.y; k += kSisBlock) impulse[order[k]] = rec[10 * P + k];
+    for (int k = rg.x + (int)threadIdx.x; k < rg.y; k += kSisBlock) impulse[order[k]] = rec[5 * P + k];
 }
 
 }  // namespace
@@ -776,7 +777,8 @@ int launch_sis_solve(const SisContacts &ct, const SisBodies &bd, const SisSolveP
                            cap, w.prep);
         if (maxSceneBodies <= kSisLdsBodies)
             hipLaunchKernelGGL(k_sis_solve<true>, dim3(numScenes), dim3(kSisBlock),
-                               (size_t)maxSceneBodies * 2 * sizeof(float4), st, w.prep, cap, ct.impulse, bd, rg,
+                               (size_t)maxSceneBodies * (2 * sizeof(float4) + 9 * sizeof(float)), st, w.prep, cap,
+                               ct.impulse, bd, rg,
                                w.numContacts, sceneBodies, numBodies, w.maxLvl, w.off, w.order, iterations);
         else
             hipLaunchKernelGGL(k_sis_solve<false>, dim3(numScenes), dim3(kSisBlock), 0, st, w.prep, cap, ct.impulse,
