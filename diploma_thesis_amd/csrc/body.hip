@@ -77,7 +77,10 @@ __global__ void k_body_integrate(BodyBufs b, int B, DevPhys ph, ZeroJobs z, int 
 // computeGlobalAttributes (rigidBodyKernels.cu:386-421) for body particles, integrate_functor
 // (particleKernels.cu:50-129) for grains, then the broadphase key of the updated position:
 // calcHashD (particleKernels.cu:170-189) or generateMortonCodes (BVHkernels.cu:670-716).
-template <int MODE, bool SCENE_UNIFORM = false>
+// Where pose_key_one takes the scene geometry from: each particle's scene (per lane), the table entry
+// of scene `scene` for all of the caller's particles (scalar loads), or the one scene passed by value.
+enum { kPoseGeomLane = 0, kPoseGeomScene = 1, kPoseGeomSingle = 2 };
+template <int MODE, int GSRC = kPoseGeomLane>
 __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
                                                  KeyOut ko, int doBodies, int doGrains, unsigned *shist,
                                                  int scene = 0);
@@ -97,7 +100,7 @@ __device__ __forceinline__ void hist_add(unsigned *h, unsigned d) {
     if ((unsigned)(__ffsll((long long)peers) - 1) == lane) atomicAdd(&h[d], (unsigned)__popcll(peers));
 }
 
-template <int MODE>
+template <int MODE, int GSRC>
 __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv, KeyOut ko,
                                                   int doBodies, int doGrains) {
     // digit histograms of this block's keys for every sort pass (when the sort asked for them)
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N,
     // are flushed into the global ones: the flush is a global atomic per non-empty bin
     const int stride = (int)(gridDim.x * blockDim.x);
     for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < N; i += stride)
-        pose_key_one<MODE>(i, p, b, N, ph, sv, ko, doBodies, doGrains, hist ? shist : nullptr);
+        pose_key_one<MODE, GSRC>(i, p, b, N, ph, sv, ko, doBodies, doGrains, hist ? shist : nullptr);
     if (hist) {
         __syncthreads();
         for (int k = threadIdx.x; k < ko.histPasses * 256; k += blockDim.x)
@@ -119,17 +122,18 @@ __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N,
     }
 }
 
-// Returns the key (0 without one); a grid key is stored only when ko.keys is set.  SCENE_UNIFORM: the
-// caller's particles all belong to scene `scene`, whose geometry is then read through scalar loads
-// from the scene table (which holds every scene, also a single one).
-template <int MODE, bool SCENE_UNIFORM>
+// Returns the key (0 without one); a grid key is stored only when ko.keys is set.  GSRC: where the
+// scene geometry comes from (kPoseGeomScene: the caller's particles all belong to scene `scene`, read
+// from the scene table, which holds every scene of an engine, also a single one).
+template <int MODE, int GSRC>
 __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
                                                  KeyOut ko, int doBodies, int doGrains, unsigned *shist, int scene) {
     if (i >= N) return 0;
     const int rbi = p.rb ? p.rb[i] : -1;
-    const int s = SCENE_UNIFORM ? scene : (p.scene ? (int)p.scene[i] : 0);
+    const int s = GSRC == kPoseGeomScene ? scene : GSRC == kPoseGeomSingle ? 0 : (p.scene ? (int)p.scene[i] : 0);
     SceneGeom g;
-    if (SCENE_UNIFORM) g = sv.table[scene];
+    if (GSRC == kPoseGeomScene) g = sv.table[scene];
+    else if (GSRC == kPoseGeomSingle) g = sv.single;
     else load_geom(sv, s, g);
     float4 P, Vfin;
     bool haveV = false;
@@ -269,7 +273,8 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
         const long long i = waveBase + k * 64 + lane;
-        keys[k] = i < tileEnd ? (unsigned)pose_key_one<kKeyGrid, true>((int)i, p, b, N, ph, sv, ko, 1, 1, nullptr, scene)
+        keys[k] = i < tileEnd ? (unsigned)pose_key_one<kKeyGrid, kPoseGeomScene>((int)i, p, b, N, ph, sv, ko, 1, 1,
+                                                                                  nullptr, scene)
                               : 0u;
     }
     __syncthreads();
@@ -544,21 +549,27 @@ void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, con
     const long long cap = need <= 4096 ? 512 : 1024;
     dim3 grid((unsigned)(ko.hist && mode != kKeyNone && need > cap ? cap : need));
     int db = doBodies ? 1 : 0, dg = doGrains ? 1 : 0;
+    // one scene: its geometry passed by value (kernel arguments, scalar registers); several: per lane
+    const bool single = sv.count <= 1;
+    auto run = [&](auto one, auto many) {
+        if (single) hipLaunchKernelGGL(one, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+        else hipLaunchKernelGGL(many, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+    };
     switch (mode) {
         case kKeyGrid:
-            hipLaunchKernelGGL(k_pose_key<kKeyGrid>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+            run(k_pose_key<kKeyGrid, kPoseGeomSingle>, k_pose_key<kKeyGrid, kPoseGeomLane>);
             break;
         case kKeyMorton30:
-            hipLaunchKernelGGL(k_pose_key<kKeyMorton30>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+            run(k_pose_key<kKeyMorton30, kPoseGeomSingle>, k_pose_key<kKeyMorton30, kPoseGeomLane>);
             break;
         case kKeyMorton63:
-            hipLaunchKernelGGL(k_pose_key<kKeyMorton63>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+            run(k_pose_key<kKeyMorton63, kPoseGeomSingle>, k_pose_key<kKeyMorton63, kPoseGeomLane>);
             break;
         case kKeyForest30:
-            hipLaunchKernelGGL(k_pose_key<kKeyForest30>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+            run(k_pose_key<kKeyForest30, kPoseGeomSingle>, k_pose_key<kKeyForest30, kPoseGeomLane>);
             break;
         default:
-            hipLaunchKernelGGL(k_pose_key<kKeyNone>, grid, dim3(block), 0, st, p, b, numParticles, ph, sv, ko, db, dg);
+            run(k_pose_key<kKeyNone, kPoseGeomSingle>, k_pose_key<kKeyNone, kPoseGeomLane>);
             break;
     }
 }
