@@ -38,6 +38,9 @@ ALGO_BYTES = {
     "bucket_sort_reorder": (72.0 + 2 * 16.0, 4.0, 0.0, 0.0),
     # the grid DEM fused with the per-body reduction and apply
     "dem_reduce": (120.0, 8.0, 32.0, 240.0),
+    # LBVH 30-bit codes: the sort within each bucket of the top 10 bits (tile-ranked keys and indices in,
+    # sorted keys and indices out)
+    "bucket_sort": (16.0, 0.0, 0.0, 0.0),
 }
 HBM_PEAK_GBS = 8000.0
 

@@ -205,7 +205,7 @@ __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, 
         float u = mnorm(P.x, lo0, e0), v = mnorm(P.y, lo1, e1), w = mnorm(P.z, lo2, e2);
         if (MODE == kKeyMorton30) {
             key = morton30(u, v, w);
-            ((unsigned *)ko.keys)[i] = (unsigned)key;
+            if (ko.keys) ((unsigned *)ko.keys)[i] = (unsigned)key;
         } else if (MODE == kKeyMorton63) {
             key = morton63(u, v, w);
             ((uint64_t *)ko.keys)[i] = key;
@@ -248,7 +248,7 @@ __device__ __forceinline__ uint64_t match_bits(unsigned d, bool ok) {
     return peers;
 }
 
-template <int THREADS, int BITS>
+template <int MODE, int THREADS, int BITS>
 __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
                                                             TileKeyOut to, int grainVelDead) {
     constexpr int WAVES = THREADS / 64;
@@ -273,8 +273,8 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
         const long long i = waveBase + k * 64 + lane;
-        keys[k] = i < tileEnd ? (unsigned)pose_key_one<kKeyGrid, kPoseGeomScene>((int)i, p, b, N, ph, sv, ko, 1, 1,
-                                                                                  nullptr, scene)
+        keys[k] = i < tileEnd ? (unsigned)pose_key_one<MODE, kPoseGeomScene>((int)i, p, b, N, ph, sv, ko, 1, 1, nullptr,
+                                                                              scene)
                               : 0u;
     }
     __syncthreads();
@@ -579,14 +579,23 @@ void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticle
     if (numParticles <= 0) return;
     const int g = grainVelDead ? 1 : 0;
     const dim3 grid(to.numTiles);
+    if (to.morton) {   // 30-bit Morton keys, buckets of their top 10 bits
+        hipLaunchKernelGGL((k_pose_key_tiles<kKeyMorton30, 1024, 10>), grid, dim3(1024), 0, st, p, b, numParticles, ph,
+                           sv, to, g);
+        return;
+    }
     if (to.tileKeys == 1024)
-        hipLaunchKernelGGL((k_pose_key_tiles<256, 8>), grid, dim3(256), 0, st, p, b, numParticles, ph, sv, to, g);
+        hipLaunchKernelGGL((k_pose_key_tiles<kKeyGrid, 256, 8>), grid, dim3(256), 0, st, p, b, numParticles, ph, sv, to,
+                           g);
     else if (to.bucketBits <= 8)
-        hipLaunchKernelGGL((k_pose_key_tiles<1024, 8>), grid, dim3(1024), 0, st, p, b, numParticles, ph, sv, to, g);
+        hipLaunchKernelGGL((k_pose_key_tiles<kKeyGrid, 1024, 8>), grid, dim3(1024), 0, st, p, b, numParticles, ph, sv,
+                           to, g);
     else if (to.bucketBits == 9)
-        hipLaunchKernelGGL((k_pose_key_tiles<1024, 9>), grid, dim3(1024), 0, st, p, b, numParticles, ph, sv, to, g);
+        hipLaunchKernelGGL((k_pose_key_tiles<kKeyGrid, 1024, 9>), grid, dim3(1024), 0, st, p, b, numParticles, ph, sv,
+                           to, g);
     else
-        hipLaunchKernelGGL((k_pose_key_tiles<1024, 10>), grid, dim3(1024), 0, st, p, b, numParticles, ph, sv, to, g);
+        hipLaunchKernelGGL((k_pose_key_tiles<kKeyGrid, 1024, 10>), grid, dim3(1024), 0, st, p, b, numParticles, ph, sv,
+                           to, g);
 }
 
 void launch_reduce_apply(const float4 *pForce, const float4 *pTorque, const float4 *relPos, const BodyBufs &b,

@@ -154,7 +154,12 @@ __device__ __forceinline__ void block_fill(unsigned *table, unsigned long long a
         *reinterpret_cast<uint4 *>(table + k) = v4;
 }
 
-// eight waves per SIMD (two workgroups per CU): registers are held to that occupancy
+// POSBITS: bits of a bucket position in the packed LDS words (the offset from c0 takes the other 32 -
+// POSBITS; a bucket sorted in LDS holds at most 2^POSBITS keys).  REORDER: also the gathered sorted
+// copies and the cell table (grid keys); without it only the sorted keys and indices (Morton keys,
+// whose sorted-copy arrays are then only the global fallback's second buffer).
+// Eight waves per SIMD (two workgroups per CU): registers are held to that occupancy.
+template <int POSBITS, bool REORDER>
 __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bucket_sort(
     TileKeyOut to, int split, unsigned *__restrict__ kOut, unsigned *__restrict__ vOut,
     const float4 *__restrict__ rec, float4 *__restrict__ sPos, float4 *__restrict__ sVel,
@@ -175,7 +180,9 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
     const unsigned *hist = to.hist + bs.bucketBase;
     const unsigned start = bs.pStart + block_sum(t < j ? hist[t] : 0u, sWave);
     const int cnt = (int)hist[j];
-    const bool inLds = cnt <= kBkCap;
+    constexpr int kCap = kBkCap < (1 << POSBITS) ? kBkCap : (1 << POSBITS);
+    constexpr unsigned kPosMask = (1u << POSBITS) - 1u;
+    const bool inLds = cnt <= kCap;
     // a bucket too large for LDS is sorted in global memory by its first workgroup alone
     if (!inLds && part != 0) return;
     const int nparts = inLds ? split : 1;
@@ -188,7 +195,7 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
                                   : c0 + (1ull << lowBits) < sceneEnd ? c0 + (1ull << lowBits) : sceneEnd;
     if (cnt == 0) {
         const unsigned long long n = c1 - c0;
-        block_fill(table, c0 + n * part / split, c0 + n * (part + 1) / split, start);
+        if (REORDER) block_fill(table, c0 + n * part / split, c0 + n * (part + 1) / split, start);
         return;
     }
     // The bucket's keys are the runs of bucket j in the scene's tiles, in tile order: tile k's run
@@ -238,11 +245,12 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
     };
     const unsigned base = (unsigned)c0;
     if (inLds) {
-        for (int i = t; i < cnt; i += kBkThreads) sA[i] = ((tKeys[slot((unsigned)i)] - base) << 16) | (unsigned)i;
+        for (int i = t; i < cnt; i += kBkThreads)
+            sA[i] = ((tKeys[slot((unsigned)i)] - base) << POSBITS) | (unsigned)i;
         __syncthreads();
         unsigned *src = sA, *dst = sB;
         for (int sh = 0; sh < lowBits; sh += 8) {
-            stable_pass(LdsItems{src}, LdsItems{dst}, cnt, 16 + sh, min(8, lowBits - sh), sCur, sWave);
+            stable_pass(LdsItems{src}, LdsItems{dst}, cnt, POSBITS + sh, min(8, lowBits - sh), sCur, sWave);
             unsigned *x = src;
             src = dst;
             dst = x;
@@ -254,29 +262,40 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
         const int ia = i0 + t, ib = ia + kBkThreads;
         const bool ha = ia < i1, hb = ib < i1;
         const unsigned wa = sorted[ha ? ia : 0], wb = sorted[hb ? ib : 0];
-        const unsigned va = tVals[slot(wa & 0xffffu)], vb = tVals[slot(wb & 0xffffu)];
-        const float4 pa = rec[2 * (size_t)va], qa = rec[2 * (size_t)va + 1];
-        const float4 pb = rec[2 * (size_t)vb], qb = rec[2 * (size_t)vb + 1];
-        fill_table([&](int i) -> unsigned { return sorted[i] >> 16; });
+        const unsigned va = tVals[slot(wa & kPosMask)], vb = tVals[slot(wb & kPosMask)];
+        float4 pa, qa, pb, qb;
+        if (REORDER) {
+            pa = rec[2 * (size_t)va];
+            qa = rec[2 * (size_t)va + 1];
+            pb = rec[2 * (size_t)vb];
+            qb = rec[2 * (size_t)vb + 1];
+            fill_table([&](int i) -> unsigned { return sorted[i] >> POSBITS; });
+        }
         if (ha) {
-            kOut[start + ia] = base + (wa >> 16);
+            kOut[start + ia] = base + (wa >> POSBITS);
             vOut[start + ia] = va;
-            sPos[start + ia] = pa;
-            sVel[start + ia] = qa;
+            if (REORDER) {
+                sPos[start + ia] = pa;
+                sVel[start + ia] = qa;
+            }
         }
         if (hb) {
-            kOut[start + ib] = base + (wb >> 16);
+            kOut[start + ib] = base + (wb >> POSBITS);
             vOut[start + ib] = vb;
-            sPos[start + ib] = pb;
-            sVel[start + ib] = qb;
+            if (REORDER) {
+                sPos[start + ib] = pb;
+                sVel[start + ib] = qb;
+            }
         }
         for (int i = ib + kBkThreads; i < i1; i += kBkThreads) {
             const unsigned w = sorted[i];
-            const unsigned val = tVals[slot(w & 0xffffu)];
-            kOut[start + i] = base + (w >> 16);
+            const unsigned val = tVals[slot(w & kPosMask)];
+            kOut[start + i] = base + (w >> POSBITS);
             vOut[start + i] = val;
-            sPos[start + i] = rec[2 * (size_t)val];
-            sVel[start + i] = rec[2 * (size_t)val + 1];
+            if (REORDER) {
+                sPos[start + i] = rec[2 * (size_t)val];
+                sVel[start + i] = rec[2 * (size_t)val + 1];
+            }
         }
     } else {
         // a bucket too large for LDS, sorted in global memory by its first workgroup alone: its runs are
@@ -303,21 +322,23 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
             }
             __syncthreads();
         }
-        for (int i = t; i < cnt; i += kBkThreads) {
-            const unsigned val = vOut[start + i];
-            sPos[start + i] = rec[2 * (size_t)val];
-            sVel[start + i] = rec[2 * (size_t)val + 1];
+        if (REORDER) {
+            for (int i = t; i < cnt; i += kBkThreads) {
+                const unsigned val = vOut[start + i];
+                sPos[start + i] = rec[2 * (size_t)val];
+                sVel[start + i] = rec[2 * (size_t)val + 1];
+            }
+            __syncthreads();
+            fill_table([&](int i) -> unsigned { return kOut[start + i]; });
+            __syncthreads();
         }
-        __syncthreads();
-        fill_table([&](int i) -> unsigned { return kOut[start + i]; });
-        __syncthreads();
         for (int i = t; i < cnt; i += kBkThreads) kOut[start + i] += base;
     }
 }
 
 }  // namespace
 
-bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *out) {
+bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *out, int maxShift, int capKeys) {
     const size_t S = shapes.size();
     long long nTot = 0, tiles4k = 0;
     for (const auto &sh : shapes) {
@@ -330,7 +351,7 @@ bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *o
     // keys per bucket on average: a 512th of all keys (512 buckets fill the GPU with one workgroup
     // each; 256 buckets with two workgroups each, which sort every bucket twice, and 1,024 buckets
     // were slower at 2^20 keys), at least 64 and at most 3/4 of what a workgroup sorts in LDS
-    const double target = std::min(0.75 * kBkCap, std::max(64.0, (double)nTot / 512.0));
+    const double target = std::min(0.75 * capKeys, std::max(64.0, (double)nTot / 512.0));
     // buckets per scene: the tile rank's digit, up to 10 bits with tiles of 4096 (1024 threads)
     const int maxBuckets = tk == kTileKeys ? 1024 : 256;
     BucketPlan p;
@@ -351,13 +372,13 @@ bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *o
             const double r = sc.extent[a] / (double)sc.grid[a];
             uneven[a] = r > 0.0 ? std::ceil(r) / r : 1.0;
         }
-        const bool measured = sc.peak.size() == 17;
-        // the widest shift (fewest buckets, at most maxBuckets, offsets of at most 16 bits) whose buckets
-        // average at most `target` keys and whose fullest bucket holds at most 4/5 of the LDS capacity
+        const bool measured = (int)sc.peak.size() > maxShift;
+        // the widest shift (fewest buckets, at most maxBuckets, offsets of at most maxShift bits) whose
+        // buckets average at most `target` keys and whose fullest bucket holds at most 4/5 of the LDS capacity
         // (measured, leaving room for the particles to move) or 3/4 (estimated); else the narrowest
         // one if its buckets still average within that
         int shift = -1, narrowest = -1;
-        for (int b = 16; b >= 0; --b) {
+        for (int b = maxShift; b >= 0; --b) {
             const long long nb = (c + (1ll << b) - 1) >> b;
             if (nb > maxBuckets) break;
             const long long span = 1ll << b;
@@ -365,15 +386,15 @@ bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *o
             double peak = avg * uneven[2];
             if (span < (long long)sc.grid[0] * sc.grid[1]) peak *= uneven[1];
             if (span < (long long)sc.grid[0]) peak *= uneven[0];
-            const bool fits = measured ? sc.peak[b] <= 0.8 * kBkCap : peak <= 0.75 * kBkCap;
-            if (avg <= 0.75 * kBkCap) narrowest = b;
+            const bool fits = measured ? sc.peak[b] <= 0.8 * capKeys : peak <= 0.75 * capKeys;
+            if (avg <= 0.75 * capKeys) narrowest = b;
             if (avg <= target && fits) {
                 shift = b;
                 break;
             }
         }
         if (shift < 0) shift = narrowest;
-        if (sc.shift >= 0 && sc.shift <= 16 && ((c + (1ll << sc.shift) - 1) >> sc.shift) <= maxBuckets)
+        if (sc.shift >= 0 && sc.shift <= maxShift && ((c + (1ll << sc.shift) - 1) >> sc.shift) <= maxBuckets)
             shift = sc.shift;
         if (shift < 0) return false;
         BucketScene &b = p.scenes[s];
@@ -406,8 +427,15 @@ bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *o
 void launch_bucket_sort(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, const float4 *rec, float4 *sortedPos,
                         float4 *sortedVel, unsigned *table, unsigned numCells, int split, hipStream_t st) {
     if (to.numBuckets <= 0) return;
-    hipLaunchKernelGGL(k_bucket_sort, dim3(to.numBuckets * split), dim3(kBkThreads), 0, st, to, split, kOut, vOut, rec,
-                       sortedPos, sortedVel, table, numCells);
+    hipLaunchKernelGGL((k_bucket_sort<16, true>), dim3(to.numBuckets * split), dim3(kBkThreads), 0, st, to, split, kOut,
+                       vOut, rec, sortedPos, sortedVel, table, numCells);
+}
+
+void launch_bucket_sort_morton(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, float4 *scratchA,
+                               float4 *scratchB, hipStream_t st) {
+    if (to.numBuckets <= 0) return;
+    hipLaunchKernelGGL((k_bucket_sort<12, false>), dim3(to.numBuckets), dim3(kBkThreads), 0, st, to, 1, kOut, vOut,
+                       nullptr, scratchA, scratchB, nullptr, 0u);
 }
 
 }  // namespace arsim

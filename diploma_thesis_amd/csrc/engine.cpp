@@ -230,6 +230,9 @@ const char *const *Engine::stage_names(int method, unsigned flags, bool bucket, 
                                         "reduce_apply"};
     static const char *const lbvh[] = {"body_integrate", "pose_key", "sort", "lbvh_build", "lbvh_refit", "dem_lbvh",
                                        "reduce_apply"};
+    // Morton codes ranked per tile by bucket in the pose stage and sorted per bucket in the bucket stage
+    static const char *const lbvhB[] = {"body_integrate", "pose_key", "bucket_sort", "lbvh_build", "lbvh_refit",
+                                        "dem_lbvh", "reduce_apply"};
     static const char *const gridF[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "dem_reduce"};
     static const char *const gridBF[] = {"body_integrate", "pose_key", "bucket_sort_reorder", "dem_reduce"};
     if (method == ARSIM_METHOD_GRID) {
@@ -241,7 +244,7 @@ const char *const *Engine::stage_names(int method, unsigned flags, bool bucket, 
         return bucket ? gridB : grid;
     }
     *count = 7;
-    return lbvh;
+    return bucket ? lbvhB : lbvh;
 }
 
 size_t Engine::scene_bytes_estimate() const { return devBytes; }
@@ -301,6 +304,30 @@ std::vector<unsigned> bucket_peaks(const float *pos4, int n, const SceneGeom &g)
     return peak;
 }
 
+// The fullest bucket of a scene's 30-bit Morton codes by their top 10 bits (1,024 buckets), at index 20 of
+// the plan's per-shift peaks (other widths: UINT_MAX), from host positions normalised as the device does
+// (for planning only).
+std::vector<unsigned> morton_peaks(const float *pos4, int n, const SceneGeom &g) {
+    auto spread = [](unsigned v) {   // bit k of v to bit 3k
+        unsigned r = 0u;
+        for (int k = 0; k < 10; ++k) r |= ((v >> k) & 1u) << (3 * k);
+        return r;
+    };
+    std::vector<unsigned> peak(21, 0xffffffffu), hist(1024, 0u);
+    for (int i = 0; i < n; ++i) {
+        unsigned q[3];
+        for (int a = 0; a < 3; ++a) {
+            const double u = g.mExt[a] == 0.0f ? 0.0 : ((double)pos4[4 * (size_t)i + a] - g.mLo[a]) / g.mExt[a];
+            const double v = u * 1024.0;
+            q[a] = v > 0.0 ? (v < 1023.0 ? (unsigned)v : 1023u) : 0u;   // NaN: 0
+        }
+        const unsigned code = spread(q[0]) * 4u + spread(q[1]) * 2u + spread(q[2]);
+        ++hist[code >> 20];
+    }
+    peak[20] = *std::max_element(hist.begin(), hist.end());
+    return peak;
+}
+
 }  // namespace
 
 int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning, int dev, hipStream_t stream) {
@@ -328,6 +355,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     // ---- layout
     S = (int)src.size();
     bucketSort = false;
+    mortonBucket = false;
     struct Desc { int n, b; ArSimParams p; ArSimStepConfig c; };
     std::vector<Desc> d(S);
     for (int s = 0; s < S; ++s) {
@@ -439,6 +467,17 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         }
     } else if (method == ARSIM_METHOD_LBVH30 && S == 1) {
         keyBits = 30; keyBytes = 4;
+        // the codes sorted per bucket of their top 10 bits (tiles of 4096, fixed Morton box): 1,024 buckets
+        // of 2^20 codes, a bucket's positions in 12 bits of an LDS word
+        if (!(flags & ARSIM_DYNAMIC_AABB) && N > 65536) {
+            std::vector<BucketSceneShape> shapes(1);
+            shapes[0].particles = N;
+            shapes[0].grid[0] = shapes[0].grid[1] = shapes[0].grid[2] = 1024u;
+            if (src[0].host) shapes[0].peak = morton_peaks(src[0].host->pos, N, geoms[0]);
+            else if (src[0].from && src[0].from->mortonBucket) shapes[0].shift = 20;
+            mortonBucket = plan_bucket_sort(shapes, &bkPlan, 20, 4096) && bkPlan.scenes[0].shift == 20 &&
+                           bkPlan.tileKeys == kTileKeys;
+        }
     } else if (method == ARSIM_METHOD_LBVH63) {
         keyBits = 63; keyBytes = 8;
     } else {
@@ -465,14 +504,6 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     if (method == ARSIM_METHOD_GRID) {
         if (tu.fuseReduce) { A(grainIdx, N); A(grainRanges, (size_t)S); }
         A(segStart, (size_t)numSegs + 1);
-        if (bucketSort) {
-            numTiles = (int)bkPlan.tileScene.size();
-            numBuckets = (int)bkPlan.bucketScene.size();
-            tileKeys = bkPlan.tileKeys;
-            A(tileInfo, std::max<size_t>(bkPlan.infoWords, 1)); A(bkScenesD, (size_t)S);
-            A(tileSceneD, std::max(numTiles, 1)); A(bucketSceneD, std::max(numBuckets, 1));
-            A(bucketHist, std::max(numBuckets, 1));
-        }
         A(sortedPos, N); A(sortedVel, N);
         longCap = fill_long_capacity((unsigned long long)numSegs + 1ull);
         A(longCnt, 2); A(longList, (size_t)longCap);
@@ -488,6 +519,14 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         if (S > 1) A(sceneRootBuf, S);
         tree.sceneRoot = S > 1 ? sceneRootBuf : nullptr;
         A(aabbPart, (size_t)2 * aabb_max_partials()); A(aabbBox, 2);
+    }
+    if (bucketSort || mortonBucket) {
+        numTiles = (int)bkPlan.tileScene.size();
+        numBuckets = (int)bkPlan.bucketScene.size();
+        tileKeys = bkPlan.tileKeys;
+        A(tileInfo, std::max<size_t>(bkPlan.infoWords, 1)); A(bkScenesD, (size_t)S);
+        A(tileSceneD, std::max(numTiles, 1)); A(bucketSceneD, std::max(numBuckets, 1));
+        A(bucketHist, std::max(numBuckets, 1));
     }
     if (flags & ARSIM_CONTACT_SIS) {
         A(sisDistA, N); A(sisPartA, N); A(sisDistB, N); A(sisBodyB, N); A(sisPartB, N); A(sceneBodyD, (size_t)S);
@@ -612,7 +651,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     if (longCnt) launch_fill_u32(longCnt, 0u, 1, st);
     maxSceneBodies = 0;
     for (int s = 0; s < S; ++s) maxSceneBodies = std::max(maxSceneBodies, bCount[s]);
-    if (bucketSort) {
+    if (bucketSort || mortonBucket) {
         ARSIM_HIP(hipMemcpyAsync(bkScenesD, bkPlan.scenes.data(), sizeof(BucketScene) * S, hipMemcpyHostToDevice, st));
         ARSIM_HIP(hipMemcpyAsync(tileSceneD, bkPlan.tileScene.data(), sizeof(int) * numTiles, hipMemcpyHostToDevice,
                                  st));
@@ -771,6 +810,37 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         prevSorted = kA;
         lastKeys = kA;
         lastIdx = vA;
+    } else if (mortonBucket) {
+        // Morton codes ranked per tile by their top 10 bits in the pose kernel, then sorted per bucket
+        ZeroJobs z{};
+        z.p[0] = bucketHist;
+        z.count[0] = (size_t)numBuckets;
+        launch_body_integrate(bb, B, ph, &z, st);
+        rec();
+        TileKeyOut to{(unsigned *)keysB, valsB, tileInfo, bucketHist, bkScenesD, tileSceneD, bucketSceneD,
+                      numTiles, numBuckets, tileKeys, bkPlan.bucketBits};
+        to.morton = 1;
+        launch_pose_key_tiles(pb, bb, N, ph, sv, to, false, st);
+        rec();
+        // (the tree's leaf arrays are the second buffer of a bucket sorted in global memory; the refit
+        // writes them afterwards)
+        launch_bucket_sort_morton(to, (unsigned *)keysA, valsA, tree.cms, tree.sortedVel, st);
+        rec();
+        launch_lbvh_build(keysA, 4, N, 0, sceneStartD, sceneCountD, S, tree, st);
+        rec();
+        launch_lbvh_refit((const int *)valsA, pos, vel, rb, nullptr, gatherRec, ph.r, N, tree, st);
+        rec();
+        DemOut o{};
+        o.pForce = pForce; o.vel = vel; o.relPos = relPos;
+        launch_dem_lbvh((const int *)valsA, tree, N, ph, sv, false, o, tu.demBlock, st, overflowD);
+        if (numStatic)
+            launch_dem_static(tree.cms, tree.sortedVel, valsA, N, stPos, stNorm, nullptr, stTable, nullptr, ph,
+                              geoms[0], pForce, nullptr, relPos, false, st);
+        rec();
+        launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
+        rec();
+        lastKeys = keysA;
+        lastIdx = valsA;
     } else {
         const SortPlan pl = sort_plan(sortWs, N, keyBytes, 0, keyBits, stt);
         const bool fused = pl.passes <= 8;   // the key kernel holds at most 8 digit histograms
@@ -1125,7 +1195,8 @@ int arsim_engine_profile_step(arsim_engine *e, float *ms, const char **names, in
     if (!e) return fail(ARSIM_ERR_INVALID_ARG, "null engine");
     Engine &E = e->e;
     int count = 0;
-    const char *const *nm = Engine::stage_names(E.method, E.flags, E.bucketSort, E.fused_dem(), &count);
+    const char *const *nm =
+        Engine::stage_names(E.method, E.flags, E.bucketSort || E.mortonBucket, E.fused_dem(), &count);
     std::vector<hipEvent_t> ev(count + 1);
     for (auto &x : ev) ARSIM_HIP(hipEventCreate(&x));
     int ns = 0;
@@ -1224,7 +1295,8 @@ int log2_bucket(long long v) {
 int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best, float *bestMs) {
     if (!e || stepsPerTrial <= 0) return fail(ARSIM_ERR_INVALID_ARG, "autotune: bad arguments");
     Engine &E = e->e;
-    const TuneKey key{E.method, (int)E.flags, E.bucketSort ? 1 : 0, E.keyBits, log2_bucket(E.N), log2_bucket(E.S)};
+    const TuneKey key{E.method, (int)E.flags, (E.bucketSort || E.mortonBucket) ? 1 : 0, E.keyBits, log2_bucket(E.N),
+                      log2_bucket(E.S)};
     {
         std::lock_guard<std::mutex> lock(g_tuneMutex);
         auto it = g_tuneCache.find(key);
@@ -1241,7 +1313,8 @@ int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best,
     }
     // with the bucket sort (and no contact-list sort) the onesweep tuning is unused: only the block size
     // and the workgroups per bucket
-    const bool sortUnused = E.method == ARSIM_METHOD_GRID && E.bucketSort && !(E.flags & ARSIM_CONTACT_SIS);
+    const bool sortUnused = (E.method == ARSIM_METHOD_GRID && E.bucketSort && !(E.flags & ARSIM_CONTACT_SIS)) ||
+                            E.mortonBucket;
     // the trials advance a scratch copy of the state: snapshot everything a step writes
     struct Snap { void *dst; const void *src; size_t bytes; };
     std::vector<Snap> snaps = {
@@ -1290,7 +1363,7 @@ int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best,
                 if (rc) break;
                 if (!arsim::sort_tuning_supported(ip, rb)) continue;
                 if (sortUnused && (ip != orig.sortItemsPerThread || rb != orig.sortRadixBits)) continue;
-                if (!E.bucketSort) {
+                if (!E.bucketSort || E.method != ARSIM_METHOD_GRID) {
                     rc = trial(ArSimTuning{bl, ip, rb, orig.useGraph, orig.fuseReduce, orig.bucketSplit});
                     continue;
                 }

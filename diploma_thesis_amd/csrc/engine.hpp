@@ -71,6 +71,7 @@ struct Engine {
     unsigned longCap = 0;
     int segBits = 3;
     bool bucketSort = false;                // grid: sort finished per bucket of cells (bucket.hip)
+    bool mortonBucket = false;              // LBVH30, one scene: Morton codes sorted per bucket of their top 10 bits
     long long numSegs = 0;
     float4 *sortedPos = nullptr, *sortedVel = nullptr;
     void *sortWs = nullptr;

@@ -165,6 +165,7 @@ struct TileKeyOut {
     const int *tileScene, *bucketScene;   // scene of each tile and of each bucket
     int numTiles, numBuckets, tileKeys;
     int bucketBits;                       // a scene has at most 2^bucketBits buckets (8, 9 or 10)
+    int morton = 0;                       // keys are 30-bit Morton codes (one scene, 1,024 buckets of 2^20 codes)
 };
 void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,
                            const SceneView &sv, const TileKeyOut &to, bool grainVelDead, hipStream_t st);
@@ -180,7 +181,7 @@ struct BucketSceneShape {
     long long particles = 0;
     unsigned grid[3] = {1, 1, 1};
     double extent[3] = {0.0, 0.0, 0.0};
-    std::vector<unsigned> peak;   // empty, or 17 entries
+    std::vector<unsigned> peak;   // empty, or one entry per shift 0 .. maxShift
     int shift = -1;
 };
 struct BucketPlan {
@@ -190,13 +191,21 @@ struct BucketPlan {
     int tileKeys = kTileKeys;
     int bucketBits = 8;   // the most buckets of one scene, rounded up to a power of two (at least 2^8)
 };
-bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *out);
+// maxShift / capKeys: the widest bucket (2^maxShift cells) and the most keys a bucket sorted in LDS holds
+// (grid keys: 2^16 and 7,776; Morton keys: 2^20 and 4,096, bucket positions taking 12 bits of a word).
+bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *out, int maxShift = 16,
+                      int capKeys = 7776);
 // The grid sort finished per bucket (bucket.hip): each bucket gathers its keys from its scene's
 // tiles' runs (tile order keeps the sort stable), `split` workgroups (1 or 2) sort it by the low bits,
 // each writing its share of the sorted keys / indices to (kOut, vOut), of the gathered sorted copies
 // and of the bucket's range of the exact cell table (whose last entry, numCells, the last bucket writes).
 void launch_bucket_sort(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, const float4 *rec, float4 *sortedPos,
                         float4 *sortedVel, unsigned *table, unsigned numCells, int split, hipStream_t st);
+// The same for 30-bit Morton keys ranked by their top 10 bits (TileKeyOut.morton): each bucket of up to
+// 4,096 keys sorted by its low 20 bits in LDS, written as sorted keys and indices only; a larger bucket is
+// sorted in global memory with scratchA / scratchB (16 B per key, N entries) as its second buffer.
+void launch_bucket_sort_morton(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, float4 *scratchA,
+                               float4 *scratchB, hipStream_t st);
 // Capacity of the long-run list of a table of tableSize entries (runs are disjoint and longer than
 // kLongRange), and the launch that fills the listed runs.
 unsigned fill_long_capacity(unsigned long long tableSize);

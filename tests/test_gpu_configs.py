@@ -28,6 +28,28 @@ def test_benchmark_scene_lbvh_two_steps_bit_exact(method):
     assert_state_equal(got, st, f"benchmark scene, method {method}")
 
 
+def test_lbvh30_bucket_past_the_lds_capacity():
+    """100,000 particles with 25,000 grains packed into one corner: their 30-bit codes share the top 10 bits,
+    so one bucket of the per-bucket Morton sort holds far more keys than LDS does (4,096) and is sorted in
+    global memory.  Two graph-replayed steps equal the oracle's."""
+    print("engine: LBVH30 Morton bucket past the LDS capacity, 2 steps", flush=True)
+    sc = scenes.mix_scene(100000, 33, capi.METHOD_LBVH30)
+    g = np.where(sc.rb < 0)[0]
+    assert len(g) >= 20000
+    rng = np.random.default_rng(33)
+    lo = sc.pos[:, :3].min(0)
+    sc.pos[g, :3] = (lo + 1.0 + rng.uniform(0.0, 6.0, (len(g), 3))).astype(np.float32)
+    with Engine(sc, use_graph=True) as e:
+        e.step(2)
+        e.sync()
+        got = e.download()
+        stages = [name for name, _ in e.profile()]
+    assert "bucket_sort" in stages, stages
+    st = oracle.State(sc, np.float32)
+    st.step(2)
+    assert_state_equal(got, st, "Morton bucket past the LDS capacity")
+
+
 def test_benchmark_batch_lbvh_forest_one_step():
     """The 512-scene batch (8.85 M particles) as one LBVH forest: 64-bit keys with the scene id above
     bit 30, one tree per scene.  Every scene's state after a step equals the oracle's step of that
