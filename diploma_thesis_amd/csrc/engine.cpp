@@ -515,7 +515,9 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         tree.blockTabLevels = lbvh_refit_levels(N);
         A(tree.blockTab, 2 * (size_t)tree.blockTabLevels * (size_t)((N + lbvh_refit_block() - 1) / lbvh_refit_block()));
         A(tree.packed, 4 * (size_t)N);
-        A(tree.bounds, 2 * nodes); A(tree.cms, N); A(tree.sortedVel, N); A(tree.counter, N);
+        // node boxes: the step keeps them in the traversal records only (tree.bounds stays NULL);
+        // treeBounds receives them when a debug view asks
+        A(treeBounds, 2 * nodes); A(tree.cms, N); A(tree.sortedVel, N); A(tree.counter, N);
         if (S > 1) A(sceneRootBuf, S);
         tree.sceneRoot = S > 1 ? sceneRootBuf : nullptr;
         A(aabbPart, (size_t)2 * aabb_max_partials()); A(aabbBox, 2);
@@ -1228,7 +1230,16 @@ int arsim_engine_debug_buffers(arsim_engine *e, void **sortedKeys, void **sorted
     if (sortedVel) *sortedVel = E.method == ARSIM_METHOD_GRID ? (void *)E.sortedVel : (void *)E.tree.sortedVel;
     if (lbvhLeft) *lbvhLeft = E.tree.left;
     if (lbvhRight) *lbvhRight = E.tree.right;
-    if (lbvhBounds) *lbvhBounds = E.tree.bounds;
+    if (lbvhBounds) {
+        *lbvhBounds = E.treeBounds;
+        if (E.treeBounds) {   // the last step's records (uninitialised before the first step)
+            ARSIM_HIP(hipSetDevice(E.device));
+            arsim::Lbvh t = E.tree;
+            t.bounds = E.treeBounds;
+            launch_lbvh_bounds(t, E.N, E.ph.r, E.st);
+            ARSIM_HIP(hipStreamSynchronize(E.st));
+        }
+    }
     if (lbvhCMs) *lbvhCMs = E.tree.cms;
     return ARSIM_OK;
 }

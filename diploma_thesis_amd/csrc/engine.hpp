@@ -77,6 +77,7 @@ struct Engine {
     void *sortWs = nullptr;
     size_t sortWsBytes = 0;
     Lbvh tree{};
+    float4 *treeBounds = nullptr;   // node boxes for debug views (the step keeps tree.bounds NULL)
     int *overflowD = nullptr, *flagD = nullptr;
     float4 *aabbPart = nullptr, *aabbBox = nullptr;
     // static collider set (one-scene engines): points and normals sorted by cell, exact cell table

@@ -231,7 +231,7 @@ void launch_dem_fused(const float4 *sortedPos, const float4 *sortedVel, const un
 struct Lbvh {
     int *left, *right, *parent, *minR, *maxR;
     unsigned char *isLeaf;        // optional
-    float4 *bounds;               // 2 float4 per node: (min, 0), (max, 0)
+    float4 *bounds;               // 2 float4 per node: (min, 0), (max, 0); NULL: not kept (range refit)
     float4 *cms;                  // leaf centres, w = rbIndex bits
     float4 *sortedVel;            // leaf velocities, w = scene id bits when batched
     int *counter;                 // N - 1 refit counters
@@ -241,7 +241,8 @@ struct Lbvh {
     float4 *packed;
     // optional (engine): internal node k as (left, right, minR, maxR), and the buffers of the range
     // refit: per leaf the AABB of the leaves from its refit workgroup's start (pre) and to its end
-    // (suf), 2 float4 each, and a sparse table of workgroup AABBs (levels x numBlocks x 2 float4)
+    // (suf), 2 float4 each, written only at the leaves where a range that crosses workgroups ends or
+    // starts, and a sparse table of workgroup AABBs (levels x numBlocks x 2 float4)
     int4 *meta;
     float4 *pre, *suf, *blockTab;
     int blockTabLevels;
@@ -253,8 +254,13 @@ int lbvh_refit_levels(int n);
 // scene-local leaf index (sceneStart[scene] is subtracted).
 void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShift, const int *sceneStart,
                        const int *sceneCount, int numScenes, const Lbvh &t, hipStream_t st);
+// With the range-refit buffers (meta, pre, suf, blockTab, packed) t.bounds may be NULL: the step then
+// keeps node boxes only in the traversal records, and launch_lbvh_bounds restates them on request.
 void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *vel, const int *rb,
                        const uint16_t *scene, const float4 *rec, float radius, int n, const Lbvh &t, hipStream_t st);
+// Fills t.bounds (2 float4 per node) from the leaf centres (box = centre -/+ radius) and the packed
+// traversal records of a range refit that ran without bounds.
+void launch_lbvh_bounds(const Lbvh &t, int n, float radius, hipStream_t st);
 // overflow: device int set to 1 when a traversal stack would exceed its LDS depth.
 void launch_dem_lbvh(const int *sortedIdx, const Lbvh &t, int n, const DevPhys &ph, const SceneView &sv,
                      bool sceneInVelW, const DemOut &o, int block, hipStream_t st, int *overflow);

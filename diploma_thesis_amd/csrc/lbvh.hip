@@ -21,13 +21,22 @@ __device__ unsigned long long g_travStats[kTravStats];
 __device__ __forceinline__ int clz_key(unsigned v) { return __clz((int)v); }
 __device__ __forceinline__ int clz_key(uint64_t v) { return __clzll((long long)v); }
 
+// The codes a workgroup's searches read, [i0 - kKarrasHalo, i0 + kKarrasBlock + kKarrasHalo), are
+// staged in LDS: a node whose range is at most kKarrasHalo / 2 long (most of them) searches LDS only;
+// wider ranges read the rest from global memory.
+constexpr int kKarrasBlock = 1024;
+constexpr int kKarrasHalo = 1024;
+constexpr int kKarrasWin = kKarrasBlock + 2 * kKarrasHalo;
+
 // longestCommonPrefix (BVHSource/BVHAuxiliary.cuh:31-43).  Equal keys always belong to one
 // scene (the scene id is in the high key bits), so the tie-break uses scene-local indices
 // li = i - base, lj = j - base and a batched subtree matches the standalone tree.
 template <class K>
-__device__ __forceinline__ int lcp(const K *__restrict__ codes, int n, K ki, int i, int j, int base) {
+__device__ __forceinline__ int lcp(const K *__restrict__ codes, const K *win, int w0, int n, K ki, int i, int j,
+                                   int base) {
     if (j < 0 || j >= n) return -1;
-    K kj = codes[j];
+    const unsigned q = (unsigned)(j - w0);
+    K kj = q < (unsigned)kKarrasWin ? win[q] : codes[j];
     if (ki != kj) return clz_key((K)(ki ^ kj));
     return (int)sizeof(K) * 8 + __clz((int)((unsigned)(i - base) ^ (unsigned)(j - base)));
 }
@@ -35,31 +44,40 @@ __device__ __forceinline__ int lcp(const K *__restrict__ codes, int n, K ki, int
 // kernelConstructRadixTreeSoA (BVHSource/BVHSoAKernels.cu:7-99).  t = ceil(l / 2^k) is computed
 // with integer shifts instead of the reference's double division (identical for l < 2^31).
 template <class K>
-__global__ void k_karras(const K *__restrict__ codes, int n, int sceneShift, const int *__restrict__ sceneStart,
-                         const int *__restrict__ sceneCount, Lbvh t) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kKarrasBlock) void k_karras(const K *__restrict__ codes, int n, int sceneShift,
+                                                         const int *__restrict__ sceneStart,
+                                                         const int *__restrict__ sceneCount, Lbvh t) {
+    __shared__ K win[kKarrasWin];
+    const int i0 = (int)blockIdx.x * kKarrasBlock, w0 = i0 - kKarrasHalo;
+    for (int q = (int)threadIdx.x; q < kKarrasWin; q += kKarrasBlock) {
+        const int j = w0 + q;
+        if (j >= 0 && j < n) win[q] = codes[j];
+    }
+    __syncthreads();
+    const int i = i0 + (int)threadIdx.x;
     if (i == 0) t.parent[n] = -1;
     if (i >= n - 1) return;
     t.counter[i] = 0;
-    const K ki = codes[i];
+    auto lcp = [&](K ki_, int i_, int j_, int base_) { return arsim::lcp(codes, win, w0, n, ki_, i_, j_, base_); };
+    const K ki = win[i - w0];
     const int base = sceneShift > 0 ? sceneStart[(int)(ki >> sceneShift)] : 0;
-    const int dPrev = lcp(codes, n, ki, i, i - 1, base);
-    const int dNext = lcp(codes, n, ki, i, i + 1, base);
+    const int dPrev = lcp(ki, i, i - 1, base);
+    const int dNext = lcp(ki, i, i + 1, base);
     const int d = (dNext - dPrev) > 0 ? 1 : -1;
-    const int sigMin = lcp(codes, n, ki, i, i - d, base);
+    const int sigMin = lcp(ki, i, i - d, base);
     long long lmax = 2;
-    while (lcp(codes, n, ki, i, (int)(i + lmax * d), base) > sigMin) lmax *= 2;
+    while (lcp(ki, i, (int)(i + lmax * d), base) > sigMin) lmax *= 2;
     long long l = 0;
     for (long long s = lmax / 2; s >= 1; s /= 2) {
-        if (lcp(codes, n, ki, i, (int)(i + (l + s) * d), base) > sigMin) l += s;
+        if (lcp(ki, i, (int)(i + (l + s) * d), base) > sigMin) l += s;
         if (s == 1) break;
     }
     const int j = (int)(i + l * d);
-    const int sigNode = lcp(codes, n, ki, i, j, base);
+    const int sigNode = lcp(ki, i, j, base);
     long long s = 0;
     for (int k = 1;; ++k) {
         long long tt = (l + (1ll << k) - 1) >> k;
-        if (lcp(codes, n, ki, i, (int)(i + (s + tt) * d), base) > sigNode) s += tt;
+        if (lcp(ki, i, (int)(i + (s + tt) * d), base) > sigNode) s += tt;
         if (tt == 1) break;
     }
     const int gamma = (int)(i + s * d + (d < 0 ? d : 0));
@@ -139,7 +157,7 @@ __device__ __forceinline__ void write_packed(const Lbvh &t, int n, int node, int
 __device__ __forceinline__ void leaf_init(int i, const int *__restrict__ sIdx, const float4 *__restrict__ pos,
                                           const float4 *__restrict__ vel, const int *__restrict__ rb,
                                           const uint16_t *__restrict__ scene, const float4 *__restrict__ rec, float r,
-                                          const Lbvh &t, float4 &lmin, float4 &lmax) {
+                                          const Lbvh &t, float4 &lmin, float4 &lmax, float4 &centre) {
     const int o = sIdx[i];
     float4 p;
     if (rec) {   // the key kernel's gather record: (pos, rb bits), (vel, scene bits)
@@ -155,6 +173,7 @@ __device__ __forceinline__ void leaf_init(int i, const int *__restrict__ sIdx, c
             t.sortedVel[i] = v;
         }
     }
+    centre = p;
     lmin = make_float4(p.x - r, p.y - r, p.z - r, 0.0f);
     lmax = make_float4(p.x + r, p.y + r, p.z + r, 0.0f);
     if (t.isLeaf) t.isLeaf[i] = 1;
@@ -168,8 +187,8 @@ __global__ __launch_bounds__(256) void k_leaf_refit(const int *__restrict__ sIdx
                                                     const float4 *__restrict__ rec, float r, int n, Lbvh t) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
-    float4 lmin, lmax;
-    leaf_init(i, sIdx, pos, vel, rb, scene, rec, r, t, lmin, lmax);
+    float4 lmin, lmax, centre;
+    leaf_init(i, sIdx, pos, vel, rb, scene, rec, r, t, lmin, lmax, centre);
     store_bounds(t.bounds, (size_t)i, lmin, lmax);
     if (n < 2) return;
     int node = t.parent[i];
@@ -223,22 +242,35 @@ __global__ __launch_bounds__(kBlk) void k_refit_local(const int *__restrict__ sI
                                                       const float4 *__restrict__ vel, const int *__restrict__ rb,
                                                       const uint16_t *__restrict__ scene,
                                                       const float4 *__restrict__ rec, float r, int n, Lbvh t) {
-    __shared__ float4 sLeaf[2 * kBlk];   // leaf boxes of this workgroup
+    __shared__ float4 sCtr[kBlk];        // leaf centres of this workgroup (a leaf's box is centre -/+ r)
     __shared__ float4 sNode[2 * kBlk];   // boxes of its local internal nodes
+    __shared__ int4 sTop[kBlk];          // internal node b0 + slot: left, right, parent, other end of its range
     __shared__ int sCnt[kBlk];
+    __shared__ unsigned char sNeedSuf[kBlk];   // a crossing range starts at this leaf: its suf is read
+    __shared__ unsigned char sNeedPre[kBlk];   // a crossing range ends at this leaf: its pre is read
     __shared__ float4 sWave[4 * (kBlk / 64)];   // per wave: prefix total min / max, suffix total min / max
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b0 = (int)blockIdx.x * kBlk;
     const int i = b0 + tid;
     const bool valid = i < n;
     sCnt[tid] = 0;
+    sNeedSuf[tid] = 0;
+    sNeedPre[tid] = 0;
+    // The topology of internal nodes [b0, b0 + kBlk), staged so that the walk up reads LDS only: an
+    // internal node's index is one end of its leaf range, so every node whose range lies inside this
+    // workgroup's leaves is one of them.
+    if (i < n - 1) {
+        const int4 m = t.meta[i];
+        sTop[tid] = make_int4(m.x, m.y, t.parent[i + n], m.z == i ? m.w : m.z);
+    }
+    int node = valid && n >= 2 ? t.parent[i] : -1;
     const float inf = __int_as_float(0x7f800000);
     float4 lmin = make_float4(inf, inf, inf, 0.0f), lmax = make_float4(-inf, -inf, -inf, 0.0f);
     if (valid) {
-        leaf_init(i, sIdx, pos, vel, rb, scene, rec, r, t, lmin, lmax);
-        store_plain(t.bounds, i, lmin, lmax);
-        sLeaf[2 * tid] = lmin;
-        sLeaf[2 * tid + 1] = lmax;
+        float4 c;
+        leaf_init(i, sIdx, pos, vel, rb, scene, rec, r, t, lmin, lmax, c);
+        if (t.bounds) store_plain(t.bounds, i, lmin, lmax);
+        sCtr[tid] = c;
     }
     // inclusive prefix and suffix min / max within the wave
     float4 pmn = lmin, pmx = lmax, smn = lmin, smx = lmax;
@@ -259,43 +291,112 @@ __global__ __launch_bounds__(kBlk) void k_refit_local(const int *__restrict__ sI
         if (w < wave) { pmn = fmin4(pmn, sWave[4 * w]); pmx = fmax4(pmx, sWave[4 * w + 1]); }
         if (w > wave) { smn = fmin4(smn, sWave[4 * w + 2]); smx = fmax4(smx, sWave[4 * w + 3]); }
     }
-    if (valid) {
-        t.pre[2 * (size_t)i] = pmn; t.pre[2 * (size_t)i + 1] = pmx;
-        t.suf[2 * (size_t)i] = smn; t.suf[2 * (size_t)i + 1] = smx;
-    }
     if (tid == kBlk - 1) {   // the whole workgroup's box: the last thread's prefix
         t.blockTab[2 * (size_t)blockIdx.x] = pmn;
         t.blockTab[2 * (size_t)blockIdx.x + 1] = pmx;
     }
-    if (!valid || n < 2) return;
-    int node = t.parent[i];
-    while (node >= 0) {
-        const int4 m = t.meta[node - n];   // left, right, lo, hi
-        if (range_crosses(m.z, m.w)) break;
-        const int slot = node - n - b0;   // an internal node's index is an end of its range
+    auto box = [&](int c, float4 &mn, float4 &mx) {
+        if (c < n) {
+            const float4 p = sCtr[c - b0];
+            mn = make_float4(p.x - r, p.y - r, p.z - r, 0.0f);
+            mx = make_float4(p.x + r, p.y + r, p.z + r, 0.0f);
+        } else {
+            mn = sNode[2 * (c - n - b0)];
+            mx = sNode[2 * (c - n - b0) + 1];
+        }
+    };
+    // Walk up while the nodes are local; the second arrival at a node merges it.  A local node (or
+    // leaf) whose parent's range crosses the workgroup's end marks its range's ends: every crossing
+    // range starts where the left-most such node of its left spine starts and ends where the
+    // right-most of its right spine ends, so the crossing kernel reads suf / pre only there.
+    int lo = i, hi = i;
+    while (valid && node >= 0) {
+        const int slot = node - n - b0;
+        int4 m = make_int4(0, 0, 0, 0);
+        bool crosses = (unsigned)slot >= (unsigned)kBlk;   // not one of ours: its range leaves the workgroup
+        if (!crosses) {
+            m = sTop[slot];
+            const int k = node - n;
+            crosses = range_crosses(min(k, m.w), max(k, m.w));
+        }
+        if (crosses) {
+            sNeedSuf[lo - b0] = 1;
+            sNeedPre[hi - b0] = 1;
+            break;
+        }
         const int prev = __hip_atomic_fetch_add(&sCnt[slot], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (prev == 0) return;
-        const int L = m.x, R = m.y;
-        const float4 aMin = L < n ? sLeaf[2 * (L - b0)] : sNode[2 * (L - n - b0)];
-        const float4 aMax = L < n ? sLeaf[2 * (L - b0) + 1] : sNode[2 * (L - n - b0) + 1];
-        const float4 bMin = R < n ? sLeaf[2 * (R - b0)] : sNode[2 * (R - n - b0)];
-        const float4 bMax = R < n ? sLeaf[2 * (R - b0) + 1] : sNode[2 * (R - n - b0) + 1];
+        if (prev == 0) break;
+        float4 aMin, aMax, bMin, bMax;
+        box(m.x, aMin, aMax);
+        box(m.y, bMin, bMax);
         const float4 mn = fmin4(aMin, bMin), mx = fmax4(aMax, bMax);
         sNode[2 * slot] = mn;
         sNode[2 * slot + 1] = mx;
-        store_plain(t.bounds, node, mn, mx);
-        write_packed(t, n, node, L, R, aMin, aMax, bMin, bMax);
-        node = t.parent[node];
+        if (t.bounds) store_plain(t.bounds, node, mn, mx);
+        write_packed(t, n, node, m.x, m.y, aMin, aMax, bMin, bMax);
+        const int k = node - n;
+        lo = min(k, m.w);
+        hi = max(k, m.w);
+        node = m.z;
+    }
+    if ((int)gridDim.x == 1) return;   // one workgroup: no range crosses
+    __syncthreads();
+    if (valid) {
+        if (sNeedSuf[tid]) { t.suf[2 * (size_t)i] = smn; t.suf[2 * (size_t)i + 1] = smx; }
+        if (sNeedPre[tid]) { t.pre[2 * (size_t)i] = pmn; t.pre[2 * (size_t)i + 1] = pmx; }
     }
 }
 
-// Level l, entry b: box of workgroups [b, b + 2^l).  One workgroup builds the levels in order.
+// Level l, entry b: box of workgroups [b, b + 2^l).  One workgroup builds the levels in order; up to
+// kTabLds workgroup boxes, each level is computed from the previous one held in LDS (as six planes
+// of floats), else read back from global memory.
+constexpr int kTabLds = 4096;
+constexpr int kTabPer = kTabLds / 1024;   // entries per thread
 __global__ __launch_bounds__(1024) void k_refit_table(int numBlocks, int levels, float4 *__restrict__ tab) {
+    __shared__ float sBox[6][kTabLds];
+    const int tid = (int)threadIdx.x;
+    if (numBlocks <= kTabLds) {
+        for (int b = tid; b < numBlocks; b += 1024) {
+            const float4 mn = tab[2 * b], mx = tab[2 * b + 1];
+            sBox[0][b] = mn.x; sBox[1][b] = mn.y; sBox[2][b] = mn.z;
+            sBox[3][b] = mx.x; sBox[4][b] = mx.y; sBox[5][b] = mx.z;
+        }
+        __syncthreads();
+        for (int l = 1; l < levels; ++l) {
+            float4 *dst = tab + 2 * (size_t)l * numBlocks;
+            const int half = 1 << (l - 1), cnt = numBlocks - (1 << l) + 1;
+            float4 mn[kTabPer], mx[kTabPer];
+#pragma unroll
+            for (int q = 0; q < kTabPer; ++q) {
+                const int b = tid + 1024 * q;
+                if (b >= cnt) break;
+                const float4 aMin = make_float4(sBox[0][b], sBox[1][b], sBox[2][b], 0.0f);
+                const float4 aMax = make_float4(sBox[3][b], sBox[4][b], sBox[5][b], 0.0f);
+                const int c = b + half;
+                const float4 bMin = make_float4(sBox[0][c], sBox[1][c], sBox[2][c], 0.0f);
+                const float4 bMax = make_float4(sBox[3][c], sBox[4][c], sBox[5][c], 0.0f);
+                mn[q] = fmin4(aMin, bMin);
+                mx[q] = fmax4(aMax, bMax);
+                dst[2 * b] = mn[q];
+                dst[2 * b + 1] = mx[q];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < kTabPer; ++q) {
+                const int b = tid + 1024 * q;
+                if (b >= cnt) break;
+                sBox[0][b] = mn[q].x; sBox[1][b] = mn[q].y; sBox[2][b] = mn[q].z;
+                sBox[3][b] = mx[q].x; sBox[4][b] = mx[q].y; sBox[5][b] = mx[q].z;
+            }
+            __syncthreads();
+        }
+        return;
+    }
     for (int l = 1; l < levels; ++l) {
         const float4 *src = tab + 2 * (size_t)(l - 1) * numBlocks;
         float4 *dst = tab + 2 * (size_t)l * numBlocks;
         const int half = 1 << (l - 1), cnt = numBlocks - (1 << l) + 1;
-        for (int b = (int)threadIdx.x; b < cnt; b += (int)blockDim.x) {
+        for (int b = tid; b < cnt; b += (int)blockDim.x) {
             dst[2 * b] = fmin4(src[2 * b], src[2 * (b + half)]);
             dst[2 * b + 1] = fmax4(src[2 * b + 1], src[2 * (b + half) + 1]);
         }
@@ -318,7 +419,23 @@ __device__ __forceinline__ void range_box(const Lbvh &t, int numBlocks, int lo, 
     }
 }
 
-__device__ __forceinline__ void child_box(const Lbvh &t, int n, int numBlocks, int c, float4 &mn, float4 &mx) {
+// The box of a node that k_refit_local merged (or of a leaf), from what the step keeps: a leaf's
+// centre -/+ r, an internal node's two child boxes in its traversal record (the same min / max as
+// its merge took).
+__device__ __forceinline__ void kept_box(const Lbvh &t, int n, float r, int c, float4 &mn, float4 &mx) {
+    if (c < n) {
+        const float4 p = t.cms[c];
+        mn = make_float4(p.x - r, p.y - r, p.z - r, 0.0f);
+        mx = make_float4(p.x + r, p.y + r, p.z + r, 0.0f);
+    } else {
+        const float4 *q = t.packed + 4 * (size_t)(c - n);
+        mn = fmin4(q[0], q[2]);
+        mx = fmax4(q[1], q[3]);
+    }
+}
+
+__device__ __forceinline__ void child_box(const Lbvh &t, int n, int numBlocks, float r, int c, float4 &mn,
+                                          float4 &mx) {
     if (c >= n) {
         const int4 m = t.meta[c - n];
         if (range_crosses(m.z, m.w)) {
@@ -326,21 +443,35 @@ __device__ __forceinline__ void child_box(const Lbvh &t, int n, int numBlocks, i
             return;
         }
     }
-    mn = t.bounds[2 * (size_t)c];
-    mx = t.bounds[2 * (size_t)c + 1];
+    if (t.bounds) {
+        mn = t.bounds[2 * (size_t)c];
+        mx = t.bounds[2 * (size_t)c + 1];
+    } else {
+        kept_box(t, n, r, c, mn, mx);
+    }
 }
 
-__global__ void k_refit_cross(int n, int numBlocks, Lbvh t) {
+__global__ void k_refit_cross(int n, int numBlocks, float r, Lbvh t) {
     const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (k >= n - 1) return;
     const int4 m = t.meta[k];
     if (!range_crosses(m.z, m.w)) return;
     float4 mn, mx, aMin, aMax, bMin, bMax;
     range_box(t, numBlocks, m.z, m.w, mn, mx);
-    child_box(t, n, numBlocks, m.x, aMin, aMax);
-    child_box(t, n, numBlocks, m.y, bMin, bMax);
-    store_plain(t.bounds, k + n, mn, mx);
+    child_box(t, n, numBlocks, r, m.x, aMin, aMax);
+    child_box(t, n, numBlocks, r, m.y, bMin, bMax);
+    if (t.bounds) store_plain(t.bounds, k + n, mn, mx);
     write_packed(t, n, k + n, m.x, m.y, aMin, aMax, bMin, bMax);
+}
+
+// Node boxes for callers that read them (the engine's step keeps none): the boxes k_refit_local and
+// k_refit_cross merged, restated from the leaf centres and the traversal records.
+__global__ void k_bounds_from_records(int n, float r, Lbvh t) {
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= 2 * n - 1) return;
+    float4 mn, mx;
+    kept_box(t, n, r, c, mn, mx);
+    store_plain(t.bounds, c, mn, mx);
 }
 
 __device__ __forceinline__ bool overlap(float4 aMin, float4 aMax, float4 bMin, float4 bMax) {
@@ -474,7 +605,7 @@ __device__ __forceinline__ float row_word(unsigned w) {
 template <bool EMIT, bool FOREST>
 __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *__restrict__ packed,
                                    const float4 *__restrict__ cms, const float4 *__restrict__ sVel,
-                                   const float4 *__restrict__ bounds, const int *__restrict__ sceneRoot, int n,
+                                   const int *__restrict__ sceneRoot, int n,
                                    DevPhys ph, SceneView sv, DemOut o, int *overflow) {
     extern __shared__ int pStack[];
     const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
@@ -488,7 +619,9 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
     const float4 C = cms[ii];
     const float4 Vq = sVel[ii];
     const int ra = __float_as_int(C.w);
-    const float4 qMin = bounds[2 * (size_t)ii], qMax = bounds[2 * (size_t)ii + 1];
+    // the leaf's box, centre -/+ r as the refit forms it
+    const float4 qMin = make_float4(C.x - ph.r, C.y - ph.r, C.z - ph.r, 0.0f);
+    const float4 qMax = make_float4(C.x + ph.r, C.y + ph.r, C.z + ph.r, 0.0f);
     const f3 pa = xyz(C), va = xyz(Vq);
     const int orig = sIdx[ii];
     f3 F = mk3(0.f, 0.f, 0.f);
@@ -674,7 +807,7 @@ void launch_lbvh_build(const void *sortedKeys, int keyBytes, int n, int sceneShi
                        const int *sceneCount, int numScenes, const Lbvh &t, hipStream_t st) {
     (void)numScenes;
     if (n <= 0) return;
-    const int block = 256;
+    const int block = kKarrasBlock;
     int threads = n - 1 > 1 ? n - 1 : 1;
     dim3 grid((threads + block - 1) / block);
     if (keyBytes == 8)
@@ -694,13 +827,19 @@ void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *ve
                            radius, n, t);
         if (numBlocks > 1) {
             hipLaunchKernelGGL(k_refit_table, dim3(1), dim3(1024), 0, st, numBlocks, lbvh_refit_levels(n), t.blockTab);
-            hipLaunchKernelGGL(k_refit_cross, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, numBlocks, t);
+            hipLaunchKernelGGL(k_refit_cross, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, numBlocks, radius, t);
         }
         return;
     }
     const int block = 256;
     hipLaunchKernelGGL(k_leaf_refit, dim3((n + block - 1) / block), dim3(block), 0, st, sortedIdx, pos, vel, rb,
                        scene, rec, radius, n, t);
+}
+
+void launch_lbvh_bounds(const Lbvh &t, int n, float radius, hipStream_t st) {
+    if (n <= 0 || !t.bounds || !t.packed) return;
+    const int nodes = 2 * n - 1;
+    hipLaunchKernelGGL(k_bounds_from_records, dim3((nodes + 255) / 256), dim3(256), 0, st, n, radius, t);
 }
 
 void launch_dem_lbvh(const int *sortedIdx, const Lbvh &t, int n, const DevPhys &ph, const SceneView &sv,
@@ -714,8 +853,8 @@ void launch_dem_lbvh(const int *sortedIdx, const Lbvh &t, int n, const DevPhys &
         grid = dim3((n + block - 1) / block);
         const size_t pl = (size_t)kWaveStackDepth * (block / kPacketLanes) * sizeof(int);
         auto run = [&](auto kernel) {
-            hipLaunchKernelGGL(kernel, grid, dim3(block), pl, st, sortedIdx, t.packed, t.cms, t.sortedVel, t.bounds,
-                               t.sceneRoot, n, ph, sv, o, overflow);
+            hipLaunchKernelGGL(kernel, grid, dim3(block), pl, st, sortedIdx, t.packed, t.cms, t.sortedVel, t.sceneRoot, n,
+                               ph, sv, o, overflow);
         };
         if (t.sceneRoot) {
             if (o.pairs) run(k_dem_lbvh_packets<true, true>);

@@ -355,3 +355,44 @@ def test_segmented_cell_table_past_the_exact_limit():
         st = oracle.State(sc, np.float32)
         st.step(2)
         assert_state_equal(got[k], st, f"segmented table, scene {k}")
+
+
+@pytest.mark.parametrize("scene_counts", [(30000,), (9000, 20000, 4000)])
+def test_engine_lbvh_node_boxes(scene_counts):
+    """The engine's step keeps node boxes only inside the traversal records; the debug view restates
+    them.  Every leaf's box must be its centre -/+ r and every internal node's box the min / max of its
+    children's, over a tree of many refit workgroups (one scene, and a forest of three)."""
+    print(f"engine: LBVH node boxes, scenes {scene_counts}", flush=True)
+    mix = [scenes.mix_scene(n, 500 + n, capi.METHOD_LBVH30) for n in scene_counts]
+    n = sum(s.n for s in mix)
+    r = np.float32(mix[0].params.particleRadius)
+    with Engine(mix if len(mix) > 1 else mix[0], use_graph=True) as e:
+        e.step(2)
+        e.sync()
+        lp, rp, bp, cp = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        nul = None
+        capi.check(L.arsim_engine_debug_buffers(e.h, nul, nul, nul, nul, nul, nul, C.byref(lp), C.byref(rp),
+                                                C.byref(bp), C.byref(cp)))
+        left = np.zeros(2 * n, np.int32)
+        right = np.zeros(2 * n, np.int32)
+        bounds = np.zeros((2 * n, 8), np.float32)
+        cms = np.zeros((n, 4), np.float32)
+        for host, dev in ((left, lp), (right, rp), (bounds, bp), (cms, cp)):
+            capi.check(L.arsim_memcpy_dtoh(capi.ptr(host), dev, host.nbytes, None))
+    m = 2 * n - 1
+    lo = np.full((m, 3), np.inf, np.float32)
+    hi = np.full((m, 3), -np.inf, np.float32)
+    lo[:n] = cms[:, :3] - r
+    hi[:n] = cms[:, :3] + r
+    L_, R_ = left[n:m], right[n:m]
+    assert L_.min() >= 0 and L_.max() < m and R_.min() >= 0 and R_.max() < m
+    for _ in range(200):   # bottom-up to a fixed point (tree depth passes)
+        nlo = np.minimum(lo[L_], lo[R_])
+        nhi = np.maximum(hi[L_], hi[R_])
+        if np.array_equal(nlo, lo[n:]) and np.array_equal(nhi, hi[n:]):
+            break
+        lo[n:], hi[n:] = nlo, nhi
+    else:
+        raise AssertionError("node boxes did not settle")
+    assert np.isfinite(lo).all() and np.isfinite(hi).all()
+    assert np.array_equal(bounds[:m, :3], lo) and np.array_equal(bounds[:m, 4:7], hi)
