@@ -637,12 +637,16 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
         pending = 0ull;
     }
     const unsigned *words = reinterpret_cast<const unsigned *>(packed);
+    // The next node's record is requested as soon as the next node is known (from the overlap tests and
+    // the stack), before this node's leaf children are tested, so that its load overlaps theirs.
+    unsigned wn = node >= 0 ? words[16 * (size_t)(node - n) + pl] : 0u;
 #ifdef ARSIM_TRAVERSAL_STATS
     const bool packetUsed = pending != 0ull || node >= 0;
     unsigned long long stVisits = 0, stLeafTests = 0, stInterest = 0, stContacts = 0, stIters = 0;
     int stMaxSp = 0;
 #endif
     while (true) {
+        bool fresh = false;   // a walk that starts here: its root's record is not requested yet
         if (FOREST) {
             // a packet without a walk starts the next pending root (all lanes take part in the exchange)
             const bool needRoot = node < 0 && pending != 0ull;
@@ -652,6 +656,7 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
                 root = cand;
                 node = cand;
                 sp = 0;
+                fresh = true;
             }
             pending &= ~(__ballot(needRoot && valid && myRoot == cand) & pmask);
         }
@@ -661,7 +666,7 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
         ++stIters;
         if (live) ++stVisits;
 #endif
-        const unsigned w = live ? words[16 * (size_t)(node - n) + pl] : 0u;
+        const unsigned w = fresh ? words[16 * (size_t)(node - n) + pl] : wn;
         const float4 a0 = make_float4(row_word<0>(w), row_word<1>(w), row_word<2>(w), 0.f);
         const float4 a1 = make_float4(row_word<4>(w), row_word<5>(w), row_word<6>(w), 0.f);
         const float4 b0 = make_float4(row_word<8>(w), row_word<9>(w), row_word<10>(w), 0.f);
@@ -670,13 +675,41 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
         const bool act = live && valid && myRoot == root;
         const bool oL = act & (L != i) & overlap(qMin, qMax, a0, a1);
         const bool oR = act & (R != i) & overlap(qMin, qMax, b0, b1);
+        const bool tL = L >= n && (__ballot(oL) & pmask) != 0ull;
+        const bool tR = R >= n && (__ballot(oR) & pmask) != 0ull;
 #ifdef ARSIM_TRAVERSAL_STATS
         stLeafTests += (L < n && oL ? 1 : 0) + (R < n && oR ? 1 : 0);
         if (L < n && oL && in_contact(pa, xyz(cms[L]), ph)) ++stContacts;
         if (R < n && oR && in_contact(pa, xyz(cms[R]), ph)) ++stContacts;
         if (L >= n) stInterest += (unsigned long long)__popcll(__ballot(oL) & pmask);
         if (R >= n) stInterest += (unsigned long long)__popcll(__ballot(oR) & pmask);
+        if (live && tL && tR) stMaxSp = sp + 1 > stMaxSp ? sp + 1 : stMaxSp;
 #endif
+        if (live) {
+            if (!tL && !tR) {
+                if (sp == 0) {
+                    node = -1;
+                } else {
+                    --sp;
+                    node = stack[sp];
+                }
+            } else if (tL && tR) {
+                if (sp == kWaveStackDepth) {
+                    if (act) {
+                        atomicOr(overflow, 1);
+                        ovf = true;
+                    }
+                    node = -1;
+                } else {
+                    if (pl == 0) stack[sp] = R;
+                    ++sp;
+                    node = L;
+                }
+            } else {
+                node = tL ? L : R;
+            }
+        }
+        wn = node >= 0 ? words[16 * (size_t)(node - n) + pl] : 0u;
         if (L < n && oL) {
             const float4 CL = cms[L];
             if (candidate(CL, pa, ra, false, ph)) {
@@ -703,35 +736,6 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
                     f3 fj;
                     if (pair_force(pa, xyz(CR), va, xyz(sVel[R]), ph, fj)) F = add3(F, fj);
                 }
-            }
-        }
-        const bool tL = L >= n && (__ballot(oL) & pmask) != 0ull;
-        const bool tR = R >= n && (__ballot(oR) & pmask) != 0ull;
-#ifdef ARSIM_TRAVERSAL_STATS
-        if (live && tL && tR) stMaxSp = sp + 1 > stMaxSp ? sp + 1 : stMaxSp;
-#endif
-        if (live) {
-            if (!tL && !tR) {
-                if (sp == 0) {
-                    node = -1;
-                } else {
-                    --sp;
-                    node = stack[sp];
-                }
-            } else if (tL && tR) {
-                if (sp == kWaveStackDepth) {
-                    if (act) {
-                        atomicOr(overflow, 1);
-                        ovf = true;
-                    }
-                    node = -1;
-                } else {
-                    if (pl == 0) stack[sp] = R;
-                    ++sp;
-                    node = L;
-                }
-            } else {
-                node = tL ? L : R;
             }
         }
     }
