@@ -518,6 +518,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         // node boxes: the step keeps them in the traversal records only (tree.bounds stays NULL);
         // treeBounds receives them when a debug view asks
         A(treeBounds, 2 * nodes); A(tree.cms, N); A(tree.sortedVel, N); A(tree.counter, N);
+        A(tree.crossList, (size_t)N); A(tree.crossCount, (size_t)N / 1024 + 1);
         if (S > 1) A(sceneRootBuf, S);
         tree.sceneRoot = S > 1 ? sceneRootBuf : nullptr;
         A(aabbPart, (size_t)2 * aabb_max_partials()); A(aabbBox, 2);

@@ -246,6 +246,11 @@ struct Lbvh {
     int4 *meta;
     float4 *pre, *suf, *blockTab;
     int blockTabLevels;
+    // optional (engine, with meta): the build lists the internal nodes whose range crosses a refit
+    // block: build workgroup g (1024 nodes) writes its count to crossCount[g] (ceil(n / 1024) entries)
+    // and its nodes to crossList[1024 g ...] (n entries in all)
+    int *crossList;
+    unsigned *crossCount;
 };
 // Workgroup (leaf block) size of the range refit and the sizes of its buffers for n leaves.
 int lbvh_refit_block();

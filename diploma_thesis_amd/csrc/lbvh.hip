@@ -1,5 +1,7 @@
 // lbvh.hip -- Karras LBVH: radix-tree build, leaf init + atomic bottom-up refit, and the
 // stack traversal with the DEM contact force.
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace arsim {
@@ -21,6 +23,10 @@ __device__ unsigned long long g_travStats[kTravStats];
 __device__ __forceinline__ int clz_key(unsigned v) { return __clz((int)v); }
 __device__ __forceinline__ int clz_key(uint64_t v) { return __clzll((long long)v); }
 
+// Leaf block of the engine's range refit: a node whose leaf range spans two blocks "crosses".
+constexpr int kBlk = 1024;
+__device__ __forceinline__ bool range_crosses(int lo, int hi) { return lo / kBlk != hi / kBlk; }
+
 // The codes a workgroup's searches read, [i0 - kKarrasHalo, i0 + kKarrasBlock + kKarrasHalo), are
 // staged in LDS: a node whose range is at most kKarrasHalo / 2 long (most of them) searches LDS only;
 // wider ranges read the rest from global memory.
@@ -41,6 +47,17 @@ __device__ __forceinline__ int lcp(const K *__restrict__ codes, const K *win, in
     return (int)sizeof(K) * 8 + __clz((int)((unsigned)(i - base) ^ (unsigned)(j - base)));
 }
 
+// Workgroup g of the build lists its nodes whose range crosses a refit block in
+// crossList[g * kKarrasBlock + j], j < crossCount[g] (the order within a workgroup is immaterial).
+// Every thread of the workgroup calls this once (it holds barriers).
+__device__ __forceinline__ void karras_list_crossing(const Lbvh &t, bool crosses, int i, unsigned &crossings) {
+    const unsigned j = crosses ? __hip_atomic_fetch_add(&crossings, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                               : 0u;
+    if (crosses) t.crossList[(size_t)blockIdx.x * kKarrasBlock + j] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) t.crossCount[blockIdx.x] = crossings;
+}
+
 // kernelConstructRadixTreeSoA (BVHSource/BVHSoAKernels.cu:7-99).  t = ceil(l / 2^k) is computed
 // with integer shifts instead of the reference's double division (identical for l < 2^31).
 template <class K>
@@ -53,49 +70,57 @@ __global__ __launch_bounds__(kKarrasBlock) void k_karras(const K *__restrict__ c
         const int j = w0 + q;
         if (j >= 0 && j < n) win[q] = codes[j];
     }
+    __shared__ unsigned crossings;
+    if (threadIdx.x == 0) crossings = 0;
     __syncthreads();
     const int i = i0 + (int)threadIdx.x;
     if (i == 0) t.parent[n] = -1;
-    if (i >= n - 1) return;
-    t.counter[i] = 0;
-    auto lcp = [&](K ki_, int i_, int j_, int base_) { return arsim::lcp(codes, win, w0, n, ki_, i_, j_, base_); };
-    const K ki = win[i - w0];
-    const int base = sceneShift > 0 ? sceneStart[(int)(ki >> sceneShift)] : 0;
-    const int dPrev = lcp(ki, i, i - 1, base);
-    const int dNext = lcp(ki, i, i + 1, base);
-    const int d = (dNext - dPrev) > 0 ? 1 : -1;
-    const int sigMin = lcp(ki, i, i - d, base);
-    long long lmax = 2;
-    while (lcp(ki, i, (int)(i + lmax * d), base) > sigMin) lmax *= 2;
-    long long l = 0;
-    for (long long s = lmax / 2; s >= 1; s /= 2) {
-        if (lcp(ki, i, (int)(i + (l + s) * d), base) > sigMin) l += s;
-        if (s == 1) break;
+    bool crosses = false;
+    if (i < n - 1) {
+        if (!t.meta) t.counter[i] = 0;   // the bottom-up refit's arrival counters (the range refit has none)
+        auto lcp = [&](K ki_, int i_, int j_, int base_) {
+            return arsim::lcp(codes, win, w0, n, ki_, i_, j_, base_);
+        };
+        const K ki = win[i - w0];
+        const int base = sceneShift > 0 ? sceneStart[(int)(ki >> sceneShift)] : 0;
+        const int dPrev = lcp(ki, i, i - 1, base);
+        const int dNext = lcp(ki, i, i + 1, base);
+        const int d = (dNext - dPrev) > 0 ? 1 : -1;
+        const int sigMin = lcp(ki, i, i - d, base);
+        long long lmax = 2;
+        while (lcp(ki, i, (int)(i + lmax * d), base) > sigMin) lmax *= 2;
+        long long l = 0;
+        for (long long s = lmax / 2; s >= 1; s /= 2) {
+            if (lcp(ki, i, (int)(i + (l + s) * d), base) > sigMin) l += s;
+            if (s == 1) break;
+        }
+        const int j = (int)(i + l * d);
+        const int sigNode = lcp(ki, i, j, base);
+        long long s = 0;
+        for (int k = 1;; ++k) {
+            long long tt = (l + (1ll << k) - 1) >> k;
+            if (lcp(ki, i, (int)(i + (s + tt) * d), base) > sigNode) s += tt;
+            if (tt == 1) break;
+        }
+        const int gamma = (int)(i + s * d + (d < 0 ? d : 0));
+        const int node = i + n;
+        const int lo = i < j ? i : j, hi = i < j ? j : i;
+        const int L = (lo == gamma) ? gamma : gamma + n;
+        const int R = (hi == gamma + 1) ? gamma + 1 : gamma + 1 + n;
+        t.left[node] = L;
+        t.right[node] = R;
+        t.parent[L] = node;
+        t.parent[R] = node;
+        if (t.minR) { t.minR[node] = lo; t.maxR[node] = hi; }
+        if (t.meta) t.meta[i] = make_int4(L, R, lo, hi);
+        if (t.isLeaf) t.isLeaf[node] = 0;
+        if (t.sceneRoot && sceneShift > 0) {
+            int sc = (int)(codes[lo] >> sceneShift);
+            if (lo == sceneStart[sc] && hi == sceneStart[sc] + sceneCount[sc] - 1) t.sceneRoot[sc] = node;
+        }
+        crosses = range_crosses(lo, hi);
     }
-    const int j = (int)(i + l * d);
-    const int sigNode = lcp(ki, i, j, base);
-    long long s = 0;
-    for (int k = 1;; ++k) {
-        long long tt = (l + (1ll << k) - 1) >> k;
-        if (lcp(ki, i, (int)(i + (s + tt) * d), base) > sigNode) s += tt;
-        if (tt == 1) break;
-    }
-    const int gamma = (int)(i + s * d + (d < 0 ? d : 0));
-    const int node = i + n;
-    const int lo = i < j ? i : j, hi = i < j ? j : i;
-    const int L = (lo == gamma) ? gamma : gamma + n;
-    const int R = (hi == gamma + 1) ? gamma + 1 : gamma + 1 + n;
-    t.left[node] = L;
-    t.right[node] = R;
-    t.parent[L] = node;
-    t.parent[R] = node;
-    if (t.minR) { t.minR[node] = lo; t.maxR[node] = hi; }
-    if (t.meta) t.meta[i] = make_int4(L, R, lo, hi);
-    if (t.isLeaf) t.isLeaf[node] = 0;
-    if (t.sceneRoot && sceneShift > 0) {
-        int sc = (int)(codes[lo] >> sceneShift);
-        if (lo == sceneStart[sc] && hi == sceneStart[sc] + sceneCount[sc] - 1) t.sceneRoot[sc] = node;
-    }
+    if (t.crossList) karras_list_crossing(t, crosses, i, crossings);   // every thread: it holds a barrier
 }
 
 // Node bounds that other workgroups read during the refit travel as agent-scope atomic stores and
@@ -221,14 +246,11 @@ __global__ __launch_bounds__(256) void k_leaf_refit(const int *__restrict__ sIdx
 //  3. k_refit_cross, one thread per internal node: a node whose range crosses a workgroup boundary
 //     takes suf[lo] ∪ pre[hi] ∪ (the full workgroups between, two table lookups), and the same for
 //     its crossing children, for its packed traversal record.
-constexpr int kBlk = 1024;
-
 __device__ __forceinline__ void store_plain(float4 *bounds, int node, float4 mn, float4 mx) {
     bounds[2 * (size_t)node] = mn;
     bounds[2 * (size_t)node + 1] = mx;
 }
 
-__device__ __forceinline__ bool range_crosses(int lo, int hi) { return lo / kBlk != hi / kBlk; }
 
 __device__ __forceinline__ void shfl_minmax(float4 &mn, float4 &mx, int d, bool up) {
     float v[6] = {mn.x, mn.y, mn.z, mx.x, mx.y, mx.z};
@@ -451,17 +473,32 @@ __device__ __forceinline__ void child_box(const Lbvh &t, int n, int numBlocks, f
     }
 }
 
-__global__ void k_refit_cross(int n, int numBlocks, float r, Lbvh t) {
-    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (k >= n - 1) return;
-    const int4 m = t.meta[k];
-    if (!range_crosses(m.z, m.w)) return;
+__device__ __forceinline__ void refit_crossing(int n, int numBlocks, float r, const Lbvh &t, int k, int4 m) {
     float4 mn, mx, aMin, aMax, bMin, bMax;
     range_box(t, numBlocks, m.z, m.w, mn, mx);
     child_box(t, n, numBlocks, r, m.x, aMin, aMax);
     child_box(t, n, numBlocks, r, m.y, bMin, bMax);
     if (t.bounds) store_plain(t.bounds, k + n, mn, mx);
     write_packed(t, n, k + n, m.x, m.y, aMin, aMax, bMin, bMax);
+}
+
+// Every internal node: those whose range crosses a block boundary are refitted.
+__global__ void k_refit_cross(int n, int numBlocks, float r, Lbvh t) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= n - 1) return;
+    const int4 m = t.meta[k];
+    if (!range_crosses(m.z, m.w)) return;
+    refit_crossing(n, numBlocks, r, t, k, m);
+}
+
+// The crossing nodes that k_karras listed: workgroup g takes build workgroup g's list.
+__global__ void k_refit_cross_list(int n, int numBlocks, float r, Lbvh t) {
+    const unsigned cnt = t.crossCount[blockIdx.x];
+    const int *list = t.crossList + (size_t)blockIdx.x * kKarrasBlock;
+    for (unsigned e = threadIdx.x; e < cnt; e += blockDim.x) {
+        const int k = list[e];
+        refit_crossing(n, numBlocks, r, t, k, t.meta[k]);
+    }
 }
 
 // Node boxes for callers that read them (the engine's step keeps none): the boxes k_refit_local and
@@ -831,7 +868,13 @@ void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *ve
                            radius, n, t);
         if (numBlocks > 1) {
             hipLaunchKernelGGL(k_refit_table, dim3(1), dim3(1024), 0, st, numBlocks, lbvh_refit_levels(n), t.blockTab);
-            hipLaunchKernelGGL(k_refit_cross, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, numBlocks, radius, t);
+            if (t.crossList) {
+                const int blocks = (n - 1 + kKarrasBlock - 1) / kKarrasBlock;   // the build's workgroups
+                hipLaunchKernelGGL(k_refit_cross_list, dim3(blocks), dim3(64), 0, st, n, numBlocks, radius, t);
+            } else {
+                hipLaunchKernelGGL(k_refit_cross, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, numBlocks, radius,
+                                   t);
+            }
         }
         return;
     }
