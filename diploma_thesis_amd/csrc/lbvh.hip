@@ -722,30 +722,19 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
         if (R >= n) stInterest += (unsigned long long)__popcll(__ballot(oR) & pmask);
         if (live && tL && tR) stMaxSp = sp + 1 > stMaxSp ? sp + 1 : stMaxSp;
 #endif
-        if (live) {
-            if (!tL && !tR) {
-                if (sp == 0) {
-                    node = -1;
-                } else {
-                    --sp;
-                    node = stack[sp];
-                }
-            } else if (tL && tR) {
-                if (sp == kWaveStackDepth) {
-                    if (act) {
-                        atomicOr(overflow, 1);
-                        ovf = true;
-                    }
-                    node = -1;
-                } else {
-                    if (pl == 0) stack[sp] = R;
-                    ++sp;
-                    node = L;
-                }
-            } else {
-                node = tL ? L : R;
-            }
+        // next node: the left child entered (the right one pushed when both are), else the right one,
+        // else the top of the stack; selects rather than branches (every value is uniform in a packet)
+        const bool both = tL && tR, full = both && sp == kWaveStackDepth;
+        const int top = stack[sp > 0 ? sp - 1 : 0];
+        if (live && full && act) {
+            atomicOr(overflow, 1);
+            ovf = true;
         }
+        if (live && both && !full && pl == 0) stack[sp] = R;
+        const int next = full ? -1 : (tL ? L : (tR ? R : (sp > 0 ? top : -1)));
+        const int dsp = (both && !full) ? 1 : ((!tL && !tR && sp > 0) ? -1 : 0);
+        node = live ? next : node;
+        sp = live ? sp + dsp : sp;
         wn = node >= 0 ? words[16 * (size_t)(node - n) + pl] : 0u;
         if (L < n && oL) {
             const float4 CL = cms[L];
