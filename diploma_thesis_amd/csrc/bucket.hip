@@ -178,7 +178,11 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
     const int j = g - bs.bucketBase;
     const int lowBits = bs.shift;
     const unsigned *hist = to.hist + bs.bucketBase;
-    const unsigned start = bs.pStart + block_sum(t < j ? hist[t] : 0u, sWave);
+    // the bucket counts before this one and the tile map's words are independent: both loads in flight
+    const int numTiles = bs.tiles;
+    const unsigned hPrev = t < j ? hist[t] : 0u;
+    const unsigned tileWord = t < numTiles ? to.info[(size_t)bs.infoBase + (size_t)j * numTiles + t] : 0u;
+    const unsigned start = bs.pStart + block_sum(hPrev, sWave);
     const int cnt = (int)hist[j];
     constexpr int kCap = kBkCap < (1 << POSBITS) ? kBkCap : (1 << POSBITS);
     constexpr unsigned kPosMask = (1u << POSBITS) - 1u;
@@ -200,14 +204,8 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
     }
     // The bucket's keys are the runs of bucket j in the scene's tiles, in tile order: tile k's run
     // starts at bucket position sOff[k] and at slot sSrc[k] of the tile-ordered arrays.
-    const int numTiles = bs.tiles;
     {
-        unsigned c = 0u, f = 0u;
-        if (t < numTiles) {
-            const unsigned w = to.info[(size_t)bs.infoBase + (size_t)j * numTiles + t];
-            c = w >> 16;
-            f = w & 0xffffu;
-        }
+        const unsigned c = tileWord >> 16, f = tileWord & 0xffffu;
         const unsigned off = block_excl_scan(c, sWave);
         if (t < numTiles) {
             sOff[t] = off;
@@ -245,8 +243,24 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
     };
     const unsigned base = (unsigned)c0;
     if (inLds) {
-        for (int i = t; i < cnt; i += kBkThreads)
-            sA[i] = ((tKeys[slot((unsigned)i)] - base) << POSBITS) | (unsigned)i;
+        // every slot found first (LDS only), then every key load in flight at once
+        constexpr int kPer = (kCap + kBkThreads - 1) / kBkThreads;
+        unsigned sl[kPer], kv[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = t + q * kBkThreads;
+            sl[q] = i < cnt ? slot((unsigned)i) : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = t + q * kBkThreads;
+            kv[q] = i < cnt ? tKeys[sl[q]] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = t + q * kBkThreads;
+            if (i < cnt) sA[i] = ((kv[q] - base) << POSBITS) | (unsigned)i;
+        }
         __syncthreads();
         unsigned *src = sA, *dst = sB;
         for (int sh = 0; sh < lowBits; sh += 8) {
