@@ -208,7 +208,7 @@ __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, 
             if (ko.keys) ((unsigned *)ko.keys)[i] = (unsigned)key;
         } else if (MODE == kKeyMorton63) {
             key = morton63(u, v, w);
-            ((uint64_t *)ko.keys)[i] = key;
+            if (ko.keys) ((uint64_t *)ko.keys)[i] = key;
         } else {
             key = ((uint64_t)(unsigned)s << 30) | (uint64_t)morton30(u, v, w);
             ((uint64_t *)ko.keys)[i] = key;
@@ -269,13 +269,16 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
     const long long waveBase = tileBase + (long long)wave * (kTileItems * 64);
     KeyOut ko{};
     ko.grainVelDead = grainVelDead != 0;
+    if (MODE == kKeyMorton63) ko.keys = to.code63;   // the full codes, in particle order
     unsigned keys[kTileItems];
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
         const long long i = waveBase + k * 64 + lane;
-        keys[k] = i < tileEnd ? (unsigned)pose_key_one<MODE, kPoseGeomScene>((int)i, p, b, N, ph, sv, ko, 1, 1, nullptr,
-                                                                              scene)
-                              : 0u;
+        const uint64_t key =
+            i < tileEnd ? pose_key_one<MODE, kPoseGeomScene>((int)i, p, b, N, ph, sv, ko, 1, 1, nullptr, scene) : 0ull;
+        // a 63-bit code's top 30 bits are the 30-bit code of the same position (both scale the
+        // normalised coordinate by a power of two and truncate)
+        keys[k] = MODE == kKeyMorton63 ? (unsigned)(key >> 33) : (unsigned)key;
     }
     __syncthreads();
     // ranks within the wave, in particle order
@@ -579,6 +582,11 @@ void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticle
     if (numParticles <= 0) return;
     const int g = grainVelDead ? 1 : 0;
     const dim3 grid(to.numTiles);
+    if (to.morton == 2) {   // 63-bit Morton codes: tiles ranked by the top 10 bits
+        hipLaunchKernelGGL((k_pose_key_tiles<kKeyMorton63, 1024, 10>), grid, dim3(1024), 0, st, p, b, numParticles, ph,
+                           sv, to, g);
+        return;
+    }
     if (to.morton) {   // 30-bit Morton keys, buckets of their top 10 bits
         hipLaunchKernelGGL((k_pose_key_tiles<kKeyMorton30, 1024, 10>), grid, dim3(1024), 0, st, p, b, numParticles, ph,
                            sv, to, g);

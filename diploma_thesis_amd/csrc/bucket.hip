@@ -350,7 +350,43 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
     }
 }
 
+// One thread per sorted position.  A position alone in its run of equal top bits keeps its place;
+// the members of a longer run rank themselves among the run (O(run length) each).
+__global__ void k_morton63_ties(const unsigned *__restrict__ k30, const unsigned *__restrict__ vIn,
+                                const uint64_t *__restrict__ code63, int n, uint64_t *__restrict__ kOut,
+                                unsigned *__restrict__ vOut) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const unsigned c = k30[i];
+    const unsigned v = vIn[i];
+    const uint64_t key = code63[v];
+    const bool prevSame = i > 0 && k30[i - 1] == c, nextSame = i + 1 < n && k30[i + 1] == c;
+    if (!prevSame && !nextSame) {
+        kOut[i] = key;
+        vOut[i] = v;
+        return;
+    }
+    int a = i;
+    while (a > 0 && k30[a - 1] == c) --a;
+    int e = i + 1;
+    while (e < n && k30[e] == c) ++e;
+    int rank = 0;
+    for (int j = a; j < e; ++j) {
+        if (j == i) continue;
+        const uint64_t kj = code63[vIn[j]];
+        rank += (kj < key || (kj == key && j < i)) ? 1 : 0;
+    }
+    kOut[a + rank] = key;
+    vOut[a + rank] = v;
+}
+
 }  // namespace
+
+void launch_morton63_ties(const unsigned *k30, const unsigned *vIn, const uint64_t *code63, int n, uint64_t *kOut,
+                          unsigned *vOut, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_morton63_ties, dim3((n + 255) / 256), dim3(256), 0, st, k30, vIn, code63, n, kOut, vOut);
+}
 
 bool plan_bucket_sort(const std::vector<BucketSceneShape> &shapes, BucketPlan *out, int maxShift, int capKeys) {
     const size_t S = shapes.size();

@@ -465,10 +465,12 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
             geoms[s].cellBase = (unsigned)base;
             base += sceneCells[s];
         }
-    } else if (method == ARSIM_METHOD_LBVH30 && S == 1) {
-        keyBits = 30; keyBytes = 4;
+    } else if ((method == ARSIM_METHOD_LBVH30 || method == ARSIM_METHOD_LBVH63) && S == 1) {
+        const bool m63 = method == ARSIM_METHOD_LBVH63;
+        keyBits = m63 ? 63 : 30; keyBytes = m63 ? 8 : 4;
         // the codes sorted per bucket of their top 10 bits (tiles of 4096, fixed Morton box): 1,024 buckets
-        // of 2^20 codes, a bucket's positions in 12 bits of an LDS word
+        // of 2^20 codes, a bucket's positions in 12 bits of an LDS word.  63-bit codes: the same sort of
+        // their top 30 bits (the 30-bit code), then the ties among equal top bits ranked by the full code
         if (!(flags & ARSIM_DYNAMIC_AABB) && N > 65536) {
             std::vector<BucketSceneShape> shapes(1);
             shapes[0].particles = N;
@@ -479,7 +481,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
                            bkPlan.tileKeys == kTileKeys;
         }
     } else if (method == ARSIM_METHOD_LBVH63) {
-        keyBits = 63; keyBytes = 8;
+        keyBits = 63; keyBytes = 8;   // a forest of 63-bit codes
     } else {
         keyBits = 30 + sceneBits; keyBytes = 8;
     }
@@ -519,6 +521,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         // treeBounds receives them when a debug view asks
         A(treeBounds, 2 * nodes); A(tree.cms, N); A(tree.sortedVel, N); A(tree.counter, N);
         A(tree.crossList, (size_t)N); A(tree.crossCount, (size_t)N / 1024 + 1);
+        if (mortonBucket && method == ARSIM_METHOD_LBVH63) A(code63, (size_t)N);
         if (S > 1) A(sceneRootBuf, S);
         tree.sceneRoot = S > 1 ? sceneRootBuf : nullptr;
         A(aabbPart, (size_t)2 * aabb_max_partials()); A(aabbBox, 2);
@@ -822,28 +825,38 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         rec();
         TileKeyOut to{(unsigned *)keysB, valsB, tileInfo, bucketHist, bkScenesD, tileSceneD, bucketSceneD,
                       numTiles, numBuckets, tileKeys, bkPlan.bucketBits};
-        to.morton = 1;
+        const bool m63 = method == ARSIM_METHOD_LBVH63;
+        to.morton = m63 ? 2 : 1;
+        to.code63 = code63;
         launch_pose_key_tiles(pb, bb, N, ph, sv, to, false, st);
         rec();
         // (the tree's leaf arrays are the second buffer of a bucket sorted in global memory; the refit
         // writes them afterwards)
         launch_bucket_sort_morton(to, (unsigned *)keysA, valsA, tree.cms, tree.sortedVel, st);
+        // 63-bit codes: sorted codes and indices into (keysB, valsB), the tile arrays being free now
+        void *kS = keysA;
+        unsigned *vS = valsA;
+        if (m63) {
+            launch_morton63_ties((const unsigned *)keysA, valsA, code63, N, (uint64_t *)keysB, valsB, st);
+            kS = keysB;
+            vS = valsB;
+        }
         rec();
-        launch_lbvh_build(keysA, 4, N, 0, sceneStartD, sceneCountD, S, tree, st);
+        launch_lbvh_build(kS, keyBytes, N, 0, sceneStartD, sceneCountD, S, tree, st);
         rec();
-        launch_lbvh_refit((const int *)valsA, pos, vel, rb, nullptr, gatherRec, ph.r, N, tree, st);
+        launch_lbvh_refit((const int *)vS, pos, vel, rb, nullptr, gatherRec, ph.r, N, tree, st);
         rec();
         DemOut o{};
         o.pForce = pForce; o.vel = vel; o.relPos = relPos;
-        launch_dem_lbvh((const int *)valsA, tree, N, ph, sv, false, o, tu.demBlock, st, overflowD);
+        launch_dem_lbvh((const int *)vS, tree, N, ph, sv, false, o, tu.demBlock, st, overflowD);
         if (numStatic)
-            launch_dem_static(tree.cms, tree.sortedVel, valsA, N, stPos, stNorm, nullptr, stTable, nullptr, ph,
+            launch_dem_static(tree.cms, tree.sortedVel, vS, N, stPos, stNorm, nullptr, stTable, nullptr, ph,
                               geoms[0], pForce, nullptr, relPos, false, st);
         rec();
         launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
         rec();
-        lastKeys = keysA;
-        lastIdx = valsA;
+        lastKeys = kS;
+        lastIdx = vS;
     } else {
         const SortPlan pl = sort_plan(sortWs, N, keyBytes, 0, keyBits, stt);
         const bool fused = pl.passes <= 8;   // the key kernel holds at most 8 digit histograms

@@ -78,6 +78,7 @@ struct Engine {
     size_t sortWsBytes = 0;
     Lbvh tree{};
     float4 *treeBounds = nullptr;   // node boxes for debug views (the step keeps tree.bounds NULL)
+    uint64_t *code63 = nullptr;     // LBVH63 bucket path: the 63-bit codes in particle order
     int *overflowD = nullptr, *flagD = nullptr;
     float4 *aabbPart = nullptr, *aabbBox = nullptr;
     // static collider set (one-scene engines): points and normals sorted by cell, exact cell table

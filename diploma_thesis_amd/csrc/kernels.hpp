@@ -165,7 +165,10 @@ struct TileKeyOut {
     const int *tileScene, *bucketScene;   // scene of each tile and of each bucket
     int numTiles, numBuckets, tileKeys;
     int bucketBits;                       // a scene has at most 2^bucketBits buckets (8, 9 or 10)
-    int morton = 0;                       // keys are 30-bit Morton codes (one scene, 1,024 buckets of 2^20 codes)
+    // Morton keys (one scene, 1,024 buckets of 2^20 codes): 1 = 30-bit codes; 2 = 63-bit codes, whose
+    // top 30 bits (the 30-bit code) are the tile keys while the full codes go to code63 in particle order
+    int morton = 0;
+    uint64_t *code63 = nullptr;
 };
 void launch_pose_key_tiles(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,
                            const SceneView &sv, const TileKeyOut &to, bool grainVelDead, hipStream_t st);
@@ -204,6 +207,12 @@ void launch_bucket_sort(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, co
 // The same for 30-bit Morton keys ranked by their top 10 bits (TileKeyOut.morton): each bucket of up to
 // 4,096 keys sorted by its low 20 bits in LDS, written as sorted keys and indices only; a larger bucket is
 // sorted in global memory with scratchA / scratchB (16 B per key, N entries) as its second buffer.
+// The 63-bit order from the stable sort by the codes' top 30 bits: code63 is in particle order, k30 /
+// vIn the sorted top 30 bits and particle indices.  Writes the sorted 63-bit codes and indices; within
+// a run of equal top bits the members are ranked by their full codes, equal codes keeping the run's
+// (particle) order, so the result is the stable sort by the 63-bit code.
+void launch_morton63_ties(const unsigned *k30, const unsigned *vIn, const uint64_t *code63, int n, uint64_t *kOut,
+                          unsigned *vOut, hipStream_t st);
 void launch_bucket_sort_morton(const TileKeyOut &to, unsigned *kOut, unsigned *vOut, float4 *scratchA,
                                float4 *scratchB, hipStream_t st);
 // Capacity of the long-run list of a table of tableSize entries (runs are disjoint and longer than

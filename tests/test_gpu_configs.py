@@ -28,12 +28,14 @@ def test_benchmark_scene_lbvh_two_steps_bit_exact(method):
     assert_state_equal(got, st, f"benchmark scene, method {method}")
 
 
-def test_lbvh30_bucket_past_the_lds_capacity():
-    """100,000 particles with 25,000 grains packed into one corner: their 30-bit codes share the top 10 bits,
+@pytest.mark.parametrize("method", [capi.METHOD_LBVH30, capi.METHOD_LBVH63])
+def test_lbvh_bucket_past_the_lds_capacity(method):
+    """100,000 particles with 25,000 grains packed into one corner: their codes share the top 10 bits,
     so one bucket of the per-bucket Morton sort holds far more keys than LDS does (4,096) and is sorted in
-    global memory.  Two graph-replayed steps equal the oracle's."""
-    print("engine: LBVH30 Morton bucket past the LDS capacity, 2 steps", flush=True)
-    sc = scenes.mix_scene(100000, 33, capi.METHOD_LBVH30)
+    global memory; with 63-bit codes, grains sharing the top 30 bits are then ranked by the full code.
+    Two graph-replayed steps equal the oracle's."""
+    print(f"engine: Morton bucket past the LDS capacity, method {method}, 2 steps", flush=True)
+    sc = scenes.mix_scene(100000, 33, method)
     g = np.where(sc.rb < 0)[0]
     assert len(g) >= 20000
     rng = np.random.default_rng(33)
@@ -47,7 +49,28 @@ def test_lbvh30_bucket_past_the_lds_capacity():
     assert "bucket_sort" in stages, stages
     st = oracle.State(sc, np.float32)
     st.step(2)
-    assert_state_equal(got, st, "Morton bucket past the LDS capacity")
+    assert_state_equal(got, st, f"Morton bucket past the LDS capacity, method {method}")
+
+
+@pytest.mark.parametrize("method", [capi.METHOD_LBVH30, capi.METHOD_LBVH63])
+def test_lbvh_bucket_long_runs_of_equal_top_bits(method):
+    """3,000 grains inside a cube a third of a particle radius wide: they share a few 30-bit codes, so the
+    63-bit path ranks runs of hundreds of equal top bits by their full codes.  A step equals the oracle's."""
+    print(f"engine: Morton bucket, long runs of equal 30-bit codes, method {method}", flush=True)
+    sc = scenes.mix_scene(100000, 34, method)
+    g = np.where(sc.rb < 0)[0][:3000]
+    rng = np.random.default_rng(34)
+    lo = sc.pos[:, :3].min(0)
+    sc.pos[g, :3] = (lo + 1.0 + rng.uniform(0.0, 0.15, (len(g), 3))).astype(np.float32)
+    with Engine(sc, use_graph=True) as e:
+        e.step(1)
+        e.sync()
+        got = e.download()
+        stages = [name for name, _ in e.profile()]
+    assert "bucket_sort" in stages, stages
+    st = oracle.State(sc, np.float32)
+    st.step(1)
+    assert_state_equal(got, st, f"long runs of equal 30-bit codes, method {method}")
 
 
 def test_benchmark_batch_lbvh_forest_one_step():
