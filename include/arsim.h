@@ -430,7 +430,9 @@ int arsim_engine_download(arsim_engine *e, int sceneIdx,
 /* Per-stage GPU time (ms) of the last profiled step: measured with HIP events on the engine
  * stream.  names[k] is a static string.  Returns the number of stages. */
 int arsim_engine_profile_step(arsim_engine *e, float *msPerStage, const char **names, int maxStages);
-/* Debug views of the last step's broadphase (device pointers, valid until the next step). */
+/* Debug views of the last step's broadphase (device pointers, valid until the next step).  The LBVH
+ * node boxes are not kept by the step (the traversal reads the packed child boxes): asking for
+ * lbvhBounds restates them from the leaf centres and the packed records and waits for that. */
 int arsim_engine_debug_buffers(arsim_engine *e, void **sortedKeys, void **sortedIndex,
                                void **cellStart, void **cellEnd, void **sortedPos, void **sortedVel,
                                void **lbvhLeft, void **lbvhRight, void **lbvhBounds, void **lbvhCMs);
