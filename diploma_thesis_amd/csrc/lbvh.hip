@@ -369,15 +369,16 @@ __global__ __launch_bounds__(kBlk) void k_refit_local(const int *__restrict__ sI
     }
 }
 
-// Level l, entry b: box of workgroups [b, b + 2^l).  One workgroup builds the levels in order; up to
-// kTabLds workgroup boxes, each level is computed from the previous one held in LDS (as six planes
-// of floats), else read back from global memory.
+// Level l, entry b: box of workgroups [b, b + 2^l).  Up to kTabLds workgroup boxes one workgroup builds
+// the levels in order, each from the previous one held in LDS (as six planes of floats); larger tables
+// take a launch per level (k_refit_table_level).
 constexpr int kTabLds = 4096;
 constexpr int kTabPer = kTabLds / 1024;   // entries per thread
 __global__ __launch_bounds__(1024) void k_refit_table(int numBlocks, int levels, float4 *__restrict__ tab) {
     __shared__ float sBox[6][kTabLds];
     const int tid = (int)threadIdx.x;
-    if (numBlocks <= kTabLds) {
+    if (numBlocks > kTabLds) return;   // larger tables: k_refit_table_level
+    {
         for (int b = tid; b < numBlocks; b += 1024) {
             const float4 mn = tab[2 * b], mx = tab[2 * b + 1];
             sBox[0][b] = mn.x; sBox[1][b] = mn.y; sBox[2][b] = mn.z;
@@ -412,18 +413,18 @@ __global__ __launch_bounds__(1024) void k_refit_table(int numBlocks, int levels,
             }
             __syncthreads();
         }
-        return;
     }
-    for (int l = 1; l < levels; ++l) {
-        const float4 *src = tab + 2 * (size_t)(l - 1) * numBlocks;
-        float4 *dst = tab + 2 * (size_t)l * numBlocks;
-        const int half = 1 << (l - 1), cnt = numBlocks - (1 << l) + 1;
-        for (int b = tid; b < cnt; b += (int)blockDim.x) {
-            dst[2 * b] = fmin4(src[2 * b], src[2 * (b + half)]);
-            dst[2 * b + 1] = fmax4(src[2 * b + 1], src[2 * (b + half) + 1]);
-        }
-        __syncthreads();
-    }
+}
+
+// Level l of the table from level l - 1, one thread per entry (tables of more than kTabLds blocks).
+__global__ void k_refit_table_level(int numBlocks, int l, float4 *__restrict__ tab) {
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int half = 1 << (l - 1), cnt = numBlocks - (1 << l) + 1;
+    if (b >= cnt) return;
+    const float4 *src = tab + 2 * (size_t)(l - 1) * numBlocks;
+    float4 *dst = tab + 2 * (size_t)l * numBlocks;
+    dst[2 * b] = fmin4(src[2 * b], src[2 * (b + half)]);
+    dst[2 * b + 1] = fmax4(src[2 * b + 1], src[2 * (b + half) + 1]);
 }
 
 // Box of the leaves [lo, hi] that span at least two workgroups.
@@ -856,7 +857,18 @@ void launch_lbvh_refit(const int *sortedIdx, const float4 *pos, const float4 *ve
         hipLaunchKernelGGL(k_refit_local, dim3(numBlocks), dim3(kBlk), 0, st, sortedIdx, pos, vel, rb, scene, rec,
                            radius, n, t);
         if (numBlocks > 1) {
-            hipLaunchKernelGGL(k_refit_table, dim3(1), dim3(1024), 0, st, numBlocks, lbvh_refit_levels(n), t.blockTab);
+            const int levels = lbvh_refit_levels(n);
+            if (numBlocks <= kTabLds) {
+                hipLaunchKernelGGL(k_refit_table, dim3(1), dim3(1024), 0, st, numBlocks, levels, t.blockTab);
+            } else {
+                // one launch per level: a single workgroup walking thousands of entries per level waits a
+                // global-memory round trip per handful of them
+                for (int l = 1; l < levels; ++l) {
+                    const int cnt = numBlocks - (1 << l) + 1;
+                    hipLaunchKernelGGL(k_refit_table_level, dim3((cnt + 255) / 256), dim3(256), 0, st, numBlocks, l,
+                                       t.blockTab);
+                }
+            }
             if (t.crossList) {
                 const int blocks = (n - 1 + kKarrasBlock - 1) / kKarrasBlock;   // the build's workgroups
                 hipLaunchKernelGGL(k_refit_cross_list, dim3(blocks), dim3(64), 0, st, n, numBlocks, radius, t);
