@@ -2,6 +2,8 @@
 oracle: the 2^20-particle scene with the 30- and 63-bit LBVH, the 512-scene batch as one LBVH forest,
 and the full 4,194,304-particle dense pile.  The engine's synchronize reports an LBVH traversal-stack
 overflow as an error, so a passing LBVH test also shows that no traversal overflowed."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -123,3 +125,37 @@ def test_dense_pile_full_size_one_step(method):
     ref = oracle.State(sc, np.float32)
     ref.step(1)
     assert_state_equal(got, ref, f"dense pile, method {method}")
+
+
+@pytest.mark.parametrize("method", [capi.METHOD_LBVH30, capi.METHOD_LBVH63])
+def test_lbvh_bucket_sort_stable_on_equal_codes(method):
+    """Grains stacked five deep on identical positions give equal codes, 30- and 63-bit alike: after a
+    step the engine's sorted codes are non-decreasing, every particle appears once, and equal codes
+    keep particle order (the stable sort).  The state itself is not compared: coincident grains have
+    no contact normal."""
+    print(f"engine: Morton bucket sort stability on equal codes, method {method}", flush=True)
+    sc = scenes.mix_scene(100000, 35, method)
+    g = np.where(sc.rb < 0)[0]
+    groups = g[: (len(g) // 5) * 5].reshape(-1, 5)
+    sc.pos[groups[:, 1:], :3] = sc.pos[groups[:, :1], :3]
+    sc.vel[groups[:, 1:], :3] = sc.vel[groups[:, :1], :3]   # still together after the step's integration
+    n = sc.n
+    with Engine(sc, use_graph=False) as e:
+        e.step(1)
+        e.sync()
+        kp, ip = C.c_void_p(), C.c_void_p()
+        nul = None
+        capi.check(capi.lib.arsim_engine_debug_buffers(e.h, C.byref(kp), C.byref(ip), nul, nul, nul, nul, nul, nul,
+                                                       nul, nul))
+        keys = np.zeros(n, np.uint64 if method == capi.METHOD_LBVH63 else np.uint32)
+        idx = np.zeros(n, np.uint32)
+        capi.check(capi.lib.arsim_memcpy_dtoh(capi.ptr(keys), kp, keys.nbytes, None))
+        capi.check(capi.lib.arsim_memcpy_dtoh(capi.ptr(idx), ip, idx.nbytes, None))
+        stages = [name for name, _ in e.profile()]
+    assert "bucket_sort" in stages, stages
+    k = keys.astype(np.uint64)
+    assert np.all(k[1:] >= k[:-1])
+    assert np.array_equal(np.sort(idx), np.arange(n, dtype=np.uint32))
+    eq = k[1:] == k[:-1]
+    assert eq.sum() >= len(groups) * 4   # the stacked grains share their codes
+    assert np.all(idx[1:][eq] > idx[:-1][eq])
