@@ -134,6 +134,49 @@ __device__ __forceinline__ void sincos_det(float x, float *s_out, float *c_out) 
     *c_out = (float)cv;
 }
 
+// One step of the rigid-body integrator (GPUintegratorKernel, Kernels/rigidBodyKernels.cu:262-348): position by
+// the old velocity, the incremental rotation from the angular velocity (theta = |w| dt), the
+// cumulative orientation qc = normalize(qi * qc), gravity and angular damping.
+struct BodyStep {
+    f3 x, v, w;
+    qf qi, qc;
+};
+__device__ __forceinline__ BodyStep body_step(float4 X, float4 V, float4 W, float4 Qc, const DevPhys &ph) {
+    BodyStep r;
+    const f3 v = xyz(V), w = xyz(W);
+    r.x = add3(xyz(X), mk3(v.x * ph.dt, v.y * ph.dt, v.z * ph.dt));
+    const float w2 = dot3(w, w);
+    const float theta = sqrtf(w2) * ph.dt;
+    qf qi{0.f, 0.f, 0.f, 1.f};
+    if (theta > 1e-5f) {
+        const float inv = 1.0f / sqrtf(w2);
+        const f3 n = mk3(w.x * inv, w.y * inv, w.z * inv);
+        float sh, ch;
+        sincos_det(theta / 2.0f, &sh, &ch);
+        qi = qf{sh * n.x, sh * n.y, sh * n.z, ch};
+    }
+    r.qi = qnorm(qi);
+    r.qc = qnorm(qmul(r.qi, qf{Qc.x, Qc.y, Qc.z, Qc.w}));
+    const bool ref = (ph.flags & kRefUnits) != 0;
+    const f3 gs = ref ? mk3(ph.gx, ph.gy, ph.gz) : mk3(ph.gx * ph.dt, ph.gy * ph.dt, ph.gz * ph.dt);
+    r.v = add3(v, gs);
+    r.w = mk3(w.x * ph.Wdamping, w.y * ph.Wdamping, w.z * ph.Wdamping);
+    return r;
+}
+// R = mat3_cast(qc) and the world inverse inertia I_w^-1 = R I_b^-1 R^T (Ib: the body's 9 floats).
+__device__ __forceinline__ void body_world_inertia(qf qc, const float *Ib, float R[9], float Iw[9]) {
+    float RT[9], B[9], tmp[9];
+    mat3_cast(qc, R);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) RT[c * 3 + r] = R[r * 3 + c];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) B[k] = Ib[k];
+    mat_mul(R, B, tmp);
+    mat_mul(tmp, RT, Iw);
+}
+
 // ----------------------------------------------------------------------------- DEM
 // Spring-dashpot-shear(-attraction) pair force of collideSpheres (particleKernels.cu:261-299).
 __device__ __forceinline__ bool pair_force(f3 pa, f3 pb, f3 va, f3 vb, const DevPhys &ph, f3 &out) {

@@ -9,69 +9,39 @@ namespace {
 
 constexpr int kMaxHistWords = 8 * 256;   // up to 8 digit passes of up to 8 bits
 
+// Zeroes the ranges of z with blocks [0, blocks) of the caller's extra blocks.
+__device__ __forceinline__ void zero_jobs(const ZeroJobs &z, int block, int blocks) {
+    const size_t t = (size_t)block * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)blocks * blockDim.x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        for (size_t e = t; e < z.count[k]; e += stride) z.p[k][e] = 0u;
+}
+
 // GPUintegratorKernel (Kernels/rigidBodyKernels.cu:262-348), with the orientation carried as a
 // cumulative quaternion (q_cum = normalize(q_inc * q_cum)) and I_w^-1 = R_cum I_b^-1 R_cum^T.
 // Blocks past bodyBlocks zero the given ranges (the next sort's histograms, tile counters and
 // first look-back array, the cell table's long-run count), which saves those a launch of their own.
-__global__ void k_body_integrate(BodyBufs b, int B, DevPhys ph, ZeroJobs z, int bodyBlocks) {
+__global__ void k_body_integrate(BodyBufs b, BodyBufs o, int B, DevPhys ph, ZeroJobs z, int bodyBlocks) {
     if ((int)blockIdx.x >= bodyBlocks) {
-        const size_t t = (size_t)((int)blockIdx.x - bodyBlocks) * blockDim.x + threadIdx.x;
-        const size_t stride = (size_t)((int)gridDim.x - bodyBlocks) * blockDim.x;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            for (size_t e = t; e < z.count[k]; e += stride) z.p[k][e] = 0u;
+        zero_jobs(z, (int)blockIdx.x - bodyBlocks, (int)gridDim.x - bodyBlocks);
         return;
     }
     int bi = blockIdx.x * blockDim.x + threadIdx.x;
     if (bi >= B) return;
-    const bool ref = (ph.flags & kRefUnits) != 0;
-    float4 X = b.pos[bi], V = b.vel[bi], W = b.ang[bi];
-    f3 x = xyz(X), v = xyz(V), w = xyz(W);
-    x = add3(x, mk3(v.x * ph.dt, v.y * ph.dt, v.z * ph.dt));
-    float w2 = dot3(w, w);
-    float theta = sqrtf(w2) * ph.dt;
-    qf qi{0.f, 0.f, 0.f, 1.f};
-    if (theta > 1e-5f) {
-        float inv = 1.0f / sqrtf(w2);
-        f3 n = mk3(w.x * inv, w.y * inv, w.z * inv);
-        float sh, ch;
-        sincos_det(theta / 2.0f, &sh, &ch);
-        qi = qf{sh * n.x, sh * n.y, sh * n.z, ch};
-    }
-    qi = qnorm(qi);
-    float4 Qc = b.qcum[bi];
-    qf qc = qnorm(qmul(qi, qf{Qc.x, Qc.y, Qc.z, Qc.w}));
-    float R[9], RT[9], Ib[9], tmp[9], Iw[9];
-    mat3_cast(qc, R);
+    const float4 X = b.pos[bi], V = b.vel[bi], W = b.ang[bi];
+    const BodyStep bs = body_step(X, V, W, b.qcum[bi], ph);
+    float R[9], Iw[9];
+    body_world_inertia(bs.qc, b.Ib + (size_t)bi * 9, R, Iw);
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int r = 0; r < 3; ++r) RT[c * 3 + r] = R[r * 3 + c];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Ib[k] = b.Ib[(size_t)bi * 9 + k];
-    mat_mul(R, Ib, tmp);
-    mat_mul(tmp, RT, Iw);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) b.Iw[(size_t)bi * 9 + k] = Iw[k];
-    f3 gs = ref ? mk3(ph.gx, ph.gy, ph.gz) : mk3(ph.gx * ph.dt, ph.gy * ph.dt, ph.gz * ph.dt);
-    v = add3(v, gs);
-    w = mk3(w.x * ph.Wdamping, w.y * ph.Wdamping, w.z * ph.Wdamping);
-    b.pos[bi] = f4(x, X.w);
-    b.vel[bi] = f4(v, V.w);
-    b.ang[bi] = f4(w, W.w);
-    b.qinc[bi] = make_float4(qi.x, qi.y, qi.z, qi.w);
-    b.qcum[bi] = make_float4(qc.x, qc.y, qc.z, qc.w);
-    if (b.model) {
-        float *m = b.model + (size_t)bi * 16;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            m[c * 4 + 0] = R[c * 3 + 0];
-            m[c * 4 + 1] = R[c * 3 + 1];
-            m[c * 4 + 2] = R[c * 3 + 2];
-            m[c * 4 + 3] = 0.0f;
-        }
-        m[12] = x.x; m[13] = x.y; m[14] = x.z; m[15] = 1.0f;
-    }
+    for (int k = 0; k < 9; ++k) o.Iw[(size_t)bi * 9 + k] = Iw[k];
+    const f3 x = bs.x;
+    o.pos[bi] = f4(x, X.w);
+    o.vel[bi] = f4(bs.v, V.w);
+    o.ang[bi] = f4(bs.w, W.w);
+    o.qinc[bi] = make_float4(bs.qi.x, bs.qi.y, bs.qi.z, bs.qi.w);
+    o.qcum[bi] = make_float4(bs.qc.x, bs.qc.y, bs.qc.z, bs.qc.w);
+    write_model(o, bi, R, x);
 }
 
 // computeGlobalAttributes (rigidBodyKernels.cu:386-421) for body particles, integrate_functor
@@ -360,6 +330,88 @@ __global__ __launch_bounds__(256) void k_reduce_apply(const float4 *__restrict__
     wave_reduce_apply(aF, aT, lane, body, b, ph);
 }
 
+// The reduction of a step whose bodies' integration the previous step stored ahead (in `a`): one wave
+// per body sums its particles' forces and torques as k_reduce_apply does (lane 0's butterfly value),
+// then one thread per body applies them to the integrated state, stores the state after the step in
+// `b` (what the integration kernel and k_reduce_apply would have left there) and the integration of
+// that state for the next step in `a`.
+constexpr int kAheadBodies = 4;   // bodies per workgroup of 256 threads
+__global__ __launch_bounds__(256) void k_reduce_apply_ahead(const float4 *__restrict__ pF,
+                                                            const float4 *__restrict__ rel, BodyBufs b, BodyBufs a,
+                                                            int B, DevPhys ph, ZeroJobs z, int bodyBlocks) {
+    __shared__ float sSum[kAheadBodies][6];
+    if ((int)blockIdx.x >= bodyBlocks) {
+        zero_jobs(z, (int)blockIdx.x - bodyBlocks, (int)gridDim.x - bodyBlocks);
+        return;
+    }
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int body = (int)blockIdx.x * kAheadBodies + wave;
+    f3 aF = mk3(0.f, 0.f, 0.f), aT = mk3(0.f, 0.f, 0.f);
+    if (body < B) {
+        const int st = b.pstart[body], cnt = b.pcount[body];
+        for (int j = reduce_first(body, ph) + lane; j < cnt; j += 64) {
+            const size_t o = (size_t)st + j;
+            const f3 F = xyz(pF[o]);
+            aF = add3(aF, F);
+            aT = add3(aT, cross3(xyz(rel[o]), F));
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        aF.x = aF.x + __shfl_xor(aF.x, off, 64);
+        aF.y = aF.y + __shfl_xor(aF.y, off, 64);
+        aF.z = aF.z + __shfl_xor(aF.z, off, 64);
+        aT.x = aT.x + __shfl_xor(aT.x, off, 64);
+        aT.y = aT.y + __shfl_xor(aT.y, off, 64);
+        aT.z = aT.z + __shfl_xor(aT.z, off, 64);
+    }
+    if (lane == 0) {
+        sSum[wave][0] = aF.x; sSum[wave][1] = aF.y; sSum[wave][2] = aF.z;
+        sSum[wave][3] = aT.x; sSum[wave][4] = aT.y; sSum[wave][5] = aT.z;
+    }
+    __syncthreads();
+    const int k = (int)threadIdx.x;
+    const int bd = (int)blockIdx.x * kAheadBodies + k;
+    if (k >= kAheadBodies || bd >= B) return;
+    const f3 sF = mk3(sSum[k][0], sSum[k][1], sSum[k][2]), sT = mk3(sSum[k][3], sSum[k][4], sSum[k][5]);
+    // the apply of wave_reduce_apply, on the integrated state
+    const float sc = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
+    const float m = b.mass[bd];
+    const f3 dv = mk3((sF.x * sc) / m, (sF.y * sc) / m, (sF.z * sc) / m);
+    float Iw[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Iw[q] = a.Iw[(size_t)bd * 9 + q];
+    const f3 dw = mat_vec(Iw, mk3(sT.x * sc, sT.y * sc, sT.z * sc));
+    float4 V = a.vel[bd], W = a.ang[bd];
+    V.x += dv.x; V.y += dv.y; V.z += dv.z;
+    W.x += dw.x; W.y += dw.y; W.z += dw.z;
+    // the state after the step
+    const float4 X = a.pos[bd], Qc = a.qcum[bd];
+    b.pos[bd] = X;
+    b.vel[bd] = V;
+    b.ang[bd] = W;
+    b.qinc[bd] = a.qinc[bd];
+    b.qcum[bd] = Qc;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) b.Iw[(size_t)bd * 9 + q] = Iw[q];
+    if (b.model) {
+        float R[9];
+        mat3_cast(qf{Qc.x, Qc.y, Qc.z, Qc.w}, R);
+        write_model(b, bd, R, xyz(X));
+    }
+    // the next step's integration of it
+    const BodyStep bs = body_step(X, V, W, Qc, ph);
+    float R2[9], Iw2[9];
+    body_world_inertia(bs.qc, b.Ib + (size_t)bd * 9, R2, Iw2);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) a.Iw[(size_t)bd * 9 + q] = Iw2[q];
+    a.pos[bd] = f4(bs.x, X.w);
+    a.vel[bd] = f4(bs.v, V.w);
+    a.ang[bd] = f4(bs.w, W.w);
+    a.qinc[bd] = make_float4(bs.qi.x, bs.qi.y, bs.qi.z, bs.qi.w);
+    a.qcum[bd] = make_float4(bs.qc.x, bs.qc.y, bs.qc.z, bs.qc.w);
+}
+
 __global__ void k_fill_u32(unsigned *p, unsigned v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -524,7 +576,7 @@ void launch_fill_u16(uint16_t *p, uint16_t value, size_t count, hipStream_t st) 
 }
 
 void launch_body_integrate(const BodyBufs &b, int numBodies, const DevPhys &ph, const ZeroJobs *zero,
-                           hipStream_t st) {
+                           hipStream_t st, const BodyBufs *out) {
     const int block = 128;
     int bodyBlocks = (int)blocks_for(numBodies, block);
     ZeroJobs z{};
@@ -536,8 +588,8 @@ void launch_body_integrate(const BodyBufs &b, int numBodies, const DevPhys &ph, 
     int zeroBlocks = zmax ? (int)blocks_for((long long)zmax, block) : 0;
     if (zeroBlocks > 512) zeroBlocks = 512;
     if (bodyBlocks + zeroBlocks == 0) return;
-    hipLaunchKernelGGL(k_body_integrate, dim3(bodyBlocks + zeroBlocks), dim3(block), 0, st, b, numBodies, ph, z,
-                       bodyBlocks);
+    hipLaunchKernelGGL(k_body_integrate, dim3(bodyBlocks + zeroBlocks), dim3(block), 0, st, b, out ? *out : b,
+                       numBodies, ph, z, bodyBlocks);
 }
 
 void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph, const SceneView &sv,
@@ -612,6 +664,23 @@ void launch_reduce_apply(const float4 *pForce, const float4 *pTorque, const floa
     const int block = 256;
     dim3 grid(blocks_for((long long)numBodies * 64, block));
     hipLaunchKernelGGL(k_reduce_apply, grid, dim3(block), 0, st, pForce, pTorque, relPos, b, numBodies, ph);
+}
+
+void launch_reduce_apply_ahead(const float4 *pForce, const float4 *relPos, const BodyBufs &b, const BodyBufs &ahead,
+                               int numBodies, const DevPhys &ph, const ZeroJobs *zero, hipStream_t st) {
+    const int block = 256;
+    const int bodyBlocks = numBodies > 0 ? (numBodies + kAheadBodies - 1) / kAheadBodies : 0;
+    ZeroJobs z{};
+    size_t zmax = 0;
+    if (zero) {
+        z = *zero;
+        for (int k = 0; k < 4; ++k) zmax = z.count[k] > zmax ? z.count[k] : zmax;
+    }
+    int zeroBlocks = zmax ? (int)blocks_for((long long)zmax, block) : 0;
+    if (zeroBlocks > 512) zeroBlocks = 512;
+    if (bodyBlocks + zeroBlocks == 0) return;
+    hipLaunchKernelGGL(k_reduce_apply_ahead, dim3(bodyBlocks + zeroBlocks), dim3(block), 0, st, pForce, relPos, b, ahead,
+                       numBodies, ph, z, bodyBlocks);
 }
 
 void launch_fill_u32(unsigned *p, unsigned value, size_t count, hipStream_t st) {

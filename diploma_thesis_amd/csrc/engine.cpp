@@ -121,6 +121,7 @@ int Engine::set_static(const float *pos4, const float *norm4, int n) {
     ARSIM_HIP(hipSetDevice(device));
     ARSIM_HIP(hipStreamSynchronize(st));
     invalidate_graph();
+    primed = false;   // the step form may change with the static set (integrate_ahead)
     free_static();
     if (n == 0) return ARSIM_OK;
     if ((flags & ARSIM_CONTACT_SIS) && sisCap < 2 * N) {
@@ -198,6 +199,24 @@ StateBufs Engine::state_bufs() const {
     return b;
 }
 
+BodyBufs Engine::ahead_bufs() const {
+    BodyBufs b = body_bufs();
+    b.pos = aPos; b.vel = aVel; b.ang = aAng; b.qinc = aQinc; b.qcum = aQcum; b.Iw = aIw;
+    b.model = nullptr;
+    return b;
+}
+
+int Engine::prime() {
+    ZeroJobs z{};
+    z.p[0] = bucketHist;
+    z.count[0] = (size_t)numBuckets;
+    const BodyBufs a = ahead_bufs();
+    launch_body_integrate(body_bufs(), B, ph, &z, st, &a);
+    ARSIM_HIP(hipGetLastError());
+    primed = true;
+    return ARSIM_OK;
+}
+
 BodyBufs Engine::body_bufs() const {
     BodyBufs b{};
     b.pos = rbPos; b.vel = rbVel; b.ang = rbAng; b.qinc = qinc; b.qcum = qcum;
@@ -213,7 +232,17 @@ PartBufs Engine::part_bufs() const {
     return p;
 }
 
-const char *const *Engine::stage_names(int method, unsigned flags, bool bucket, bool fusedDem, int *count) {
+const char *const *Engine::stage_names(int method, unsigned flags, bool bucket, bool fusedDem, bool ahead,
+                                       int *count) {
+    const char *const *names = stage_names_all(method, flags, bucket, fusedDem, count);
+    if (ahead) {   // every list starts with the body-integration stage
+        --*count;
+        return names + 1;
+    }
+    return names;
+}
+
+const char *const *Engine::stage_names_all(int method, unsigned flags, bool bucket, bool fusedDem, int *count) {
     // with the bucket sort the pose stage also ranks each tile by bucket (the sort's first pass)
     // and the bucket stage finishes the sort with the reorder
     static const char *const sis[] = {"body_integrate", "pose_key", "sort", "reorder_cellstart", "sis_find",
@@ -356,6 +385,7 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
     S = (int)src.size();
     bucketSort = false;
     mortonBucket = false;
+    primed = false;
     struct Desc { int n, b; ArSimParams p; ArSimStepConfig c; };
     std::vector<Desc> d(S);
     for (int s = 0; s < S; ++s) {
@@ -533,6 +563,9 @@ int Engine::build(const std::vector<SceneSource> &src, const ArSimTuning *tuning
         A(tileInfo, std::max<size_t>(bkPlan.infoWords, 1)); A(bkScenesD, (size_t)S);
         A(tileSceneD, std::max(numTiles, 1)); A(bucketSceneD, std::max(numBuckets, 1));
         A(bucketHist, std::max(numBuckets, 1));
+        if (!(flags & ARSIM_CONTACT_SIS)) {
+            A(aPos, nb); A(aVel, nb); A(aAng, nb); A(aQinc, nb); A(aQcum, nb); A(aIw, nb * 9);
+        }
     }
     if (flags & ARSIM_CONTACT_SIS) {
         A(sisDistA, N); A(sisPartA, N); A(sisDistB, N); A(sisBodyB, N); A(sisPartB, N); A(sceneBodyD, (size_t)S);
@@ -705,19 +738,21 @@ int Engine::reset_grid_tables() {
 // hold the sorted keys and particle indices.
 template <class Rec>
 int Engine::enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, const SceneView &sv, const SortTuning &tuIn,
-                                    bool grainVelDead, unsigned **kAOut, unsigned **vAOut, Rec &rec) {
+                                    bool grainVelDead, bool ahead, unsigned **kAOut, unsigned **vAOut, Rec &rec) {
     if (bucketSort) {
         // the pose kernel ranks each tile of particles by bucket (the sort's first pass); the bucket
         // kernel gathers each bucket's runs, finishes the sort there, and writes the sorted copies and
-        // the cell table
-        ZeroJobs z{};
-        z.p[0] = bucketHist;
-        z.count[0] = (size_t)numBuckets;
-        launch_body_integrate(bb, B, ph, &z, st);
-        rec();
+        // the cell table.  ahead: the bodies' integration is already in the ahead buffers
+        if (!ahead) {
+            ZeroJobs z{};
+            z.p[0] = bucketHist;
+            z.count[0] = (size_t)numBuckets;
+            launch_body_integrate(bb, B, ph, &z, st);
+            rec();
+        }
         const TileKeyOut to{(unsigned *)keysB, valsB, tileInfo, bucketHist, bkScenesD, tileSceneD, bucketSceneD,
                             numTiles, numBuckets, tileKeys, bkPlan.bucketBits};
-        launch_pose_key_tiles(pb, bb, N, ph, sv, to, grainVelDead, st);
+        launch_pose_key_tiles(pb, ahead ? ahead_bufs() : bb, N, ph, sv, to, grainVelDead, st);
         rec();
         const int split = tu.bucketSplit == 1 || tu.bucketSplit == 2 ? tu.bucketSplit : (numBuckets < 512 ? 2 : 1);
         launch_bucket_sort(to, (unsigned *)keysA, valsA, gatherRec, sortedPos, sortedVel, segStart, (unsigned)numCells,
@@ -764,11 +799,15 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         ++k;
     };
     rec();
+    const bool ahead = integrate_ahead();
+    ZeroJobs zHist{};   // with ahead: the bucket counts, zeroed by the reduction for the next step
+    zHist.p[0] = bucketHist;
+    zHist.count[0] = (size_t)numBuckets;
     if (flags & ARSIM_CONTACT_SIS) {
         // updateUniformGridSIS (RigidBodies/UniformGridUpdate.cpp:734-797): integrate and pose, the grid
         // broadphase, deepest contacts (static set first), the contact list, the sweeps, the pose again
         unsigned *kA = nullptr, *vA = nullptr;
-        if (int rc = enqueue_grid_broadphase(bb, pb, sv, stt, false, &kA, &vA, rec)) return rc;
+        if (int rc = enqueue_grid_broadphase(bb, pb, sv, stt, false, false, &kA, &vA, rec)) return rc;
         if (numStatic)
             launch_sis_find_static(sortedPos, vA, N, stPos, nullptr, stTable, nullptr, geoms[0], ph.r, false, sisDistA,
                                    sisPartA, nullptr, st);
@@ -796,7 +835,7 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         // keys are generated, so the sort itself is only its digit passes
         unsigned *kA = nullptr, *vA = nullptr;
         // the DEM below writes every grain's velocity, so the pose kernel's copy is not stored
-        if (int rc = enqueue_grid_broadphase(bb, pb, sv, stt, true, &kA, &vA, rec)) return rc;
+        if (int rc = enqueue_grid_broadphase(bb, pb, sv, stt, true, ahead, &kA, &vA, rec)) return rc;
         if (fused_dem()) {
             const FusedDemIn in{gatherRec, relPos, grainIdx, numGrains, vel};
             launch_dem_fused(sortedPos, sortedVel, vA, segStart, in, bb, B, ph, sv, S > 1, st);
@@ -810,7 +849,8 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
                 launch_dem_static(sortedPos, sortedVel, vA, N, stPos, stNorm, nullptr, stTable, nullptr, ph, geoms[0],
                                   pForce, nullptr, relPos, false, st);
             rec();
-            launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
+            if (ahead) launch_reduce_apply_ahead(pForce, relPos, bb, ahead_bufs(), B, ph, &zHist, st);
+            else launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
             rec();
         }
         prevSorted = kA;
@@ -818,17 +858,16 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
         lastIdx = vA;
     } else if (mortonBucket) {
         // Morton codes ranked per tile by their top 10 bits in the pose kernel, then sorted per bucket
-        ZeroJobs z{};
-        z.p[0] = bucketHist;
-        z.count[0] = (size_t)numBuckets;
-        launch_body_integrate(bb, B, ph, &z, st);
-        rec();
+        if (!ahead) {
+            launch_body_integrate(bb, B, ph, &zHist, st);
+            rec();
+        }
         TileKeyOut to{(unsigned *)keysB, valsB, tileInfo, bucketHist, bkScenesD, tileSceneD, bucketSceneD,
                       numTiles, numBuckets, tileKeys, bkPlan.bucketBits};
         const bool m63 = method == ARSIM_METHOD_LBVH63;
         to.morton = m63 ? 2 : 1;
         to.code63 = code63;
-        launch_pose_key_tiles(pb, bb, N, ph, sv, to, false, st);
+        launch_pose_key_tiles(pb, ahead ? ahead_bufs() : bb, N, ph, sv, to, false, st);
         rec();
         // (the tree's leaf arrays are the second buffer of a bucket sorted in global memory; the refit
         // writes them afterwards)
@@ -853,7 +892,8 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
             launch_dem_static(tree.cms, tree.sortedVel, vS, N, stPos, stNorm, nullptr, stTable, nullptr, ph,
                               geoms[0], pForce, nullptr, relPos, false, st);
         rec();
-        launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
+        if (ahead) launch_reduce_apply_ahead(pForce, relPos, bb, ahead_bufs(), B, ph, &zHist, st);
+        else launch_reduce_apply(pForce, nullptr, relPos, bb, B, ph, st);
         rec();
         lastKeys = kS;
         lastIdx = vS;
@@ -913,6 +953,8 @@ int Engine::enqueue_step(hipEvent_t *ev, int *numStages) {
 }
 
 int Engine::step(int n) {
+    if (n > 0 && integrate_ahead() && !primed)
+        if (int rc = prime()) return rc;
     for (int i = 0; i < n; ++i) {
         if (tu.useGraph && !first) {
             if (!graphReady) {
@@ -1061,6 +1103,7 @@ int arsim_engine_set_dt(arsim_engine *e, float dt) {
         e->e.ph.dt = dt;
         e->e.c0.dt = dt;
         e->e.invalidate_graph();   // dt is a kernel argument of the captured launches
+        e->e.primed = false;       // the integration held ahead used the old dt
     }
     return ARSIM_OK;
 }
@@ -1212,9 +1255,12 @@ int arsim_engine_profile_step(arsim_engine *e, float *ms, const char **names, in
     Engine &E = e->e;
     int count = 0;
     const char *const *nm =
-        Engine::stage_names(E.method, E.flags, E.bucketSort || E.mortonBucket, E.fused_dem(), &count);
+        Engine::stage_names(E.method, E.flags, E.bucketSort || E.mortonBucket, E.fused_dem(), E.integrate_ahead(),
+                            &count);
     std::vector<hipEvent_t> ev(count + 1);
     for (auto &x : ev) ARSIM_HIP(hipEventCreate(&x));
+    if (E.integrate_ahead() && !E.primed)
+        if (int r = E.prime()) return r;
     int ns = 0;
     int rc = E.enqueue_step(ev.data(), &ns);
     if (rc == 0) {
@@ -1367,6 +1413,7 @@ int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best,
     auto trial = [&](const ArSimTuning &t) -> int {
         for (auto &s : snaps)
             if (s.dst) hipMemcpyAsync((void *)s.src, s.dst, s.bytes, hipMemcpyDeviceToDevice, E.st);
+        E.primed = false;   // the state was restored: the integration ahead is recomputed from it
         E.tu = t;
         E.reset_grid_tables();
         if (int r = E.step(2)) return r;
@@ -1399,6 +1446,7 @@ int arsim_engine_autotune(arsim_engine *e, int stepsPerTrial, ArSimTuning *best,
     hipEventDestroy(b);
     for (auto &s : snaps)
         if (s.dst) hipMemcpyAsync((void *)s.src, s.dst, s.bytes, hipMemcpyDeviceToDevice, E.st);
+    E.primed = false;
     hipStreamSynchronize(E.st);
     for (auto &s : snaps)
         if (s.dst) hipFree(s.dst);

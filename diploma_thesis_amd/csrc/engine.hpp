@@ -78,6 +78,10 @@ struct Engine {
     size_t sortWsBytes = 0;
     Lbvh tree{};
     float4 *treeBounds = nullptr;   // node boxes for debug views (the step keeps tree.bounds NULL)
+    // the bodies' integration one step ahead (integrate_ahead): position, velocities, quaternions,
+    // inverse world inertia
+    float4 *aPos = nullptr, *aVel = nullptr, *aAng = nullptr, *aQinc = nullptr, *aQcum = nullptr;
+    float *aIw = nullptr;
     uint64_t *code63 = nullptr;     // LBVH63 bucket path: the 63-bit codes in particle order
     int *overflowD = nullptr, *flagD = nullptr;
     float4 *aabbPart = nullptr, *aabbBox = nullptr;
@@ -114,7 +118,20 @@ struct Engine {
     int set_static(const float *pos4, const float *norm4, int n);
     void free_static();
     int reset_grid_tables();
-    static const char *const *stage_names(int method, unsigned flags, bool bucket, bool fusedDem, int *count);
+    static const char *const *stage_names_all(int method, unsigned flags, bool bucket, bool fusedDem, int *count);
+    // ahead: the step has no body-integration stage of its own (integrate_ahead)
+    static const char *const *stage_names(int method, unsigned flags, bool bucket, bool fusedDem, bool ahead,
+                                          int *count);
+    // Bucket-sorted steps with the plain DEM keep the bodies' integration one step ahead: the previous
+    // step's reduction stored it in the ahead buffers, the pose reads it there, and the reduction
+    // stores the state after the step (in the state buffers) and the next integration (ahead).  The
+    // step then has no integration launch.  The ahead buffers are filled from the state (prime) before
+    // a step whenever the state or dt may have changed outside the steps (build, a trial's restore, a
+    // new dt or static set).
+    bool integrate_ahead() const { return (bucketSort || mortonBucket) && !(flags & ARSIM_CONTACT_SIS) && !fused_dem(); }
+    int prime();
+    BodyBufs ahead_bufs() const;
+    bool primed = false;
     // the grid DEM and the per-body reduction run as one kernel (on request; exact cell table, no static set)
     bool fused_dem() const { return tu.fuseReduce && method == ARSIM_METHOD_GRID && !(flags & ARSIM_CONTACT_SIS) &&
                                     segBits == 0 && numStatic == 0; }
@@ -133,7 +150,7 @@ struct Engine {
     int alloc_sis(int cap);
     template <class Rec>
     int enqueue_grid_broadphase(const BodyBufs &bb, const PartBufs &pb, const SceneView &sv, const SortTuning &stt,
-                                bool grainVelDead, unsigned **kAOut, unsigned **vAOut, Rec &rec);
+                                bool grainVelDead, bool ahead, unsigned **kAOut, unsigned **vAOut, Rec &rec);
 };
 
 }  // namespace arsim

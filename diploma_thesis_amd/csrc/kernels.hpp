@@ -81,14 +81,21 @@ struct ZeroJobs {
 };
 
 // ---------------------------------------------------------------- particle / body stages (body.hip)
+// out: where the integrated state goes (NULL: in place, into b)
 void launch_body_integrate(const BodyBufs &b, int numBodies, const DevPhys &ph, const ZeroJobs *zero,
-                           hipStream_t st);
+                           hipStream_t st, const BodyBufs *out = nullptr);
 // doBodies / doGrains select which particles are updated; mode selects the key written.
 void launch_pose_key(const PartBufs &p, const BodyBufs &b, int numParticles, const DevPhys &ph,
                      const SceneView &sv, int mode, const KeyOut &ko, bool doBodies, bool doGrains,
                      hipStream_t st);
 void launch_reduce_apply(const float4 *pForce, const float4 *pTorque, const float4 *relPos, const BodyBufs &b,
                          int numBodies, const DevPhys &ph, hipStream_t st);
+// The reduction of a step whose bodies were integrated ahead, into `ahead` (the step's pose read them
+// there): the impulses are applied to the integrated state, the state after the step is stored in
+// `b`, and the next step's integration of it in `ahead` (the integration kernel's arithmetic).  zero:
+// ranges zeroed by extra blocks, for the next step.
+void launch_reduce_apply_ahead(const float4 *pForce, const float4 *relPos, const BodyBufs &b, const BodyBufs &ahead,
+                               int numBodies, const DevPhys &ph, const ZeroJobs *zero, hipStream_t st);
 void launch_fill_u32(unsigned *p, unsigned value, size_t count, hipStream_t st);
 void launch_aabb(const float4 *pos, int n, float4 *outMin, float4 *outMax, float4 *partials, int maxPartials,
                  hipStream_t st);

@@ -9,6 +9,20 @@ namespace arsim {
 // The intended sum of UniformGridDEM.cu:421-460: every lane holds the partial sums (aF, aT) of the
 // body's particles l, l+64, ... in index order; a butterfly over lane distance 32..1 completes them,
 // and lane 0 applies v += J dt / m and ω += I_w^-1 τ dt (J and τ without dt in the reference units).
+// The body's render model matrix (column-major 4x4: R and the centre), when kept.
+__device__ __forceinline__ void write_model(const BodyBufs &b, int body, const float R[9], f3 x) {
+    if (!b.model) return;
+    float *m = b.model + (size_t)body * 16;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        m[c * 4 + 0] = R[c * 3 + 0];
+        m[c * 4 + 1] = R[c * 3 + 1];
+        m[c * 4 + 2] = R[c * 3 + 2];
+        m[c * 4 + 3] = 0.0f;
+    }
+    m[12] = x.x; m[13] = x.y; m[14] = x.z; m[15] = 1.0f;
+}
+
 __device__ __forceinline__ void wave_reduce_apply(f3 aF, f3 aT, int lane, int body, const BodyBufs &b,
                                                   const DevPhys &ph) {
 #pragma unroll

@@ -132,7 +132,10 @@ def test_profile_and_autotune_leave_state_consistent():
     with Engine(sc) as e:
         e.step(2)
         stages = e.profile()
-        assert [n for n, _ in stages][0] == "body_integrate" and all(t >= 0 for _, t in stages)
+        names = [n for n, _ in stages]
+        # a bucket-sorted grid step: the bodies are integrated one step ahead, in the reduction
+        assert names[0] == "pose_key" and names[-1] == "reduce_apply" and "body_integrate" not in names, names
+        assert all(t >= 0 for _, t in stages)
         before = e.download()
         best, ms = e.autotune(steps_per_trial=2)
         after = e.download()
