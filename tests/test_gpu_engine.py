@@ -221,3 +221,26 @@ def test_fused_contact_reduce_kernel_bit_exact(batched):
         st = oracle.State(sc, np.float32)
         st.step(3)
         assert_state_equal(got[k], st, f"fused kernel, scene {k}")
+
+
+@pytest.mark.parametrize("method", [capi.METHOD_GRID, capi.METHOD_LBVH30])
+def test_update_with_changing_dt(method):
+    """ParticleSystem::update(dt) (particleSystem.h:288) with dt changing between steps.  The bucket-sorted
+    steps hold the bodies' integration one step ahead, so a new dt has to reach the next step's
+    integration; four graph-replayed steps (dt, dt, 2.5 dt, dt) equal the oracle's with the same dts."""
+    print(f"engine: update(dt) with changing dt, method {method}", flush=True)
+    sc = scenes.mix_scene(70000 if method == capi.METHOD_LBVH30 else 30000, 41, method)
+    dts = [1e-4, 1e-4, 2.5e-4, 1e-4]
+    with Engine(sc, use_graph=True) as e:
+        for dt in dts:
+            e.update(dt)
+        e.sync()
+        got = e.download()
+        stages = [name for name, _ in e.profile()]
+    assert "body_integrate" not in stages, stages   # the integration-ahead form of the step
+    st = oracle.State(sc, np.float32)
+    st.cfg = type(sc.cfg).from_buffer_copy(sc.cfg)
+    for dt in dts:
+        st.cfg.dt = dt
+        st.step(1)
+    assert_state_equal(got, st, f"update with changing dt, method {method}")
