@@ -243,9 +243,11 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
     };
     const unsigned base = (unsigned)c0;
     if (inLds) {
-        // every slot found first (LDS only), then every key load in flight at once
+        // every slot found first (LDS only), then every key and value load in flight at once; the
+        // values wait in registers while the keys are sorted, then go to the buffer the sort left free,
+        // so that the gathers below find a sorted key's value in LDS rather than in global memory
         constexpr int kPer = (kCap + kBkThreads - 1) / kBkThreads;
-        unsigned sl[kPer], kv[kPer];
+        unsigned sl[kPer], kv[kPer], vv[kPer];
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int i = t + q * kBkThreads;
@@ -255,6 +257,7 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
         for (int q = 0; q < kPer; ++q) {
             const int i = t + q * kBkThreads;
             kv[q] = i < cnt ? tKeys[sl[q]] : 0u;
+            vv[q] = i < cnt ? tVals[sl[q]] : 0u;
         }
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
@@ -270,13 +273,20 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
             dst = x;
         }
         const unsigned *sorted = src;   // packed words in sorted order
+        unsigned *valAt = dst;          // valAt[i]: the value of the bucket's i-th key in gather order
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = t + q * kBkThreads;
+            if (i < cnt) valAt[i] = vv[q];
+        }
+        __syncthreads();
         // the gathers of the first two rounds of positions are issued before the table is filled, so
         // that their latency overlaps the table's stores; a lane past the part's end loads sorted
         // position 0 (the bucket is not empty) and stores nothing, so the loads need no branch
         const int ia = i0 + t, ib = ia + kBkThreads;
         const bool ha = ia < i1, hb = ib < i1;
         const unsigned wa = sorted[ha ? ia : 0], wb = sorted[hb ? ib : 0];
-        const unsigned va = tVals[slot(wa & kPosMask)], vb = tVals[slot(wb & kPosMask)];
+        const unsigned va = valAt[wa & kPosMask], vb = valAt[wb & kPosMask];
         float4 pa, qa, pb, qb;
         if (REORDER) {
             pa = rec[2 * (size_t)va];
@@ -303,7 +313,7 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
         }
         for (int i = ib + kBkThreads; i < i1; i += kBkThreads) {
             const unsigned w = sorted[i];
-            const unsigned val = tVals[slot(w & kPosMask)];
+            const unsigned val = valAt[w & kPosMask];
             kOut[start + i] = base + (w >> POSBITS);
             vOut[start + i] = val;
             if (REORDER) {
