@@ -49,11 +49,14 @@ __global__ void k_body_integrate(BodyBufs b, BodyBufs o, int B, DevPhys ph, Zero
 // calcHashD (particleKernels.cu:170-189) or generateMortonCodes (BVHkernels.cu:670-716).
 // Where pose_key_one takes the scene geometry from: each particle's scene (per lane), the table entry
 // of scene `scene` for all of the caller's particles (scalar loads), or the one scene passed by value.
+// A caller that has loaded the particle's rbIndex (rbKnown) or, with kPoseGeomScene, the scene's
+// geometry (gKnown) already passes them; null pointers have them loaded here.
 enum { kPoseGeomLane = 0, kPoseGeomScene = 1, kPoseGeomSingle = 2 };
 template <int MODE, int GSRC = kPoseGeomLane>
 __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
                                                  KeyOut ko, int doBodies, int doGrains, unsigned *shist,
-                                                 int scene = 0);
+                                                 int scene = 0, const int *rbKnown = nullptr,
+                                                 const SceneGeom *gKnown = nullptr);
 
 // Adds 1 to an LDS histogram bin for every active lane.  Spatially ordered keys share digits across
 // a wave, so lanes are grouped by digit (8 ballots find the lanes with the same 8-bit digit) and
@@ -97,27 +100,31 @@ __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N,
 // from the scene table, which holds every scene of an engine, also a single one).
 template <int MODE, int GSRC>
 __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
-                                                 KeyOut ko, int doBodies, int doGrains, unsigned *shist, int scene) {
+                                                 KeyOut ko, int doBodies, int doGrains, unsigned *shist, int scene,
+                                                 const int *rbKnown, const SceneGeom *gKnown) {
     if (i >= N) return 0;
-    const int rbi = p.rb ? p.rb[i] : -1;
+    const int rbi = rbKnown ? *rbKnown : (p.rb ? p.rb[i] : -1);
     const int s = GSRC == kPoseGeomScene ? scene : GSRC == kPoseGeomSingle ? 0 : (p.scene ? (int)p.scene[i] : 0);
     SceneGeom g;
-    if (GSRC == kPoseGeomScene) g = sv.table[scene];
+    if (GSRC == kPoseGeomScene) g = gKnown ? *gKnown : sv.table[scene];
     else if (GSRC == kPoseGeomSingle) g = sv.single;
     else load_geom(sv, s, g);
     float4 P, Vfin;
     bool haveV = false;
     if (rbi >= 0) {
         if (doBodies) {
-            float4 Qc = b.qcum[rbi];
+            // every load before the first store: the compiler may not move a load above a store to
+            // a buffer it cannot tell apart, so loads after a store would each cost a round trip
+            const float4 Qc = b.qcum[rbi];
+            const float4 frame = p.bodyFrame[i];
+            const float4 CM = b.pos[rbi];
+            const float4 V = b.vel[rbi], W = b.ang[rbi];
             float R[9];
             mat3_cast(qf{Qc.x, Qc.y, Qc.z, Qc.w}, R);
-            f3 r = mat_vec(R, xyz(p.bodyFrame[i]));
+            f3 r = mat_vec(R, xyz(frame));
             p.relPos[i] = f4(r, 0.0f);
-            float4 CM = b.pos[rbi];
             P = f4(add3(xyz(CM), r), CM.w);
             p.pos[i] = P;
-            float4 V = b.vel[rbi], W = b.ang[rbi];
             Vfin = f4(add3(xyz(V), cross3(xyz(W), r)), V.w);
             haveV = true;
             p.vel[i] = Vfin;
@@ -240,12 +247,26 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
     KeyOut ko{};
     ko.grainVelDead = grainVelDead != 0;
     if (MODE == kKeyMorton63) ko.keys = to.code63;   // the full codes, in particle order
+    // the scene's geometry and every item's rbIndex requested first: the items' poses depend on them,
+    // and the stores of one item's pose keep the compiler from moving the next item's loads above them
+    const SceneGeom geom = sv.table[scene];
+    int rbv[kTileItems];
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k) {
+        const long long i = waveBase + k * 64 + lane;
+        rbv[k] = i < tileEnd && p.rb ? p.rb[i] : -1;
+    }
+    // wait for them here, on every path: otherwise the wait for a later item's rbIndex, placed after
+    // the previous item's stores, waits for those stores as well (vector memory counter 0, the rest
+    // of the counters left at their maxima in the gfx9 encoding)
+    __builtin_amdgcn_s_waitcnt(0x0f70);
     unsigned keys[kTileItems];
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
         const long long i = waveBase + k * 64 + lane;
         const uint64_t key =
-            i < tileEnd ? pose_key_one<MODE, kPoseGeomScene>((int)i, p, b, N, ph, sv, ko, 1, 1, nullptr, scene) : 0ull;
+            i < tileEnd ? pose_key_one<MODE, kPoseGeomScene>((int)i, p, b, N, ph, sv, ko, 1, 1, nullptr, scene, &rbv[k], &geom)
+                        : 0ull;
         // a 63-bit code's top 30 bits are the 30-bit code of the same position (both scale the
         // normalised coordinate by a power of two and truncate)
         keys[k] = MODE == kKeyMorton63 ? (unsigned)(key >> 33) : (unsigned)key;
