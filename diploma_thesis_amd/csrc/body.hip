@@ -340,6 +340,7 @@ __global__ __launch_bounds__(256) void k_reduce_apply(const float4 *__restrict__
     const int lane = threadIdx.x & 63;
     if (body >= B) return;
     const int st = b.pstart[body], cnt = b.pcount[body];
+    asm volatile("" : : "v"(st), "v"(cnt));   // both loads issued together (k_reduce_apply_ahead)
     f3 aF = mk3(0.f, 0.f, 0.f), aT = mk3(0.f, 0.f, 0.f);
     for (int j = reduce_first(body, ph) + lane; j < cnt; j += 64) {
         size_t o = (size_t)st + j;
@@ -370,6 +371,9 @@ __global__ __launch_bounds__(256) void k_reduce_apply_ahead(const float4 *__rest
     f3 aF = mk3(0.f, 0.f, 0.f), aT = mk3(0.f, 0.f, 0.f);
     if (body < B) {
         const int st = b.pstart[body], cnt = b.pcount[body];
+        // both values needed here, so that their loads are issued together (the compiler would
+        // otherwise load the start only once the count has let the loop begin: two round trips)
+        asm volatile("" : : "v"(st), "v"(cnt));
         for (int j = reduce_first(body, ph) + lane; j < cnt; j += 64) {
             const size_t o = (size_t)st + j;
             const f3 F = xyz(pF[o]);
@@ -395,23 +399,27 @@ __global__ __launch_bounds__(256) void k_reduce_apply_ahead(const float4 *__rest
     const int bd = (int)blockIdx.x * kAheadBodies + k;
     if (k >= kAheadBodies || bd >= B) return;
     const f3 sF = mk3(sSum[k][0], sSum[k][1], sSum[k][2]), sT = mk3(sSum[k][3], sSum[k][4], sSum[k][5]);
-    // the apply of wave_reduce_apply, on the integrated state
-    const float sc = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
+    // every load before the first store: the compiler may not move a load above a store to a buffer
+    // it cannot tell apart, so a load after a store would cost a round trip of its own
     const float m = b.mass[bd];
-    const f3 dv = mk3((sF.x * sc) / m, (sF.y * sc) / m, (sF.z * sc) / m);
-    float Iw[9];
+    float Iw[9], Ib[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) Iw[q] = a.Iw[(size_t)bd * 9 + q];
-    const f3 dw = mat_vec(Iw, mk3(sT.x * sc, sT.y * sc, sT.z * sc));
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Ib[q] = b.Ib[(size_t)bd * 9 + q];
     float4 V = a.vel[bd], W = a.ang[bd];
+    const float4 X = a.pos[bd], Qc = a.qcum[bd], Qi = a.qinc[bd];
+    // the apply of wave_reduce_apply, on the integrated state
+    const float sc = (ph.flags & kRefUnits) ? 1.0f : ph.dt;
+    const f3 dv = mk3((sF.x * sc) / m, (sF.y * sc) / m, (sF.z * sc) / m);
+    const f3 dw = mat_vec(Iw, mk3(sT.x * sc, sT.y * sc, sT.z * sc));
     V.x += dv.x; V.y += dv.y; V.z += dv.z;
     W.x += dw.x; W.y += dw.y; W.z += dw.z;
     // the state after the step
-    const float4 X = a.pos[bd], Qc = a.qcum[bd];
     b.pos[bd] = X;
     b.vel[bd] = V;
     b.ang[bd] = W;
-    b.qinc[bd] = a.qinc[bd];
+    b.qinc[bd] = Qi;
     b.qcum[bd] = Qc;
 #pragma unroll
     for (int q = 0; q < 9; ++q) b.Iw[(size_t)bd * 9 + q] = Iw[q];
@@ -423,7 +431,7 @@ __global__ __launch_bounds__(256) void k_reduce_apply_ahead(const float4 *__rest
     // the next step's integration of it
     const BodyStep bs = body_step(X, V, W, Qc, ph);
     float R2[9], Iw2[9];
-    body_world_inertia(bs.qc, b.Ib + (size_t)bd * 9, R2, Iw2);
+    body_world_inertia(bs.qc, Ib, R2, Iw2);
 #pragma unroll
     for (int q = 0; q < 9; ++q) a.Iw[(size_t)bd * 9 + q] = Iw2[q];
     a.pos[bd] = f4(bs.x, X.w);
