@@ -82,7 +82,10 @@ __global__ __launch_bounds__(kKarrasBlock) void k_karras(const K *__restrict__ c
             return arsim::lcp(codes, win, w0, n, ki_, i_, j_, base_);
         };
         const K ki = win[i - w0];
-        const int base = sceneShift > 0 ? sceneStart[(int)(ki >> sceneShift)] : 0;
+        const int sci = sceneShift > 0 ? (int)(ki >> sceneShift) : 0;
+        const int base = sceneShift > 0 ? sceneStart[sci] : 0;
+        // loaded here rather than after the node's stores, where it would cost a round trip of its own
+        const int sceneCnt = t.sceneRoot && sceneShift > 0 ? sceneCount[sci] : 0;
         const int dPrev = lcp(ki, i, i - 1, base);
         const int dNext = lcp(ki, i, i + 1, base);
         const int d = (dNext - dPrev) > 0 ? 1 : -1;
@@ -114,10 +117,9 @@ __global__ __launch_bounds__(kKarrasBlock) void k_karras(const K *__restrict__ c
         if (t.minR) { t.minR[node] = lo; t.maxR[node] = hi; }
         if (t.meta) t.meta[i] = make_int4(L, R, lo, hi);
         if (t.isLeaf) t.isLeaf[node] = 0;
-        if (t.sceneRoot && sceneShift > 0) {
-            int sc = (int)(codes[lo] >> sceneShift);
-            if (lo == sceneStart[sc] && hi == sceneStart[sc] + sceneCount[sc] - 1) t.sceneRoot[sc] = node;
-        }
+        // the root of leaf i's scene: a range that starts in another scene starts before this one's
+        // first leaf and spans two scenes, so testing against leaf i's scene alone decides
+        if (t.sceneRoot && sceneShift > 0 && lo == base && hi == base + sceneCnt - 1) t.sceneRoot[sci] = node;
         crosses = range_crosses(lo, hi);
     }
     if (t.crossList) karras_list_crossing(t, crosses, i, crossings);   // every thread: it holds a barrier
@@ -179,25 +181,27 @@ __device__ __forceinline__ void write_packed(const Lbvh &t, int n, int node, int
 }
 
 // Leaf initialisation shared by both refit forms: centre (w = rbIndex bits), velocity copy, AABB.
-__device__ __forceinline__ void leaf_init(int i, const int *__restrict__ sIdx, const float4 *__restrict__ pos,
+// o = sIdx[i], loaded by the caller.  Every load precedes the first store: the compiler may not move
+// a load above a store to a buffer it cannot tell apart, so a load after a store would cost a round
+// trip of its own.
+__device__ __forceinline__ void leaf_init(int i, int o, const float4 *__restrict__ pos,
                                           const float4 *__restrict__ vel, const int *__restrict__ rb,
                                           const uint16_t *__restrict__ scene, const float4 *__restrict__ rec, float r,
                                           const Lbvh &t, float4 &lmin, float4 &lmax, float4 &centre) {
-    const int o = sIdx[i];
-    float4 p;
+    float4 p, v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (rec) {   // the key kernel's gather record: (pos, rb bits), (vel, scene bits)
         p = rec[2 * (size_t)o];
-        t.cms[i] = p;
-        if (t.sortedVel) t.sortedVel[i] = rec[2 * (size_t)o + 1];
+        if (t.sortedVel) v = rec[2 * (size_t)o + 1];
     } else {
         p = pos[o];
-        t.cms[i] = make_float4(p.x, p.y, p.z, rb ? __int_as_float(rb[o]) : p.w);
+        if (rb) p.w = __int_as_float(rb[o]);
         if (t.sortedVel) {
-            float4 v = vel[o];
+            v = vel[o];
             if (scene) v.w = __int_as_float((int)scene[o]);
-            t.sortedVel[i] = v;
         }
     }
+    t.cms[i] = p;
+    if (t.sortedVel) t.sortedVel[i] = v;
     centre = p;
     lmin = make_float4(p.x - r, p.y - r, p.z - r, 0.0f);
     lmax = make_float4(p.x + r, p.y + r, p.z + r, 0.0f);
@@ -213,7 +217,7 @@ __global__ __launch_bounds__(256) void k_leaf_refit(const int *__restrict__ sIdx
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
     float4 lmin, lmax, centre;
-    leaf_init(i, sIdx, pos, vel, rb, scene, rec, r, t, lmin, lmax, centre);
+    leaf_init(i, sIdx[i], pos, vel, rb, scene, rec, r, t, lmin, lmax, centre);
     store_bounds(t.bounds, (size_t)i, lmin, lmax);
     if (n < 2) return;
     int node = t.parent[i];
@@ -281,19 +285,25 @@ __global__ __launch_bounds__(kBlk) void k_refit_local(const int *__restrict__ sI
     // The topology of internal nodes [b0, b0 + kBlk), staged so that the walk up reads LDS only: an
     // internal node's index is one end of its leaf range, so every node whose range lies inside this
     // workgroup's leaves is one of them.
+    // (the topology's loads and the leaf's sorted index are issued together, the leaf's own loads
+    // next, and the staging waits for them after that)
+    int4 m = make_int4(0, 0, 0, 0);
+    int upTop = -1;
     if (i < n - 1) {
-        const int4 m = t.meta[i];
-        sTop[tid] = make_int4(m.x, m.y, t.parent[i + n], m.z == i ? m.w : m.z);
+        m = t.meta[i];
+        upTop = t.parent[i + n];
     }
     int node = valid && n >= 2 ? t.parent[i] : -1;
+    const int o = valid ? sIdx[i] : 0;
     const float inf = __int_as_float(0x7f800000);
     float4 lmin = make_float4(inf, inf, inf, 0.0f), lmax = make_float4(-inf, -inf, -inf, 0.0f);
     if (valid) {
         float4 c;
-        leaf_init(i, sIdx, pos, vel, rb, scene, rec, r, t, lmin, lmax, c);
+        leaf_init(i, o, pos, vel, rb, scene, rec, r, t, lmin, lmax, c);
         if (t.bounds) store_plain(t.bounds, i, lmin, lmax);
         sCtr[tid] = c;
     }
+    if (i < n - 1) sTop[tid] = make_int4(m.x, m.y, upTop, m.z == i ? m.w : m.z);
     // inclusive prefix and suffix min / max within the wave
     float4 pmn = lmin, pmx = lmax, smn = lmin, smx = lmax;
 #pragma unroll
