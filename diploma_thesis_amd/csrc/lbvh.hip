@@ -651,7 +651,8 @@ __device__ __forceinline__ float row_word(unsigned w) {
 // FOREST: scenes' trees under their own roots (scene id in sortedVel.w); a packet whose lanes start at
 // different roots walks them one root after another, the lanes of the current root taking part.
 template <bool EMIT, bool FOREST>
-__global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *__restrict__ packed,
+__global__ __attribute__((amdgpu_waves_per_eu(8))) void k_dem_lbvh_packets(const int *__restrict__ sIdx,
+                                                                           const float4 *__restrict__ packed,
                                    const float4 *__restrict__ cms, const float4 *__restrict__ sVel,
                                    const int *__restrict__ sceneRoot, int n,
                                    DevPhys ph, SceneView sv, DemOut o, int *overflow) {
@@ -747,8 +748,12 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
         node = live ? next : node;
         sp = live ? sp + dsp : sp;
         wn = node >= 0 ? words[16 * (size_t)(node - n) + pl] : 0u;
-        if (L < n && oL) {
-            const float4 CL = cms[L];
+        // both leaf children's centres requested before either is tested (one round trip, not two)
+        const bool hL = L < n && oL, hR = R < n && oR;
+        float4 CL, CR;
+        if (hL) CL = cms[L];
+        if (hR) CR = cms[R];
+        if (hL) {
             if (candidate(CL, pa, ra, false, ph)) {
                 if (EMIT) {
                     if (in_contact(pa, xyz(CL), ph)) {
@@ -761,8 +766,7 @@ __global__ void k_dem_lbvh_packets(const int *__restrict__ sIdx, const float4 *_
                 }
             }
         }
-        if (R < n && oR) {
-            const float4 CR = cms[R];
+        if (hR) {
             if (candidate(CR, pa, ra, false, ph)) {
                 if (EMIT) {
                     if (in_contact(pa, xyz(CR), ph)) {
