@@ -29,6 +29,9 @@ struct SceneView {
     const SceneGeom *table;
     int count;
     SceneGeom single;
+    // several scenes: scene s's particles are the sorted positions [sceneStart[s], sceneStart[s + 1])
+    // (NULL where the caller has no such table)
+    const int *sceneStart;
 };
 
 struct DevPhys {

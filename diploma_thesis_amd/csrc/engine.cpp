@@ -188,6 +188,7 @@ SceneView Engine::scene_view() const {
     sv.count = S;
     sv.table = geomD;
     if (S == 1) sv.single = geoms[0];
+    else sv.sceneStart = sceneStartD;
     return sv;
 }
 

@@ -296,7 +296,18 @@ __global__ void k_dem_grid_seg(const float4 *__restrict__ sPos, const float4 *__
                                const unsigned *__restrict__ keys, const unsigned *__restrict__ sIdx,
                                const unsigned *__restrict__ segStart, int segBits, int n, DevPhys ph, SceneView sv,
                                DemOut o) {
-    const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    int lb;
+    if (GEOM == kGeomLane && sv.sceneStart) {
+        // one workgroup per boundary between scenes k - 1 and k: the block that holds it, if the boundary
+        // lies strictly inside a block and no earlier boundary lies strictly inside the same one
+        const int k = (int)blockIdx.x + 1, bs = (int)blockDim.x;
+        const int b = sv.sceneStart[k], pb = sv.sceneStart[k - 1];
+        lb = b / bs;
+        if (b % bs == 0 || b >= n) return;
+        if (k > 1 && pb / bs == lb && pb % bs != 0) return;
+    } else {
+        lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    }
     const int first = lb * (int)blockDim.x;
     SceneGeom g;
     if (GEOM == kGeomSingle) {
@@ -477,7 +488,12 @@ void launch_dem_grid_seg(const float4 *sortedPos, const float4 *sortedVel, const
             run(single);
         } else {
             run(perBlock);
-            run(perLane);
+            // the blocks that straddle two scenes: with the scenes' sorted ranges known, one workgroup
+            // per boundary instead of a full grid whose other blocks return at once
+            if (!sv.sceneStart) run(perLane);
+            else if (sv.count > 1)
+                hipLaunchKernelGGL(perLane, dim3((unsigned)(sv.count - 1)), dim3(block), 0, st, sortedPos, sortedVel,
+                                   sortedKeys, sortedIdx, segStart, segBits, n, ph, sv, o);
         }
     };
     if (o.pairs) {
