@@ -49,14 +49,38 @@ __global__ void k_body_integrate(BodyBufs b, BodyBufs o, int B, DevPhys ph, Zero
 // calcHashD (particleKernels.cu:170-189) or generateMortonCodes (BVHkernels.cu:670-716).
 // Where pose_key_one takes the scene geometry from: each particle's scene (per lane), the table entry
 // of scene `scene` for all of the caller's particles (scalar loads), or the one scene passed by value.
-// A caller that has loaded the particle's rbIndex (rbKnown) or, with kPoseGeomScene, the scene's
-// geometry (gKnown) already passes them; null pointers have them loaded here.
+// The loads a particle's pose starts from: a body particle's cumulative quaternion, body frame,
+// body centre, velocity and angular velocity (a .. e), or a grain's position and velocity (a, b).
+struct PoseIn {
+    float4 a, b, c, d, e;
+};
+__device__ __forceinline__ PoseIn pose_load(int i, int rbi, const PartBufs &p, const BodyBufs &b, int doBodies,
+                                            int doGrains) {
+    PoseIn in;
+    if (rbi >= 0) {
+        if (doBodies) {
+            in.a = b.qcum[rbi];
+            in.b = p.bodyFrame[i];
+            in.c = b.pos[rbi];
+            in.d = b.vel[rbi];
+            in.e = b.ang[rbi];
+        } else {
+            in.a = p.pos[i];
+        }
+    } else {
+        in.a = p.pos[i];
+        if (doGrains) in.b = p.vel[i];
+    }
+    return in;
+}
+// Every load of a pose precedes its first store (pose_load, then pose_key_in): the compiler may not
+// move a load above a store to a buffer it cannot tell apart, so a load after a store would cost a
+// round trip of its own.
 enum { kPoseGeomLane = 0, kPoseGeomScene = 1, kPoseGeomSingle = 2 };
 template <int MODE, int GSRC = kPoseGeomLane>
 __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
                                                  KeyOut ko, int doBodies, int doGrains, unsigned *shist,
-                                                 int scene = 0, const int *rbKnown = nullptr,
-                                                 const SceneGeom *gKnown = nullptr);
+                                                 int scene = 0);
 
 // Adds 1 to an LDS histogram bin for every active lane.  Spatially ordered keys share digits across
 // a wave, so lanes are grouped by digit (8 ballots find the lanes with the same 8-bit digit) and
@@ -98,27 +122,16 @@ __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N,
 // Returns the key (0 without one); a grid key is stored only when ko.keys is set.  GSRC: where the
 // scene geometry comes from (kPoseGeomScene: the caller's particles all belong to scene `scene`, read
 // from the scene table, which holds every scene of an engine, also a single one).
-template <int MODE, int GSRC>
-__device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
-                                                 KeyOut ko, int doBodies, int doGrains, unsigned *shist, int scene,
-                                                 const int *rbKnown, const SceneGeom *gKnown) {
-    if (i >= N) return 0;
-    const int rbi = rbKnown ? *rbKnown : (p.rb ? p.rb[i] : -1);
-    const int s = GSRC == kPoseGeomScene ? scene : GSRC == kPoseGeomSingle ? 0 : (p.scene ? (int)p.scene[i] : 0);
-    SceneGeom g;
-    if (GSRC == kPoseGeomScene) g = gKnown ? *gKnown : sv.table[scene];
-    else if (GSRC == kPoseGeomSingle) g = sv.single;
-    else load_geom(sv, s, g);
+// The pose and key of particle i (rbIndex rbi, scene s with geometry g) from its loaded inputs.
+template <int MODE>
+__device__ __forceinline__ uint64_t pose_key_in(int i, int rbi, int s, const SceneGeom &g, const PoseIn &in,
+                                                PartBufs p, DevPhys ph, KeyOut ko, int doBodies, int doGrains,
+                                                unsigned *shist) {
     float4 P, Vfin;
     bool haveV = false;
     if (rbi >= 0) {
         if (doBodies) {
-            // every load before the first store: the compiler may not move a load above a store to
-            // a buffer it cannot tell apart, so loads after a store would each cost a round trip
-            const float4 Qc = b.qcum[rbi];
-            const float4 frame = p.bodyFrame[i];
-            const float4 CM = b.pos[rbi];
-            const float4 V = b.vel[rbi], W = b.ang[rbi];
+            const float4 Qc = in.a, frame = in.b, CM = in.c, V = in.d, W = in.e;
             float R[9];
             mat3_cast(qf{Qc.x, Qc.y, Qc.z, Qc.w}, R);
             f3 r = mat_vec(R, xyz(frame));
@@ -129,12 +142,12 @@ __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, 
             haveV = true;
             p.vel[i] = Vfin;
         } else {
-            P = p.pos[i];
+            P = in.a;
         }
     } else {
-        P = p.pos[i];
+        P = in.a;
         if (doGrains) {
-            float4 Vv = p.vel[i];
+            const float4 Vv = in.b;
             f3 x = xyz(P), v = xyz(Vv);
             v = add3(v, mk3(ph.gx * ph.dt, ph.gy * ph.dt, ph.gz * ph.dt));
             v = mk3(v.x * ph.globalDamping, v.y * ph.globalDamping, v.z * ph.globalDamping);
@@ -203,6 +216,20 @@ __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, 
     return key;
 }
 
+template <int MODE, int GSRC>
+__device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
+                                                 KeyOut ko, int doBodies, int doGrains, unsigned *shist, int scene) {
+    if (i >= N) return 0;
+    const int rbi = p.rb ? p.rb[i] : -1;
+    const PoseIn in = pose_load(i, rbi, p, b, doBodies, doGrains);
+    const int s = GSRC == kPoseGeomScene ? scene : GSRC == kPoseGeomSingle ? 0 : (p.scene ? (int)p.scene[i] : 0);
+    SceneGeom g;
+    if (GSRC == kPoseGeomScene) g = sv.table[scene];
+    else if (GSRC == kPoseGeomSingle) g = sv.single;
+    else load_geom(sv, s, g);
+    return pose_key_in<MODE>(i, rbi, s, g, in, p, ph, ko, doBodies, doGrains, shist);
+}
+
 // Pose and grid key of one tile of up to THREADS * 4 consecutive particles of one scene, the tile then
 // ranked by the key's bucket (its scene's cell range cut into 2^shift-cell ranges) in LDS and written
 // in bucket order (stable: particle order within a bucket), with the tile's per-bucket count and first
@@ -260,16 +287,24 @@ __global__ __launch_bounds__(THREADS) void k_pose_key_tiles(PartBufs p, BodyBufs
     // the previous item's stores, waits for those stores as well (vector memory counter 0, the rest
     // of the counters left at their maxima in the gfx9 encoding)
     __builtin_amdgcn_s_waitcnt(0x0f70);
+    // items in pairs: both items' loads are issued before either item's stores, so a pair costs one
+    // round trip (a load after a store also waits for the store)
     unsigned keys[kTileItems];
+    static_assert(kTileItems % 2 == 0, "items in pairs");
 #pragma unroll
-    for (int k = 0; k < kTileItems; ++k) {
-        const long long i = waveBase + k * 64 + lane;
-        const uint64_t key =
-            i < tileEnd ? pose_key_one<MODE, kPoseGeomScene>((int)i, p, b, N, ph, sv, ko, 1, 1, nullptr, scene, &rbv[k], &geom)
-                        : 0ull;
+    for (int k = 0; k < kTileItems; k += 2) {
+        const long long iA = waveBase + k * 64 + lane, iB = iA + 64;
+        PoseIn inA{}, inB{};
+        if (iA < tileEnd) inA = pose_load((int)iA, rbv[k], p, b, 1, 1);
+        if (iB < tileEnd) inB = pose_load((int)iB, rbv[k + 1], p, b, 1, 1);
+        const uint64_t keyA =
+            iA < tileEnd ? pose_key_in<MODE>((int)iA, rbv[k], scene, geom, inA, p, ph, ko, 1, 1, nullptr) : 0ull;
+        const uint64_t keyB =
+            iB < tileEnd ? pose_key_in<MODE>((int)iB, rbv[k + 1], scene, geom, inB, p, ph, ko, 1, 1, nullptr) : 0ull;
         // a 63-bit code's top 30 bits are the 30-bit code of the same position (both scale the
         // normalised coordinate by a power of two and truncate)
-        keys[k] = MODE == kKeyMorton63 ? (unsigned)(key >> 33) : (unsigned)key;
+        keys[k] = MODE == kKeyMorton63 ? (unsigned)(keyA >> 33) : (unsigned)keyA;
+        keys[k + 1] = MODE == kKeyMorton63 ? (unsigned)(keyB >> 33) : (unsigned)keyB;
     }
     __syncthreads();
     // ranks within the wave, in particle order
