@@ -81,6 +81,10 @@ template <int MODE, int GSRC = kPoseGeomLane>
 __device__ __forceinline__ uint64_t pose_key_one(int i, PartBufs p, BodyBufs b, int N, DevPhys ph, SceneView sv,
                                                  KeyOut ko, int doBodies, int doGrains, unsigned *shist,
                                                  int scene = 0);
+template <int MODE>
+__device__ __forceinline__ uint64_t pose_key_in(int i, int rbi, int s, const SceneGeom &g, const PoseIn &in,
+                                                PartBufs p, DevPhys ph, KeyOut ko, int doBodies, int doGrains,
+                                                unsigned *shist);
 
 // Adds 1 to an LDS histogram bin for every active lane.  Spatially ordered keys share digits across
 // a wave, so lanes are grouped by digit (8 ballots find the lanes with the same 8-bit digit) and
@@ -110,8 +114,25 @@ __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N,
     // a block handles several 256-particle chunks (grid-stride), so that fewer block histograms
     // are flushed into the global ones: the flush is a global atomic per non-empty bin
     const int stride = (int)(gridDim.x * blockDim.x);
-    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < N; i += stride)
-        pose_key_one<MODE, GSRC>(i, p, b, N, ph, sv, ko, doBodies, doGrains, hist ? shist : nullptr);
+    if constexpr (GSRC == kPoseGeomSingle) {
+        // particles in pairs (i, i + stride): both particles' loads are issued before either's stores, so
+        // a pair costs the round trips of one particle (a load after a store also waits for the store)
+        for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < N; i += 2 * stride) {
+            const int j = i + stride;
+            const bool hasB = j < N;
+            const int rbA = p.rb ? p.rb[i] : -1;
+            const int rbB = hasB && p.rb ? p.rb[j] : -1;
+            const PoseIn inA = pose_load(i, rbA, p, b, doBodies, doGrains);
+            PoseIn inB{};
+            if (hasB) inB = pose_load(j, rbB, p, b, doBodies, doGrains);
+            pose_key_in<MODE>(i, rbA, 0, sv.single, inA, p, ph, ko, doBodies, doGrains, hist ? shist : nullptr);
+            if (hasB)
+                pose_key_in<MODE>(j, rbB, 0, sv.single, inB, p, ph, ko, doBodies, doGrains, hist ? shist : nullptr);
+        }
+    } else {
+        for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < N; i += stride)
+            pose_key_one<MODE, GSRC>(i, p, b, N, ph, sv, ko, doBodies, doGrains, hist ? shist : nullptr);
+    }
     if (hist) {
         __syncthreads();
         for (int k = threadIdx.x; k < ko.histPasses * 256; k += blockDim.x)
