@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N,
     // a block handles several 256-particle chunks (grid-stride), so that fewer block histograms
     // are flushed into the global ones: the flush is a global atomic per non-empty bin
     const int stride = (int)(gridDim.x * blockDim.x);
-    if constexpr (GSRC == kPoseGeomSingle) {
+    if constexpr (GSRC == kPoseGeomSingle || GSRC == kPoseGeomLane) {
         // particles in pairs (i, i + stride): both particles' loads are issued before either's stores, so
         // a pair costs the round trips of one particle (a load after a store also waits for the store)
         for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < N; i += 2 * stride) {
@@ -122,12 +122,21 @@ __global__ __launch_bounds__(256) void k_pose_key(PartBufs p, BodyBufs b, int N,
             const bool hasB = j < N;
             const int rbA = p.rb ? p.rb[i] : -1;
             const int rbB = hasB && p.rb ? p.rb[j] : -1;
+            const int sA = GSRC == kPoseGeomLane && p.scene ? (int)p.scene[i] : 0;
+            const int sB = GSRC == kPoseGeomLane && p.scene && hasB ? (int)p.scene[j] : 0;
+            SceneGeom gA, gB;
+            if (GSRC == kPoseGeomSingle) {
+                gA = sv.single;
+                gB = sv.single;
+            } else {
+                load_geom(sv, sA, gA);
+                load_geom(sv, sB, gB);
+            }
             const PoseIn inA = pose_load(i, rbA, p, b, doBodies, doGrains);
             PoseIn inB{};
             if (hasB) inB = pose_load(j, rbB, p, b, doBodies, doGrains);
-            pose_key_in<MODE>(i, rbA, 0, sv.single, inA, p, ph, ko, doBodies, doGrains, hist ? shist : nullptr);
-            if (hasB)
-                pose_key_in<MODE>(j, rbB, 0, sv.single, inB, p, ph, ko, doBodies, doGrains, hist ? shist : nullptr);
+            pose_key_in<MODE>(i, rbA, sA, gA, inA, p, ph, ko, doBodies, doGrains, hist ? shist : nullptr);
+            if (hasB) pose_key_in<MODE>(j, rbB, sB, gB, inB, p, ph, ko, doBodies, doGrains, hist ? shist : nullptr);
         }
     } else {
         for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < N; i += stride)
