@@ -311,14 +311,33 @@ __global__ __launch_bounds__(kBkThreads) __attribute__((amdgpu_waves_per_eu(8)))
                 sVel[start + ib] = qb;
             }
         }
-        for (int i = ib + kBkThreads; i < i1; i += kBkThreads) {
-            const unsigned w = sorted[i];
-            const unsigned val = valAt[w & kPosMask];
+        // further rounds in pairs: both positions' gathers are issued before either's stores (a load
+        // after a store also waits for the store); a position past the end reloads the first one
+        for (int i = ib + kBkThreads; i < i1; i += 2 * kBkThreads) {
+            const int i2 = i + kBkThreads;
+            const bool h2 = i2 < i1;
+            const unsigned w = sorted[i], w2 = sorted[h2 ? i2 : i];
+            const unsigned val = valAt[w & kPosMask], val2 = valAt[w2 & kPosMask];
+            float4 P, Q, P2, Q2;
+            if (REORDER) {
+                P = rec[2 * (size_t)val];
+                Q = rec[2 * (size_t)val + 1];
+                P2 = rec[2 * (size_t)val2];
+                Q2 = rec[2 * (size_t)val2 + 1];
+            }
             kOut[start + i] = base + (w >> POSBITS);
             vOut[start + i] = val;
             if (REORDER) {
-                sPos[start + i] = rec[2 * (size_t)val];
-                sVel[start + i] = rec[2 * (size_t)val + 1];
+                sPos[start + i] = P;
+                sVel[start + i] = Q;
+            }
+            if (h2) {
+                kOut[start + i2] = base + (w2 >> POSBITS);
+                vOut[start + i2] = val2;
+                if (REORDER) {
+                    sPos[start + i2] = P2;
+                    sVel[start + i2] = Q2;
+                }
             }
         }
     } else {
